@@ -1,0 +1,88 @@
+"""Build recipe for the native libraries (run on any host with ROCm; no GPU needed).
+
+    python -m afquantumsim_amd.build        # or __graft_entry__.build()
+
+Produces, in-tree:
+  afquantumsim_amd/libaqs_hip.so   HIP kernels for gfx950 + the C ABI (include/aqs_hip.h)
+  afquantumsim_amd/libaqs.so       the aqs:: C++ API over that C ABI (include/aqs/quantum.h)
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+CPPSRC = os.path.join(PKG, "cpp")
+INCLUDE = os.path.join(ROOT, "include")
+BUILD = os.path.join(PKG, "_build")
+HIP_LIB = os.path.join(PKG, "libaqs_hip.so")
+AQS_LIB = os.path.join(PKG, "libaqs.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+# Packed FP32 instructions stay off in every device pass.
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-Wall",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I", INCLUDE, "-I", CSRC,
+]
+
+HIP_SOURCES = ["state.hip", "gates.hip", "mfma.hip", "measure.hip", "batch.hip", "fusion.cpp"]
+CPP_SOURCES = ["quantum.cpp", "quantum_algo.cpp"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + r.stdout + r.stderr)
+        raise RuntimeError(f"build step failed: {cmd[-1]}")
+    return r
+
+
+def _obj(src):
+    return os.path.join(BUILD, os.path.basename(src) + ".o")
+
+
+def build_hip(jobs: int = 8) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+
+    def compile_one(src):
+        lang = ["-x", "hip"] if src.endswith(".cpp") else []
+        _run([HIPCC, *HIP_FLAGS, *lang, "-c", src, "-o", _obj(src)])
+
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(compile_one, srcs))
+    tmp = HIP_LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *[_obj(s) for s in srcs]])
+    os.replace(tmp, HIP_LIB)
+    return HIP_LIB
+
+
+def build_cpp() -> str:
+    srcs = [os.path.join(CPPSRC, s) for s in CPP_SOURCES if os.path.exists(os.path.join(CPPSRC, s))]
+    if not srcs:
+        return ""
+    tmp = AQS_LIB + ".tmp"
+    _run(["g++", "-O2", "-fPIC", "-std=c++17", "-Wall", "-shared", "-I", INCLUDE, "-o", tmp, *srcs,
+          "-L", PKG, "-laqs_hip", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, AQS_LIB)
+    return AQS_LIB
+
+
+def build_all() -> None:
+    if not os.path.exists(HIPCC) and not shutil.which("hipcc"):
+        raise RuntimeError("hipcc not found")
+    build_hip()
+    build_cpp()
+
+
+if __name__ == "__main__":
+    build_all()
+    print("built", HIP_LIB, AQS_LIB)

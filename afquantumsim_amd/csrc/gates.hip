@@ -1,0 +1,764 @@
+// Gate-application kernels for gfx950.
+//
+// The reference applies every gate as a 2^n x 2^n operator: a CSR SpMV for
+// the permutation / diagonal gates (e.g. X src/quantum.cpp:540-562, CZ
+// :1115-1152) and a dense Kronecker-product GEMV for H / RotX / RotY / RotZ
+// (:671-831).  Here each gate is one in-place pass over the amplitudes it
+// can change:
+//
+//   MAT1   2x2 butterfly on (i0, i1 = i0 | target bit)        f = 1 (1/2 per control)
+//   DIAG1  per-amplitude phase chosen by the target bit        f = 1
+//   PHASE  phase on amplitudes whose target+control bits are 1 f = 1/2^(#bits)
+//   X      exchange of the two halves of each pair             f = 1 (Or: 3/4)
+//   SWAP   exchange of |..1..0..> and |..0..1..>               f = 1/2
+//   MATK   dense 2^k x 2^k on k target bits (k <= 3 here; k = 4, 5 complex64
+//          go to the MFMA kernel in mfma.hip)
+//
+// Complex64 states move two amplitudes per 16-byte access (float4) whenever
+// neither the target nor a control is bit 0, so every wavefront touches
+// 64 x 16 B contiguous runs; bit-0 targets pair the two halves of one float4
+// in registers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "internal.h"
+
+using namespace aqs_hip;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kUnroll = 4;  // independent units per thread (memory-level parallelism)
+
+// Sorted bit positions that are held fixed while enumerating work units.
+struct Fixed {
+    int n;
+    int pos[8];
+};
+
+__device__ __forceinline__ uint64_t insert_zeros(uint64_t t, const Fixed& f) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        if (j >= f.n) break;
+        const uint64_t p = (uint64_t)f.pos[j];
+        t = ((t >> p) << (p + 1)) | (t & ((1ull << p) - 1ull));
+    }
+    return t;
+}
+
+inline unsigned blocks_for(uint64_t units, int per_thread) {
+    uint64_t threads = (units + per_thread - 1) / per_thread;
+    uint64_t b = (threads + kBlock - 1) / kBlock;
+    return (unsigned)std::max<uint64_t>(b, 1);
+}
+
+template <typename R>
+struct C2 {
+    R re, im;
+};
+
+template <typename R>
+__device__ __forceinline__ void cmul_add(R ar, R ai, R br, R bi, R& accr, R& acci) {
+    accr += ar * br - ai * bi;
+    acci += ar * bi + ai * br;
+}
+
+// ---------------------------------------------------------------------------
+// MAT1, complex64, float4 path (target bit >= 1, control bits >= 1).
+// Work unit u -> float4 index f (2 amplitudes) with bit (t-1) = 0 and the
+// control bits (c-1) = 1; partner float4 at f + 2^(t-1).
+struct Mat1V4Args {
+    float4* sv;
+    uint64_t units;
+    Fixed fixed;      // positions in float4-index space
+    uint64_t cmask;   // control mask in float4-index space
+    uint64_t stride;  // 2^(t-1)
+    float m[8];
+};
+
+__global__ __launch_bounds__(kBlock) void k_mat1_v4(Mat1V4Args a) {
+    const uint64_t base = (uint64_t)blockIdx.x * (kBlock * kUnroll) + threadIdx.x;
+    float4 lo[kUnroll], hi[kUnroll];
+    uint64_t idx[kUnroll];
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) {
+        const uint64_t u = base + (uint64_t)j * kBlock;
+        idx[j] = (u < a.units) ? (insert_zeros(u, a.fixed) | a.cmask) : ~0ull;
+        if (idx[j] != ~0ull) {
+            lo[j] = ld_nt(&a.sv[idx[j]]);
+            hi[j] = ld_nt(&a.sv[idx[j] + a.stride]);
+        }
+    }
+    const float m0r = a.m[0], m0i = a.m[1], m1r = a.m[2], m1i = a.m[3];
+    const float m2r = a.m[4], m2i = a.m[5], m3r = a.m[6], m3i = a.m[7];
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) {
+        if (idx[j] == ~0ull) continue;
+        float4 x = lo[j], y = hi[j], nx, ny;
+        // amplitude pair (x.xy, y.xy)
+        nx.x = m0r * x.x - m0i * x.y + m1r * y.x - m1i * y.y;
+        nx.y = m0r * x.y + m0i * x.x + m1r * y.y + m1i * y.x;
+        ny.x = m2r * x.x - m2i * x.y + m3r * y.x - m3i * y.y;
+        ny.y = m2r * x.y + m2i * x.x + m3r * y.y + m3i * y.x;
+        // amplitude pair (x.zw, y.zw)
+        nx.z = m0r * x.z - m0i * x.w + m1r * y.z - m1i * y.w;
+        nx.w = m0r * x.w + m0i * x.z + m1r * y.w + m1i * y.z;
+        ny.z = m2r * x.z - m2i * x.w + m3r * y.z - m3i * y.w;
+        ny.w = m2r * x.w + m2i * x.z + m3r * y.w + m3i * y.z;
+        st_nt(nx, &a.sv[idx[j]]);
+        st_nt(ny, &a.sv[idx[j] + a.stride]);
+    }
+}
+
+// MAT1, complex64, target bit 0 (controls >= 1): the pair is one float4.
+__global__ __launch_bounds__(kBlock) void k_mat1_b0(Mat1V4Args a) {
+    const uint64_t base = (uint64_t)blockIdx.x * (kBlock * kUnroll) + threadIdx.x;
+    float4 v[kUnroll];
+    uint64_t idx[kUnroll];
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) {
+        const uint64_t u = base + (uint64_t)j * kBlock;
+        idx[j] = (u < a.units) ? (insert_zeros(u, a.fixed) | a.cmask) : ~0ull;
+        if (idx[j] != ~0ull) v[j] = ld_nt(&a.sv[idx[j]]);
+    }
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) {
+        if (idx[j] == ~0ull) continue;
+        float4 x = v[j], n;
+        n.x = a.m[0] * x.x - a.m[1] * x.y + a.m[2] * x.z - a.m[3] * x.w;
+        n.y = a.m[0] * x.y + a.m[1] * x.x + a.m[2] * x.w + a.m[3] * x.z;
+        n.z = a.m[4] * x.x - a.m[5] * x.y + a.m[6] * x.z - a.m[7] * x.w;
+        n.w = a.m[4] * x.y + a.m[5] * x.x + a.m[6] * x.w + a.m[7] * x.z;
+        st_nt(n, &a.sv[idx[j]]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Generic per-amplitude-pair kernels (any bit positions, both precisions).
+template <typename T2, typename R>
+struct PairArgs {
+    T2* sv;
+    uint64_t units;
+    Fixed fixed;      // amplitude-index space
+    uint64_t cmask;   // bits forced to 1
+    uint64_t tmask;   // partner offset (target bit, or pa|pb for SWAP)
+    uint64_t anymask; // AQS_CTRL_ANY: at least one of these bits must be 1 (0 = unused)
+    R m[8];
+};
+
+// op: 0 = MAT1, 1 = X (exchange), 2 = SWAP (exchange), 3 = DIAG1 (per-bit phase)
+template <typename T2, typename R, int OP>
+__global__ __launch_bounds__(kBlock) void k_pair(PairArgs<T2, R> a) {
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t u = (uint64_t)blockIdx.x * kBlock + threadIdx.x; u < a.units; u += stride) {
+        const uint64_t i0 = insert_zeros(u, a.fixed) | a.cmask;
+        if (a.anymask && !(i0 & a.anymask)) continue;
+        const uint64_t i1 = i0 ^ a.tmask;
+        T2 x = a.sv[i0], y = a.sv[i1];
+        if (OP == 0) {
+            T2 nx, ny;
+            nx.x = a.m[0] * x.x - a.m[1] * x.y + a.m[2] * y.x - a.m[3] * y.y;
+            nx.y = a.m[0] * x.y + a.m[1] * x.x + a.m[2] * y.y + a.m[3] * y.x;
+            ny.x = a.m[4] * x.x - a.m[5] * x.y + a.m[6] * y.x - a.m[7] * y.y;
+            ny.y = a.m[4] * x.y + a.m[5] * x.x + a.m[6] * y.y + a.m[7] * y.x;
+            a.sv[i0] = nx;
+            a.sv[i1] = ny;
+        } else if (OP == 1 || OP == 2) {
+            a.sv[i0] = y;
+            a.sv[i1] = x;
+        } else {
+            T2 nx, ny;
+            nx.x = a.m[0] * x.x - a.m[1] * x.y;
+            nx.y = a.m[0] * x.y + a.m[1] * x.x;
+            ny.x = a.m[6] * y.x - a.m[7] * y.y;
+            ny.y = a.m[6] * y.y + a.m[7] * y.x;
+            a.sv[i0] = nx;
+            a.sv[i1] = ny;
+        }
+    }
+}
+
+// Exchange kernels, complex64 float4 path (all fixed bits >= 1).
+struct XchgV4Args {
+    float4* sv;
+    uint64_t units;
+    Fixed fixed;
+    uint64_t setmask;  // bits forced to 1 (controls, SWAP's bit a)
+    uint64_t partner;  // xor to the partner float4
+};
+
+__global__ __launch_bounds__(kBlock) void k_xchg_v4(XchgV4Args a) {
+    const uint64_t base = (uint64_t)blockIdx.x * (kBlock * kUnroll) + threadIdx.x;
+    float4 lo[kUnroll], hi[kUnroll];
+    uint64_t idx[kUnroll];
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) {
+        const uint64_t u = base + (uint64_t)j * kBlock;
+        idx[j] = (u < a.units) ? (insert_zeros(u, a.fixed) | a.setmask) : ~0ull;
+        if (idx[j] != ~0ull) {
+            lo[j] = ld_nt(&a.sv[idx[j]]);
+            hi[j] = ld_nt(&a.sv[idx[j] ^ a.partner]);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) {
+        if (idx[j] == ~0ull) continue;
+        st_nt(hi[j], &a.sv[idx[j]]);
+        st_nt(lo[j], &a.sv[idx[j] ^ a.partner]);
+    }
+}
+
+// X on bit 0 (complex64): swap the halves of each float4 (controls >= 1).
+__global__ __launch_bounds__(kBlock) void k_x_b0(XchgV4Args a) {
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t u = (uint64_t)blockIdx.x * kBlock + threadIdx.x; u < a.units; u += stride) {
+        const uint64_t f = insert_zeros(u, a.fixed) | a.setmask;
+        float4 v = a.sv[f];
+        a.sv[f] = make_float4(v.z, v.w, v.x, v.y);
+    }
+}
+
+// Phase on amplitudes with all fixed bits set.
+template <typename T2, typename R>
+struct PhaseArgs {
+    T2* sv;
+    uint64_t units;
+    Fixed fixed;
+    uint64_t setmask;
+    R pr, pi;
+};
+
+template <typename T2, typename R>
+__global__ __launch_bounds__(kBlock) void k_phase(PhaseArgs<T2, R> a) {
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t u = (uint64_t)blockIdx.x * kBlock + threadIdx.x; u < a.units; u += stride) {
+        const uint64_t i = insert_zeros(u, a.fixed) | a.setmask;
+        T2 x = a.sv[i], y;
+        y.x = a.pr * x.x - a.pi * x.y;
+        y.y = a.pr * x.y + a.pi * x.x;
+        a.sv[i] = y;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_phase_v4(PhaseArgs<float4, float> a) {
+    const uint64_t base = (uint64_t)blockIdx.x * (kBlock * kUnroll) + threadIdx.x;
+    float4 v[kUnroll];
+    uint64_t idx[kUnroll];
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) {
+        const uint64_t u = base + (uint64_t)j * kBlock;
+        idx[j] = (u < a.units) ? (insert_zeros(u, a.fixed) | a.setmask) : ~0ull;
+        if (idx[j] != ~0ull) v[j] = ld_nt(&a.sv[idx[j]]);
+    }
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) {
+        if (idx[j] == ~0ull) continue;
+        float4 x = v[j], y;
+        y.x = a.pr * x.x - a.pi * x.y;
+        y.y = a.pr * x.y + a.pi * x.x;
+        y.z = a.pr * x.z - a.pi * x.w;
+        y.w = a.pr * x.w + a.pi * x.z;
+        st_nt(y, &a.sv[idx[j]]);
+    }
+}
+
+// DIAG1 complex64 float4 path (controls >= 1; any target bit).
+struct Diag1V4Args {
+    float4* sv;
+    uint64_t units;
+    Fixed fixed;
+    uint64_t cmask;
+    int tbit;  // target bit in amplitude space
+    float d0r, d0i, d1r, d1i;
+};
+
+__global__ __launch_bounds__(kBlock) void k_diag1_v4(Diag1V4Args a) {
+    const uint64_t base = (uint64_t)blockIdx.x * (kBlock * kUnroll) + threadIdx.x;
+    float4 v[kUnroll];
+    uint64_t idx[kUnroll];
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) {
+        const uint64_t u = base + (uint64_t)j * kBlock;
+        idx[j] = (u < a.units) ? (insert_zeros(u, a.fixed) | a.cmask) : ~0ull;
+        if (idx[j] != ~0ull) v[j] = ld_nt(&a.sv[idx[j]]);
+    }
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) {
+        if (idx[j] == ~0ull) continue;
+        float ar, ai, br, bi;
+        if (a.tbit == 0) {
+            ar = a.d0r; ai = a.d0i; br = a.d1r; bi = a.d1i;
+        } else {
+            const bool one = (idx[j] >> (a.tbit - 1)) & 1ull;
+            ar = one ? a.d1r : a.d0r;
+            ai = one ? a.d1i : a.d0i;
+            br = ar;
+            bi = ai;
+        }
+        float4 x = v[j], y;
+        y.x = ar * x.x - ai * x.y;
+        y.y = ar * x.y + ai * x.x;
+        y.z = br * x.z - bi * x.w;
+        y.w = br * x.w + bi * x.z;
+        st_nt(y, &a.sv[idx[j]]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// MATK scalar kernel: one thread per group of K = 2^k amplitudes.
+template <typename T2, typename R, int K>
+struct MatkArgs {
+    T2* sv;
+    uint64_t units;
+    Fixed fixed;
+    uint64_t cmask;
+    uint64_t off[K];   // amplitude offset of sub-index r
+    const R* u;        // K x K complex, row-major, interleaved
+};
+
+template <typename T2, typename R, int K>
+__global__ __launch_bounds__(kBlock) void k_matk(MatkArgs<T2, R, K> a) {
+    __shared__ R su[2 * K * K];
+    for (int i = threadIdx.x; i < 2 * K * K; i += kBlock) su[i] = a.u[i];
+    __syncthreads();
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x; t < a.units; t += stride) {
+        const uint64_t base = insert_zeros(t, a.fixed) | a.cmask;
+        R xr[K], xi[K];
+#pragma unroll
+        for (int r = 0; r < K; ++r) {
+            T2 v = a.sv[base + a.off[r]];
+            xr[r] = v.x;
+            xi[r] = v.y;
+        }
+#pragma unroll
+        for (int r = 0; r < K; ++r) {
+            R accr = 0, acci = 0;
+#pragma unroll
+            for (int c = 0; c < K; ++c) cmul_add(su[2 * (r * K + c)], su[2 * (r * K + c) + 1], xr[c], xi[c], accr, acci);
+            T2 o;
+            o.x = accr;
+            o.y = acci;
+            a.sv[base + a.off[r]] = o;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+
+Fixed make_fixed(std::vector<int> pos) {
+    std::sort(pos.begin(), pos.end());
+    Fixed f{};
+    f.n = (int)pos.size();
+    for (int i = 0; i < f.n; ++i) f.pos[i] = pos[i];
+    return f;
+}
+
+unsigned grid_stride_blocks(uint64_t units) {
+    uint64_t b = (units + kBlock - 1) / kBlock;
+    if (b > 65536) b = 65536;
+    return (unsigned)std::max<uint64_t>(b, 1);
+}
+
+template <typename T2, typename R>
+void launch_pair(aqs_state s, int op_code, const aqs_op& op, std::vector<int> fixed_pos, uint64_t cmask,
+                 uint64_t tmask, uint64_t anymask, uint64_t units) {
+    PairArgs<T2, R> a{};
+    a.sv = (T2*)s->amps;
+    a.units = units;
+    a.fixed = make_fixed(fixed_pos);
+    a.cmask = cmask;
+    a.tmask = tmask;
+    a.anymask = anymask;
+    for (int i = 0; i < 8; ++i) a.m[i] = (R)op.m[i];
+    dim3 g(grid_stride_blocks(units)), b(kBlock);
+    switch (op_code) {
+        case 0: hipLaunchKernelGGL((k_pair<T2, R, 0>), g, b, 0, s->stream, a); break;
+        case 1: hipLaunchKernelGGL((k_pair<T2, R, 1>), g, b, 0, s->stream, a); break;
+        case 2: hipLaunchKernelGGL((k_pair<T2, R, 2>), g, b, 0, s->stream, a); break;
+        default: hipLaunchKernelGGL((k_pair<T2, R, 3>), g, b, 0, s->stream, a); break;
+    }
+}
+
+template <typename T2, typename R, int K>
+void launch_matk(aqs_state s, const aqs_op& op, const void* dev_u) {
+    const int n = s->n, k = op.ntargets;
+    MatkArgs<T2, R, K> a{};
+    a.sv = (T2*)s->amps;
+    std::vector<int> fp;
+    for (int j = 0; j < k; ++j) fp.push_back(qpos(n, op.targets[j]));
+    uint64_t cmask = 0;
+    for (int j = 0; j < op.ncontrols; ++j) {
+        fp.push_back(qpos(n, op.controls[j]));
+        cmask |= qbit(n, op.controls[j]);
+    }
+    a.fixed = make_fixed(fp);
+    a.cmask = cmask;
+    a.units = s->N >> (k + op.ncontrols);
+    for (int r = 0; r < K; ++r) {
+        uint64_t o = 0;
+        for (int j = 0; j < k; ++j)
+            if ((r >> (k - 1 - j)) & 1) o |= qbit(n, op.targets[j]);
+        a.off[r] = o;
+    }
+    a.u = (const R*)dev_u;
+    hipLaunchKernelGGL((k_matk<T2, R, K>), dim3(grid_stride_blocks(a.units)), dim3(kBlock), 0, s->stream, a);
+}
+
+template <typename T2, typename R>
+void dispatch_matk(aqs_state s, const aqs_op& op, const void* dev_u) {
+    switch (op.ntargets) {
+        case 1: launch_matk<T2, R, 2>(s, op, dev_u); break;
+        case 2: launch_matk<T2, R, 4>(s, op, dev_u); break;
+        case 3: launch_matk<T2, R, 8>(s, op, dev_u); break;
+        case 4: launch_matk<T2, R, 16>(s, op, dev_u); break;
+        default: launch_matk<T2, R, 32>(s, op, dev_u); break;
+    }
+}
+
+bool all_ge1(const std::vector<int>& p) {
+    for (int x : p)
+        if (x < 1) return false;
+    return true;
+}
+
+void apply_c64_fast_or_generic(aqs_state s, const aqs_op& op);
+
+}  // namespace
+
+namespace aqs_hip {
+
+void validate_op(const aqs_op& op, int n) {
+    auto chk_q = [&](int q) {
+        if (q < 0 || q >= n) fail(AQS_ERR_OUT_OF_RANGE, "Cannot add gate at the given qubit position");
+    };
+    if (op.ntargets < 1 || op.ntargets > 5) fail(AQS_ERR_INVALID_ARGUMENT, "op target count must be 1..5");
+    if (op.ncontrols < 0 || op.ncontrols > 3) fail(AQS_ERR_INVALID_ARGUMENT, "op control count must be 0..3");
+    switch (op.kind) {
+        case AQS_OP_MAT1:
+        case AQS_OP_DIAG1:
+        case AQS_OP_X:
+            if (op.ntargets != 1) fail(AQS_ERR_INVALID_ARGUMENT, "one-qubit op needs exactly one target");
+            break;
+        case AQS_OP_SWAP:
+            if (op.ntargets != 2) fail(AQS_ERR_INVALID_ARGUMENT, "swap needs two targets");
+            break;
+        case AQS_OP_MATK:
+            if (!op.u) fail(AQS_ERR_INVALID_ARGUMENT, "MATK op without matrix");
+            break;
+        case AQS_OP_PHASE:
+            break;
+        default:
+            fail(AQS_ERR_INVALID_ARGUMENT, "unknown op kind " + std::to_string(op.kind));
+    }
+    uint64_t seen = 0;
+    for (int j = 0; j < op.ntargets; ++j) {
+        chk_q(op.targets[j]);
+        uint64_t b = 1ull << op.targets[j];
+        if (seen & b) {
+            if (op.kind == AQS_OP_SWAP) fail(AQS_ERR_INVALID_ARGUMENT, "Cannot use the swap gate on the same target qubits");
+            fail(AQS_ERR_INVALID_ARGUMENT, "repeated target qubit");
+        }
+        seen |= b;
+    }
+    uint64_t cseen = 0;
+    for (int j = 0; j < op.ncontrols; ++j) {
+        chk_q(op.controls[j]);
+        uint64_t b = 1ull << op.controls[j];
+        if (seen & b) fail(AQS_ERR_INVALID_ARGUMENT, "Control qubit cannot be the same as the target qubit");
+        if (cseen & b) fail(AQS_ERR_INVALID_ARGUMENT, "repeated control qubit");
+        cseen |= b;
+    }
+}
+
+}  // namespace aqs_hip
+
+namespace {
+
+// Collects fixed positions / masks for an op (controls de-duplicated: the
+// reference's CCNot accepts control A == control B, src/quantum.cpp:1524-1540).
+struct OpBits {
+    std::vector<int> cpos;
+    uint64_t cmask = 0;
+};
+
+OpBits control_bits(int n, const aqs_op& op) {
+    OpBits b;
+    for (int j = 0; j < op.ncontrols; ++j) {
+        uint64_t m = qbit(n, op.controls[j]);
+        if (b.cmask & m) continue;
+        b.cmask |= m;
+        b.cpos.push_back(qpos(n, op.controls[j]));
+    }
+    return b;
+}
+
+void apply_c64_fast_or_generic(aqs_state s, const aqs_op& op) {
+    const int n = s->n;
+    OpBits cb = control_bits(n, op);
+    const bool any = (op.flags & AQS_CTRL_ANY) && op.ncontrols > 0;
+    float mf[8];
+    for (int i = 0; i < 8; ++i) mf[i] = (float)op.m[i];
+
+    switch (op.kind) {
+        case AQS_OP_MAT1: {
+            ProfScope ps(s, AQS_OP_MAT1);
+            const int t = qpos(n, op.targets[0]);
+            if (!any && all_ge1(cb.cpos) && n >= 2) {
+                Mat1V4Args a{};
+                a.sv = (float4*)s->amps;
+                std::vector<int> fp;
+                for (int p : cb.cpos) fp.push_back(p - 1);
+                a.cmask = cb.cmask >> 1;
+                for (int i = 0; i < 8; ++i) a.m[i] = mf[i];
+                if (t >= 1) {
+                    fp.push_back(t - 1);
+                    a.stride = 1ull << (t - 1);
+                    a.units = s->N >> (2 + cb.cpos.size());
+                    a.fixed = make_fixed(fp);
+                    hipLaunchKernelGGL(k_mat1_v4, dim3(blocks_for(a.units, kUnroll)), dim3(kBlock), 0, s->stream, a);
+                } else {
+                    a.units = s->N >> (1 + cb.cpos.size());
+                    a.fixed = make_fixed(fp);
+                    hipLaunchKernelGGL(k_mat1_b0, dim3(blocks_for(a.units, kUnroll)), dim3(kBlock), 0, s->stream, a);
+                }
+            } else {
+                std::vector<int> fp = any ? std::vector<int>{} : cb.cpos;
+                fp.push_back(t);
+                uint64_t units = s->N >> fp.size();
+                launch_pair<float2, float>(s, 0, op, fp, any ? 0 : cb.cmask, 1ull << t, any ? cb.cmask : 0, units);
+            }
+            break;
+        }
+        case AQS_OP_DIAG1: {
+            ProfScope ps(s, AQS_OP_DIAG1);
+            const int t = qpos(n, op.targets[0]);
+            if (!any && all_ge1(cb.cpos) && n >= 2) {
+                Diag1V4Args a{};
+                a.sv = (float4*)s->amps;
+                std::vector<int> fp;
+                for (int p : cb.cpos) fp.push_back(p - 1);
+                a.fixed = make_fixed(fp);
+                a.cmask = cb.cmask >> 1;
+                a.units = s->N >> (1 + cb.cpos.size());
+                a.tbit = t;
+                a.d0r = mf[0]; a.d0i = mf[1]; a.d1r = mf[6]; a.d1i = mf[7];
+                hipLaunchKernelGGL(k_diag1_v4, dim3(blocks_for(a.units, kUnroll)), dim3(kBlock), 0, s->stream, a);
+            } else {
+                std::vector<int> fp = any ? std::vector<int>{} : cb.cpos;
+                fp.push_back(t);
+                launch_pair<float2, float>(s, 3, op, fp, any ? 0 : cb.cmask, 1ull << t, any ? cb.cmask : 0,
+                                           s->N >> fp.size());
+            }
+            break;
+        }
+        case AQS_OP_PHASE: {
+            ProfScope ps(s, AQS_OP_PHASE);
+            std::vector<int> fp = cb.cpos;
+            uint64_t mask = cb.cmask;
+            for (int j = 0; j < op.ntargets; ++j) {
+                uint64_t m = qbit(n, op.targets[j]);
+                if (mask & m) continue;
+                mask |= m;
+                fp.push_back(qpos(n, op.targets[j]));
+            }
+            if (all_ge1(fp) && n >= 2) {
+                PhaseArgs<float4, float> a{};
+                a.sv = (float4*)s->amps;
+                std::vector<int> f4;
+                for (int p : fp) f4.push_back(p - 1);
+                a.fixed = make_fixed(f4);
+                a.setmask = mask >> 1;
+                a.units = s->N >> (1 + fp.size());
+                a.pr = mf[0];
+                a.pi = mf[1];
+                hipLaunchKernelGGL(k_phase_v4, dim3(blocks_for(a.units, kUnroll)), dim3(kBlock), 0, s->stream, a);
+            } else {
+                PhaseArgs<float2, float> a{};
+                a.sv = (float2*)s->amps;
+                a.fixed = make_fixed(fp);
+                a.setmask = mask;
+                a.units = s->N >> fp.size();
+                a.pr = mf[0];
+                a.pi = mf[1];
+                hipLaunchKernelGGL((k_phase<float2, float>), dim3(grid_stride_blocks(a.units)), dim3(kBlock), 0,
+                                   s->stream, a);
+            }
+            break;
+        }
+        case AQS_OP_X: {
+            ProfScope ps(s, AQS_OP_X);
+            const int t = qpos(n, op.targets[0]);
+            if (!any && all_ge1(cb.cpos) && n >= 2) {
+                XchgV4Args a{};
+                a.sv = (float4*)s->amps;
+                std::vector<int> fp;
+                for (int p : cb.cpos) fp.push_back(p - 1);
+                a.setmask = cb.cmask >> 1;
+                if (t >= 1) {
+                    fp.push_back(t - 1);
+                    a.fixed = make_fixed(fp);
+                    a.partner = 1ull << (t - 1);
+                    a.units = s->N >> (2 + cb.cpos.size());
+                    hipLaunchKernelGGL(k_xchg_v4, dim3(blocks_for(a.units, kUnroll)), dim3(kBlock), 0, s->stream, a);
+                } else {
+                    a.fixed = make_fixed(fp);
+                    a.units = s->N >> (1 + cb.cpos.size());
+                    hipLaunchKernelGGL(k_x_b0, dim3(grid_stride_blocks(a.units)), dim3(kBlock), 0, s->stream, a);
+                }
+            } else {
+                std::vector<int> fp = any ? std::vector<int>{} : cb.cpos;
+                fp.push_back(t);
+                launch_pair<float2, float>(s, 1, op, fp, any ? 0 : cb.cmask, 1ull << t, any ? cb.cmask : 0,
+                                           s->N >> fp.size());
+            }
+            break;
+        }
+        case AQS_OP_SWAP: {
+            ProfScope ps(s, AQS_OP_SWAP);
+            const int pa = qpos(n, op.targets[0]), pb = qpos(n, op.targets[1]);
+            std::vector<int> fp = any ? std::vector<int>{} : cb.cpos;
+            fp.push_back(pa);
+            fp.push_back(pb);
+            const uint64_t set = (any ? 0 : cb.cmask) | (1ull << pa);
+            const uint64_t partner = (1ull << pa) | (1ull << pb);
+            if (!any && all_ge1(fp) && n >= 3) {
+                XchgV4Args a{};
+                a.sv = (float4*)s->amps;
+                std::vector<int> f4;
+                for (int p : fp) f4.push_back(p - 1);
+                a.fixed = make_fixed(f4);
+                a.setmask = set >> 1;
+                a.partner = partner >> 1;
+                a.units = s->N >> (1 + fp.size());
+                hipLaunchKernelGGL(k_xchg_v4, dim3(blocks_for(a.units, kUnroll)), dim3(kBlock), 0, s->stream, a);
+            } else {
+                launch_pair<float2, float>(s, 2, op, fp, set, partner, any ? cb.cmask : 0, s->N >> fp.size());
+            }
+            break;
+        }
+        default:
+            break;
+    }
+    AQS_HIP(hipGetLastError());
+}
+
+void apply_c128(aqs_state s, const aqs_op& op) {
+    const int n = s->n;
+    OpBits cb = control_bits(n, op);
+    const bool any = (op.flags & AQS_CTRL_ANY) && op.ncontrols > 0;
+    ProfScope ps(s, op.kind);
+    switch (op.kind) {
+        case AQS_OP_MAT1:
+        case AQS_OP_DIAG1:
+        case AQS_OP_X: {
+            const int t = qpos(n, op.targets[0]);
+            std::vector<int> fp = any ? std::vector<int>{} : cb.cpos;
+            fp.push_back(t);
+            int code = op.kind == AQS_OP_MAT1 ? 0 : (op.kind == AQS_OP_X ? 1 : 3);
+            launch_pair<double2, double>(s, code, op, fp, any ? 0 : cb.cmask, 1ull << t, any ? cb.cmask : 0,
+                                         s->N >> fp.size());
+            break;
+        }
+        case AQS_OP_SWAP: {
+            const int pa = qpos(n, op.targets[0]), pb = qpos(n, op.targets[1]);
+            std::vector<int> fp = any ? std::vector<int>{} : cb.cpos;
+            fp.push_back(pa);
+            fp.push_back(pb);
+            launch_pair<double2, double>(s, 2, op, fp, (any ? 0 : cb.cmask) | (1ull << pa), (1ull << pa) | (1ull << pb),
+                                         any ? cb.cmask : 0, s->N >> fp.size());
+            break;
+        }
+        case AQS_OP_PHASE: {
+            std::vector<int> fp = cb.cpos;
+            uint64_t mask = cb.cmask;
+            for (int j = 0; j < op.ntargets; ++j) {
+                uint64_t m = qbit(n, op.targets[j]);
+                if (mask & m) continue;
+                mask |= m;
+                fp.push_back(qpos(n, op.targets[j]));
+            }
+            PhaseArgs<double2, double> a{};
+            a.sv = (double2*)s->amps;
+            a.fixed = make_fixed(fp);
+            a.setmask = mask;
+            a.units = s->N >> fp.size();
+            a.pr = op.m[0];
+            a.pi = op.m[1];
+            hipLaunchKernelGGL((k_phase<double2, double>), dim3(grid_stride_blocks(a.units)), dim3(kBlock), 0,
+                               s->stream, a);
+            break;
+        }
+        default:
+            break;
+    }
+    AQS_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+namespace aqs_hip {
+
+void apply_op(aqs_state s, const aqs_op& op) {
+    validate_op(op, s->n);
+    if (op.kind == AQS_OP_MATK) {
+        const int k = op.ntargets;
+        const size_t K = (size_t)1 << k;
+        if (op.flags & AQS_CTRL_ANY) fail(AQS_ERR_INVALID_ARGUMENT, "MATK does not take AQS_CTRL_ANY");
+        if (s->precision == AQS_C64) {
+            if (k >= 4 && op.ncontrols == 0) {
+                std::vector<float> pk = pack_mfma_operand(k, op.u);
+                ensure_staging(s, pk.size() * sizeof(float));
+                AQS_HIP(hipMemcpyAsync(s->staging, pk.data(), pk.size() * sizeof(float), hipMemcpyHostToDevice,
+                                       s->stream));
+                if (apply_matk_mfma(s, op, (const float*)s->staging)) return;
+            }
+            std::vector<float> uf(2 * K * K);
+            for (size_t i = 0; i < uf.size(); ++i) uf[i] = (float)op.u[i];
+            ensure_staging(s, uf.size() * sizeof(float));
+            AQS_HIP(hipMemcpyAsync(s->staging, uf.data(), uf.size() * sizeof(float), hipMemcpyHostToDevice, s->stream));
+            ProfScope ps(s, AQS_OP_MATK);
+            dispatch_matk<float2, float>(s, op, s->staging);
+        } else {
+            ensure_staging(s, 2 * K * K * sizeof(double));
+            AQS_HIP(hipMemcpyAsync(s->staging, op.u, 2 * K * K * sizeof(double), hipMemcpyHostToDevice, s->stream));
+            ProfScope ps(s, AQS_OP_MATK);
+            dispatch_matk<double2, double>(s, op, s->staging);
+        }
+        AQS_HIP(hipGetLastError());
+        return;
+    }
+    if (s->precision == AQS_C64)
+        apply_c64_fast_or_generic(s, op);
+    else
+        apply_c128(s, op);
+}
+
+// Apply a MATK whose matrix is already on the device (FP32 or FP64 per the
+// state's precision, plain row-major layout).
+void apply_matk_device(aqs_state s, const aqs_op& op, const void* dev_u) {
+    ProfScope ps(s, AQS_OP_MATK);
+    if (s->precision == AQS_C64)
+        dispatch_matk<float2, float>(s, op, dev_u);
+    else
+        dispatch_matk<double2, double>(s, op, dev_u);
+    AQS_HIP(hipGetLastError());
+}
+
+}  // namespace aqs_hip
+
+extern "C" aqs_status aqs_apply(aqs_state s, const aqs_op* ops, size_t count) {
+    return guard([&] {
+        if (!s || !s->amps) fail(AQS_ERR_INVALID_ARGUMENT, "null state handle");
+        if (count && !ops) fail(AQS_ERR_INVALID_ARGUMENT, "null op array");
+        for (size_t i = 0; i < count; ++i) {
+            // A staged MATK matrix must not be overwritten while a previous
+            // kernel may still read it.
+            if (ops[i].kind == AQS_OP_MATK && i > 0) AQS_HIP(hipStreamSynchronize(s->stream));
+            apply_op(s, ops[i]);
+        }
+    });
+}

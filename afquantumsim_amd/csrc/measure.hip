@@ -1,0 +1,531 @@
+// Measurement: FP64 reductions, exact fixed-point sampling, collapse.
+//
+// The reference forms |a|^2 in FP32 and reduces / scans it with ArrayFire
+// (af::sum, af::accum: src/quantum.cpp:349-356,372-391,467-501); sampling
+// there tiles the CDF against every uniform, O(shots * 2^n).  Here:
+//   - norms, P(q=1) and <sum Z> are FP64 block reductions with a fixed grid,
+//     so results are run-to-run identical;
+//   - sampling quantises |a|^2 (formed in FP64) to 2^-60 fixed point and
+//     builds the cumulative sum exactly in uint64: per-1024-amplitude block
+//     sums, one exact scan over the blocks, then per shot a binary search
+//     over block prefixes and a wavefront scan inside the chosen block.
+//     Exact integer sums make the sampled index independent of summation
+//     order, so a CPU restatement reproduces it bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "internal.h"
+
+using namespace aqs_hip;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kRedGrid = 1024;       // fixed reduction grid (deterministic order)
+constexpr int kSampleBlock = 1024;   // amplitudes per sampling block
+constexpr double kFixScale = 1152921504606846976.0;  // 2^60
+
+template <typename T2>
+__device__ __forceinline__ double prob_of(const T2& v) {
+    const double re = (double)v.x, im = (double)v.y;
+    return fma(re, re, im * im);
+}
+
+__device__ __forceinline__ uint64_t quantise(double p) {
+    return (uint64_t)rint(p * kFixScale);
+}
+
+template <typename T>
+__device__ T block_reduce_sum(T v, T* sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    T r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// MODE 0: sum |a|^2; 1: sum |a|^2 over (i & mask) != 0; 2: sum |a|^2 (n - 2 popcount(i))
+template <typename T2, int MODE>
+__global__ __launch_bounds__(kBlock) void k_reduce(const T2* __restrict__ a, uint64_t N, uint64_t mask, int n,
+                                                   double* __restrict__ partial) {
+    __shared__ double sh[kBlock];
+    double acc = 0.0;
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += stride) {
+        const double p = prob_of(a[i]);
+        if (MODE == 0) acc += p;
+        if (MODE == 1) acc += (i & mask) ? p : 0.0;
+        if (MODE == 2) acc += p * (double)(n - 2 * __popcll(i));
+    }
+    double r = block_reduce_sum(acc, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = r;
+}
+
+// <a|b> = sum conj(a_i) b_i
+template <typename TA, typename TB>
+__global__ __launch_bounds__(kBlock) void k_inner(const TA* __restrict__ a, const TB* __restrict__ b, uint64_t N,
+                                                  double* __restrict__ partial) {
+    __shared__ double sh[kBlock];
+    double re = 0.0, im = 0.0;
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += stride) {
+        const double ar = a[i].x, ai = a[i].y, br = b[i].x, bi = b[i].y;
+        re += ar * br + ai * bi;
+        im += ar * bi - ai * br;
+    }
+    double r0 = block_reduce_sum(re, sh);
+    double r1 = block_reduce_sum(im, sh);
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = r0;
+        partial[2 * blockIdx.x + 1] = r1;
+    }
+}
+
+// Sum `count` partial vectors of width w in a fixed order (one block).
+__global__ __launch_bounds__(kBlock) void k_reduce_final(const double* __restrict__ partial, int count, int w,
+                                                         double* __restrict__ out) {
+    __shared__ double sh[kBlock];
+    for (int c = 0; c < w; ++c) {
+        double acc = 0.0;
+        for (int i = threadIdx.x; i < count; i += kBlock) acc += partial[i * w + c];
+        double r = block_reduce_sum(acc, sh);
+        if (threadIdx.x == 0) out[c] = r;
+    }
+}
+
+template <typename T2, typename R>
+__global__ __launch_bounds__(kBlock) void k_scale(T2* __restrict__ a, uint64_t N, R f) {
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += stride) {
+        T2 v = a[i];
+        v.x *= f;
+        v.y *= f;
+        a[i] = v;
+    }
+}
+
+// Keep amplitudes whose bit matches `keep_one`, scaled by f; zero the rest.
+template <typename T2, typename R>
+__global__ __launch_bounds__(kBlock) void k_collapse_qubit(T2* __restrict__ a, uint64_t N, uint64_t mask, int keep_one,
+                                                           R f) {
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += stride) {
+        const bool one = (i & mask) != 0;
+        T2 v = a[i];
+        if (one == (keep_one != 0)) {
+            v.x *= f;
+            v.y *= f;
+        } else {
+            v.x = 0;
+            v.y = 0;
+        }
+        a[i] = v;
+    }
+}
+
+template <typename T2>
+__global__ __launch_bounds__(kBlock) void k_probs(const T2* __restrict__ a, uint64_t N, float* __restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += stride) {
+        const T2 v = a[i];
+        out[i] = (float)(v.x * v.x + v.y * v.y);
+    }
+}
+
+// ---- sampling -------------------------------------------------------------
+
+// sums[b] = exact sum of quantised |a|^2 over block b; lastnz[b] = last index
+// with non-zero quantised probability in block b (-1 when none).
+template <typename T2>
+__global__ __launch_bounds__(kBlock) void k_block_sums(const T2* __restrict__ a, uint64_t N, uint64_t* __restrict__ sums,
+                                                       int64_t* __restrict__ lastnz) {
+    __shared__ uint64_t sh[kBlock];
+    __shared__ int64_t shl[kBlock];
+    const uint64_t b = blockIdx.x;
+    uint64_t acc = 0;
+    int64_t last = -1;
+    for (int j = 0; j < kSampleBlock / kBlock; ++j) {
+        const uint64_t i = b * kSampleBlock + (uint64_t)j * kBlock + threadIdx.x;
+        if (i < N) {
+            const uint64_t q = quantise(prob_of(a[i]));
+            acc += q;
+            if (q) last = (int64_t)i;
+        }
+    }
+    uint64_t s = block_reduce_sum(acc, sh);
+    shl[threadIdx.x] = last;
+    __syncthreads();
+    for (int st = kBlock / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st) shl[threadIdx.x] = max(shl[threadIdx.x], shl[threadIdx.x + st]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        sums[b] = s;
+        lastnz[b] = shl[0];
+    }
+}
+
+// Inclusive prefix of the block sums (exact), plus the global total and the
+// global last non-zero index.  One workgroup of 1024 threads.
+__global__ __launch_bounds__(1024) void k_scan_blocks(const uint64_t* __restrict__ sums, const int64_t* __restrict__ lastnz,
+                                                      uint64_t nb, uint64_t* __restrict__ incl, uint64_t* __restrict__ misc) {
+    __shared__ uint64_t sh[1024];
+    __shared__ int64_t shl[1024];
+    const int t = threadIdx.x;
+    const uint64_t per = (nb + 1023) / 1024;
+    const uint64_t lo = (uint64_t)t * per, hi = min(nb, lo + per);
+    uint64_t acc = 0;
+    int64_t last = -1;
+    for (uint64_t i = lo; i < hi; ++i) {
+        acc += sums[i];
+        last = max(last, lastnz[i]);
+    }
+    sh[t] = acc;
+    shl[t] = last;
+    __syncthreads();
+    // inclusive Hillis-Steele scan over 1024 chunk totals
+    for (int off = 1; off < 1024; off <<= 1) {
+        uint64_t v = (t >= off) ? sh[t - off] : 0;
+        __syncthreads();
+        sh[t] += v;
+        __syncthreads();
+    }
+    uint64_t run = (t > 0) ? sh[t - 1] : 0;
+    for (uint64_t i = lo; i < hi; ++i) {
+        run += sums[i];
+        incl[i] = run;
+    }
+    for (int st = 512; st > 0; st >>= 1) {
+        if (t < st) shl[t] = max(shl[t], shl[t + st]);
+        __syncthreads();
+    }
+    if (t == 0) {
+        misc[0] = sh[1023];           // total
+        misc[1] = (uint64_t)shl[0];   // last non-zero index (as int64 bits)
+    }
+}
+
+// One wavefront per shot.
+template <typename T2>
+__global__ __launch_bounds__(kBlock) void k_sample(const T2* __restrict__ a, uint64_t N, const uint64_t* __restrict__ sums,
+                                                   const uint64_t* __restrict__ incl, const uint64_t* __restrict__ misc,
+                                                   uint64_t nb, const double* __restrict__ u, uint32_t shots,
+                                                   uint64_t* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t shot = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    if (shot >= shots) return;
+    const uint64_t thr = (uint64_t)floor(u[shot] * kFixScale);
+    const uint64_t total = misc[0];
+    if (thr >= total) {
+        // at or above the total: the last index of non-zero probability
+        // (0 for a null state)
+        if (lane == 0) out[shot] = ((int64_t)misc[1] < 0) ? 0 : misc[1];
+        return;
+    }
+    // first block with incl[b] > thr
+    uint64_t lo = 0, hi = nb;
+    while (lo < hi) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        if (incl[mid] > thr) hi = mid;
+        else lo = mid + 1;
+    }
+    const uint64_t b = lo;
+    const uint64_t before = incl[b] - sums[b];
+    constexpr int per = kSampleBlock / 64;  // 16 amplitudes per lane
+    const uint64_t i0 = b * kSampleBlock + (uint64_t)lane * per;
+    uint64_t q[per];
+    uint64_t lsum = 0;
+#pragma unroll
+    for (int j = 0; j < per; ++j) {
+        const uint64_t i = i0 + j;
+        q[j] = (i < N) ? quantise(prob_of(a[i])) : 0;
+        lsum += q[j];
+    }
+    // inclusive wavefront scan of lane sums
+    uint64_t incl_l = lsum;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint64_t v = __shfl_up(incl_l, off, 64);
+        if (lane >= off) incl_l += v;
+    }
+    const uint64_t excl_l = incl_l - lsum;
+    // the lane whose range crosses thr
+    const bool crosses = (before + excl_l <= thr) && (before + incl_l > thr);
+    const unsigned long long ballot = __ballot(crosses);
+    const int owner = ballot ? (__ffsll((long long)ballot) - 1) : 63;
+    if (lane == owner) {
+        uint64_t run = before + excl_l;
+        uint64_t idx = i0 + per - 1;
+#pragma unroll
+        for (int j = 0; j < per; ++j) {
+            run += q[j];
+            if (run > thr) {
+                idx = i0 + j;
+                break;
+            }
+        }
+        out[shot] = idx;
+    }
+}
+
+unsigned red_grid(uint64_t N) {
+    uint64_t b = (N + kBlock - 1) / kBlock;
+    return (unsigned)std::min<uint64_t>(std::max<uint64_t>(b, 1), kRedGrid);
+}
+
+double* partial_buf(aqs_state s) { return (double*)s->scratch; }
+double* result_buf(aqs_state s) { return (double*)s->scratch + 2 * kRedGrid; }
+
+void read_results(aqs_state s, double* out, int w) {
+    ensure_pinned(s, sizeof(double) * w);
+    AQS_HIP(hipMemcpyAsync(s->pinned, result_buf(s), sizeof(double) * w, hipMemcpyDeviceToHost, s->stream));
+    AQS_HIP(hipStreamSynchronize(s->stream));
+    std::copy((double*)s->pinned, (double*)s->pinned + w, out);
+}
+
+template <int MODE>
+double reduce_mode(aqs_state s, uint64_t mask) {
+    const unsigned g = red_grid(s->N);
+    {
+        ProfScope ps(s, 0);
+        if (s->precision == AQS_C64)
+            hipLaunchKernelGGL((k_reduce<float2, MODE>), dim3(g), dim3(kBlock), 0, s->stream, (const float2*)s->amps, s->N,
+                               mask, s->n, partial_buf(s));
+        else
+            hipLaunchKernelGGL((k_reduce<double2, MODE>), dim3(g), dim3(kBlock), 0, s->stream, (const double2*)s->amps,
+                               s->N, mask, s->n, partial_buf(s));
+        hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlock), 0, s->stream, (const double*)partial_buf(s), (int)g, 1,
+                           result_buf(s));
+        AQS_HIP(hipGetLastError());
+    }
+    double r;
+    read_results(s, &r, 1);
+    return r;
+}
+
+aqs_state check(aqs_state s) {
+    if (!s || !s->amps) fail(AQS_ERR_INVALID_ARGUMENT, "null state handle");
+    return s;
+}
+
+void check_qubit(aqs_state s, int q, const char* msg) {
+    if (q < 0 || q >= s->n) fail(AQS_ERR_OUT_OF_RANGE, msg);
+}
+
+}  // namespace
+
+namespace aqs_hip {
+
+double reduce_norm2(aqs_state s) { return reduce_mode<0>(s, 0); }
+
+void scale_state(aqs_state s, double factor) {
+    const unsigned g = red_grid(s->N) * 4;
+    ProfScope ps(s, 0);
+    if (s->precision == AQS_C64)
+        hipLaunchKernelGGL((k_scale<float2, float>), dim3(g), dim3(kBlock), 0, s->stream, (float2*)s->amps, s->N,
+                           (float)factor);
+    else
+        hipLaunchKernelGGL((k_scale<double2, double>), dim3(g), dim3(kBlock), 0, s->stream, (double2*)s->amps, s->N,
+                           factor);
+    AQS_HIP(hipGetLastError());
+}
+
+}  // namespace aqs_hip
+
+extern "C" {
+
+aqs_status aqs_norm2(aqs_state s, double* out) {
+    return guard([&] {
+        check(s);
+        double r = reduce_norm2(s);
+        if (out) *out = r;
+    });
+}
+
+aqs_status aqs_normalize(aqs_state s) {
+    return guard([&] {
+        check(s);
+        double n2 = reduce_norm2(s);
+        if (!(n2 > 0.0)) fail(AQS_ERR_INVALID_ARGUMENT, "Cannot normalize a null state");
+        scale_state(s, 1.0 / std::sqrt(n2));
+    });
+}
+
+aqs_status aqs_prob1(aqs_state s, int qubit, double* out) {
+    return guard([&] {
+        check(s);
+        check_qubit(s, qubit, "Cannot obtain probability of the given qubit");
+        double r = reduce_mode<1>(s, qbit(s->n, qubit));
+        if (out) *out = r;
+    });
+}
+
+aqs_status aqs_prob1_all(aqs_state s, double* out) {
+    return guard([&] {
+        check(s);
+        for (int q = 0; q < s->n; ++q) {
+            double r = reduce_mode<1>(s, qbit(s->n, q));
+            if (out) out[q] = r;
+        }
+    });
+}
+
+aqs_status aqs_expect_sum_z(aqs_state s, double* out) {
+    return guard([&] {
+        check(s);
+        double r = reduce_mode<2>(s, 0);
+        if (out) *out = r;
+    });
+}
+
+aqs_status aqs_probabilities(aqs_state s, float* out) {
+    return guard([&] {
+        check(s);
+        if (!out) fail(AQS_ERR_INVALID_ARGUMENT, "null output");
+        float* d = nullptr;
+        AQS_HIP(hipMalloc(&d, s->N * sizeof(float)));
+        try {
+            const unsigned g = red_grid(s->N) * 4;
+            if (s->precision == AQS_C64)
+                hipLaunchKernelGGL(k_probs<float2>, dim3(g), dim3(kBlock), 0, s->stream, (const float2*)s->amps, s->N, d);
+            else
+                hipLaunchKernelGGL(k_probs<double2>, dim3(g), dim3(kBlock), 0, s->stream, (const double2*)s->amps, s->N, d);
+            AQS_HIP(hipGetLastError());
+            AQS_HIP(hipMemcpyAsync(out, d, s->N * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+            AQS_HIP(hipStreamSynchronize(s->stream));
+        } catch (...) {
+            (void)hipFree(d);
+            throw;
+        }
+        AQS_HIP(hipFree(d));
+    });
+}
+
+aqs_status aqs_amplitude(aqs_state s, uint64_t index, double* re_im) {
+    return guard([&] {
+        check(s);
+        if (index >= s->N) fail(AQS_ERR_OUT_OF_RANGE, "amplitude index out of range");
+        ensure_pinned(s, 16);
+        AQS_HIP(hipMemcpyAsync(s->pinned, (const char*)s->amps + index * s->amp_bytes(), s->amp_bytes(),
+                               hipMemcpyDeviceToHost, s->stream));
+        AQS_HIP(hipStreamSynchronize(s->stream));
+        if (re_im) {
+            if (s->precision == AQS_C64) {
+                re_im[0] = ((float*)s->pinned)[0];
+                re_im[1] = ((float*)s->pinned)[1];
+            } else {
+                re_im[0] = ((double*)s->pinned)[0];
+                re_im[1] = ((double*)s->pinned)[1];
+            }
+        }
+    });
+}
+
+aqs_status aqs_sample(aqs_state s, const double* u, uint32_t shots, uint64_t* out_index) {
+    return guard([&] {
+        check(s);
+        if (shots == 0) return;
+        if (!u || !out_index) fail(AQS_ERR_INVALID_ARGUMENT, "null uniforms or output");
+        const uint64_t nb = (s->N + kSampleBlock - 1) / kSampleBlock;
+        uint64_t* sums = (uint64_t*)s->scratch;
+        uint64_t* incl = sums + nb;
+        int64_t* lastnz = (int64_t*)(incl + nb);
+        uint64_t* misc = (uint64_t*)(lastnz + nb);
+        if ((size_t)((char*)(misc + 2) - (char*)s->scratch) > s->scratch_bytes)
+            fail(AQS_ERR_RUNTIME, "sampling scratch too small");
+        // uniforms in, indices out through the staging buffer
+        const size_t ubytes = sizeof(double) * shots, obytes = sizeof(uint64_t) * shots;
+        ensure_staging(s, ubytes + obytes);
+        AQS_HIP(hipMemcpyAsync(s->staging, u, ubytes, hipMemcpyHostToDevice, s->stream));
+        double* du = (double*)s->staging;
+        uint64_t* dout = (uint64_t*)((char*)s->staging + ubytes);
+        {
+            ProfScope ps(s, 0);
+            if (s->precision == AQS_C64) {
+                hipLaunchKernelGGL(k_block_sums<float2>, dim3((unsigned)nb), dim3(kBlock), 0, s->stream,
+                                   (const float2*)s->amps, s->N, sums, lastnz);
+            } else {
+                hipLaunchKernelGGL(k_block_sums<double2>, dim3((unsigned)nb), dim3(kBlock), 0, s->stream,
+                                   (const double2*)s->amps, s->N, sums, lastnz);
+            }
+            hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s->stream, (const uint64_t*)sums,
+                               (const int64_t*)lastnz, nb, incl, misc);
+            const unsigned g = (shots + (kBlock / 64) - 1) / (kBlock / 64);
+            if (s->precision == AQS_C64)
+                hipLaunchKernelGGL(k_sample<float2>, dim3(g), dim3(kBlock), 0, s->stream, (const float2*)s->amps, s->N,
+                                   (const uint64_t*)sums, (const uint64_t*)incl, (const uint64_t*)misc, nb, du, shots, dout);
+            else
+                hipLaunchKernelGGL(k_sample<double2>, dim3(g), dim3(kBlock), 0, s->stream, (const double2*)s->amps, s->N,
+                                   (const uint64_t*)sums, (const uint64_t*)incl, (const uint64_t*)misc, nb, du, shots, dout);
+            AQS_HIP(hipGetLastError());
+        }
+        AQS_HIP(hipMemcpyAsync(out_index, dout, obytes, hipMemcpyDeviceToHost, s->stream));
+        AQS_HIP(hipStreamSynchronize(s->stream));
+    });
+}
+
+aqs_status aqs_measure_qubit(aqs_state s, int qubit, double u, int* outcome, double* p1_out) {
+    return guard([&] {
+        check(s);
+        check_qubit(s, qubit, "Cannot measure the state of the given qubit");
+        const uint64_t mask = qbit(s->n, qubit);
+        const double p1 = reduce_mode<1>(s, mask);
+        const int one = u < p1 ? 1 : 0;
+        const double p = one ? p1 : 1.0 - p1;
+        if (!(p > 0.0)) fail(AQS_ERR_RUNTIME, "measured outcome has zero probability");
+        const double f = 1.0 / std::sqrt(p);
+        const unsigned g = red_grid(s->N) * 4;
+        ProfScope ps(s, 0);
+        if (s->precision == AQS_C64)
+            hipLaunchKernelGGL((k_collapse_qubit<float2, float>), dim3(g), dim3(kBlock), 0, s->stream, (float2*)s->amps,
+                               s->N, mask, one, (float)f);
+        else
+            hipLaunchKernelGGL((k_collapse_qubit<double2, double>), dim3(g), dim3(kBlock), 0, s->stream,
+                               (double2*)s->amps, s->N, mask, one, f);
+        AQS_HIP(hipGetLastError());
+        if (outcome) *outcome = one;
+        if (p1_out) *p1_out = p1;
+    });
+}
+
+aqs_status aqs_collapse_to(aqs_state s, uint64_t index) { return aqs_state_set_basis(s, index); }
+
+aqs_status aqs_inner(aqs_state a, aqs_state b, double* re_im) {
+    return guard([&] {
+        check(a);
+        check(b);
+        if (a->n != b->n) fail(AQS_ERR_INVALID_ARGUMENT, "inner product of states with different qubit counts");
+        if (a->stream != b->stream) AQS_HIP(hipStreamSynchronize(b->stream));
+        const unsigned g = red_grid(a->N);
+        double* part = partial_buf(a);
+        if (a->precision == AQS_C64 && b->precision == AQS_C64)
+            hipLaunchKernelGGL((k_inner<float2, float2>), dim3(g), dim3(kBlock), 0, a->stream, (const float2*)a->amps,
+                               (const float2*)b->amps, a->N, part);
+        else if (a->precision == AQS_C64)
+            hipLaunchKernelGGL((k_inner<float2, double2>), dim3(g), dim3(kBlock), 0, a->stream, (const float2*)a->amps,
+                               (const double2*)b->amps, a->N, part);
+        else if (b->precision == AQS_C64)
+            hipLaunchKernelGGL((k_inner<double2, float2>), dim3(g), dim3(kBlock), 0, a->stream, (const double2*)a->amps,
+                               (const float2*)b->amps, a->N, part);
+        else
+            hipLaunchKernelGGL((k_inner<double2, double2>), dim3(g), dim3(kBlock), 0, a->stream,
+                               (const double2*)a->amps, (const double2*)b->amps, a->N, part);
+        hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlock), 0, a->stream, (const double*)part, (int)g, 2,
+                           result_buf(a));
+        AQS_HIP(hipGetLastError());
+        double r[2];
+        read_results(a, r, 2);
+        if (re_im) {
+            re_im[0] = r[0];
+            re_im[1] = r[1];
+        }
+    });
+}
+
+}  // extern "C"

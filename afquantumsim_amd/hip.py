@@ -1,0 +1,373 @@
+"""ctypes binding of the C ABI in include/aqs_hip.h (libaqs_hip.so).
+
+This is the product's Python host layer: it loads the in-tree HIP library
+and fails loudly when it is missing -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libaqs_hip.so")
+
+AQS_OK = 0
+AQS_ERR_INVALID_ARGUMENT = 1
+AQS_ERR_OUT_OF_RANGE = 2
+AQS_ERR_DOMAIN = 3
+AQS_ERR_RUNTIME = 4
+AQS_ERR_DEVICE = 5
+AQS_ERR_NO_MEMORY = 6
+
+C64 = 0
+C128 = 1
+
+OP_MAT1 = 1
+OP_DIAG1 = 2
+OP_X = 3
+OP_SWAP = 4
+OP_MATK = 5
+OP_PHASE = 6
+CTRL_ANY = 1
+
+PROFILE_SLOTS = ["other", "MAT1", "DIAG1", "X", "SWAP", "MATK", "PHASE", "MATK_MFMA"]
+
+
+class AqsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+class InvalidArgument(AqsError, ValueError):
+    pass
+
+
+class OutOfRange(AqsError, IndexError):
+    pass
+
+
+class DomainError(AqsError, ValueError):
+    pass
+
+
+_EXC = {AQS_ERR_INVALID_ARGUMENT: InvalidArgument, AQS_ERR_OUT_OF_RANGE: OutOfRange,
+        AQS_ERR_DOMAIN: DomainError}
+
+
+class Op(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("ntargets", ctypes.c_int32),
+        ("targets", ctypes.c_int32 * 5),
+        ("ncontrols", ctypes.c_int32),
+        ("controls", ctypes.c_int32 * 3),
+        ("flags", ctypes.c_int32),
+        ("m", ctypes.c_double * 8),
+        ("u", ctypes.c_void_p),
+    ]
+
+
+EXPORTS = [
+    "aqs_last_error", "aqs_init", "aqs_device_info", "aqs_state_create", "aqs_state_destroy",
+    "aqs_state_copy", "aqs_state_qubits", "aqs_state_precision", "aqs_state_device_ptr",
+    "aqs_state_set_stream", "aqs_state_sync", "aqs_state_set_basis", "aqs_state_product",
+    "aqs_state_gaussian", "aqs_state_upload", "aqs_state_download", "aqs_apply", "aqs_fuse",
+    "aqs_schedule_destroy", "aqs_schedule_info", "aqs_schedule_get", "aqs_apply_schedule",
+    "aqs_norm2", "aqs_normalize", "aqs_prob1", "aqs_prob1_all", "aqs_expect_sum_z",
+    "aqs_probabilities", "aqs_amplitude", "aqs_sample", "aqs_measure_qubit", "aqs_collapse_to",
+    "aqs_inner", "aqs_profile_enable", "aqs_profile_read", "aqs_timer_start", "aqs_timer_stop",
+]
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load libaqs_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m afquantumsim_amd.build`")
+    L = ctypes.CDLL(path)
+    vp, sz, i32, u64, dp = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)
+    st = ctypes.c_int
+    sigs = {
+        "aqs_init": [i32],
+        "aqs_device_info": [ctypes.c_char_p, sz, ctypes.POINTER(i32), ctypes.POINTER(sz)],
+        "aqs_state_create": [i32, i32, ctypes.POINTER(vp)],
+        "aqs_state_destroy": [vp],
+        "aqs_state_copy": [vp, vp],
+        "aqs_state_qubits": [vp, ctypes.POINTER(i32)],
+        "aqs_state_precision": [vp, ctypes.POINTER(i32)],
+        "aqs_state_device_ptr": [vp, ctypes.POINTER(vp)],
+        "aqs_state_set_stream": [vp, vp],
+        "aqs_state_sync": [vp],
+        "aqs_state_set_basis": [vp, u64],
+        "aqs_state_product": [vp, dp],
+        "aqs_state_gaussian": [vp, u64],
+        "aqs_state_upload": [vp, vp, i32],
+        "aqs_state_download": [vp, vp],
+        "aqs_apply": [vp, ctypes.POINTER(Op), sz],
+        "aqs_fuse": [ctypes.POINTER(Op), sz, i32, i32, ctypes.POINTER(vp)],
+        "aqs_schedule_destroy": [vp],
+        "aqs_schedule_info": [vp, ctypes.POINTER(sz), ctypes.POINTER(sz)],
+        "aqs_schedule_get": [vp, sz, ctypes.POINTER(Op)],
+        "aqs_apply_schedule": [vp, vp],
+        "aqs_norm2": [vp, dp],
+        "aqs_normalize": [vp],
+        "aqs_prob1": [vp, i32, dp],
+        "aqs_prob1_all": [vp, dp],
+        "aqs_expect_sum_z": [vp, dp],
+        "aqs_probabilities": [vp, ctypes.POINTER(ctypes.c_float)],
+        "aqs_amplitude": [vp, u64, dp],
+        "aqs_sample": [vp, dp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)],
+        "aqs_measure_qubit": [vp, i32, ctypes.c_double, ctypes.POINTER(i32), dp],
+        "aqs_collapse_to": [vp, u64],
+        "aqs_inner": [vp, vp, dp],
+        "aqs_profile_enable": [vp, i32],
+        "aqs_profile_read": [vp, dp, ctypes.POINTER(ctypes.c_uint64)],
+        "aqs_timer_start": [vp],
+        "aqs_timer_stop": [vp, dp],
+    }
+    for name, args in sigs.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = st
+    L.aqs_last_error.restype = ctypes.c_char_p
+    L.aqs_last_error.argtypes = []
+    _lib = L
+    return L
+
+
+def check(code: int):
+    if code != AQS_OK:
+        msg = _lib.aqs_last_error().decode(errors="replace")
+        raise _EXC.get(code, AqsError)(code, msg)
+
+
+def init(device: int = 0):
+    check(load().aqs_init(device))
+
+
+def device_info():
+    L = load()
+    name = ctypes.create_string_buffer(256)
+    cu = ctypes.c_int()
+    mem = ctypes.c_size_t()
+    check(L.aqs_device_info(name, 256, ctypes.byref(cu), ctypes.byref(mem)))
+    return {"arch": name.value.decode(), "cu": cu.value, "hbm_bytes": mem.value}
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class State:
+    """A device-resident 2^n state vector (complex64 or complex128)."""
+
+    def __init__(self, n: int, precision: int = C64):
+        L = load()
+        h = ctypes.c_void_p()
+        check(L.aqs_state_create(n, precision, ctypes.byref(h)))
+        self.h = h
+        self.n = n
+        self.precision = precision
+        self.dtype = np.complex64 if precision == C64 else np.complex128
+
+    def close(self):
+        if self.h:
+            check(_lib.aqs_state_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # preparation
+    def set_basis(self, index: int):
+        check(_lib.aqs_state_set_basis(self.h, index))
+
+    def product(self, qamps):
+        qa = np.ascontiguousarray(qamps, dtype=np.complex128).reshape(self.n, 2)
+        check(_lib.aqs_state_product(self.h, _dp(qa.view(np.float64))))
+
+    def gaussian(self, seed: int):
+        check(_lib.aqs_state_gaussian(self.h, seed))
+
+    def upload(self, host, normalize: bool = False):
+        a = np.ascontiguousarray(host, dtype=self.dtype)
+        assert a.size == 1 << self.n
+        check(_lib.aqs_state_upload(self.h, a.ctypes.data, int(normalize)))
+
+    def download(self):
+        out = np.empty(1 << self.n, dtype=self.dtype)
+        check(_lib.aqs_state_download(self.h, out.ctypes.data))
+        return out
+
+    def copy_from(self, other: "State"):
+        check(_lib.aqs_state_copy(self.h, other.h))
+
+    def sync(self):
+        check(_lib.aqs_state_sync(self.h))
+
+    # gates
+    def apply(self, ops):
+        arr, keep = pack_ops(ops)
+        check(_lib.aqs_apply(self.h, arr, len(ops)))
+        del keep
+
+    def apply_schedule(self, sch: "Schedule"):
+        check(_lib.aqs_apply_schedule(self.h, sch.h))
+
+    # measurement
+    def norm2(self) -> float:
+        r = ctypes.c_double()
+        check(_lib.aqs_norm2(self.h, ctypes.byref(r)))
+        return r.value
+
+    def normalize(self):
+        check(_lib.aqs_normalize(self.h))
+
+    def prob1(self, q: int) -> float:
+        r = ctypes.c_double()
+        check(_lib.aqs_prob1(self.h, q, ctypes.byref(r)))
+        return r.value
+
+    def prob1_all(self):
+        out = np.zeros(self.n, dtype=np.float64)
+        check(_lib.aqs_prob1_all(self.h, _dp(out)))
+        return out
+
+    def expect_sum_z(self) -> float:
+        r = ctypes.c_double()
+        check(_lib.aqs_expect_sum_z(self.h, ctypes.byref(r)))
+        return r.value
+
+    def probabilities(self):
+        out = np.empty(1 << self.n, dtype=np.float32)
+        check(_lib.aqs_probabilities(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    def amplitude(self, i: int) -> complex:
+        r = (ctypes.c_double * 2)()
+        check(_lib.aqs_amplitude(self.h, i, r))
+        return complex(r[0], r[1])
+
+    def sample(self, u) -> np.ndarray:
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        out = np.zeros(len(u), dtype=np.uint64)
+        check(_lib.aqs_sample(self.h, _dp(u), len(u), out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
+        return out
+
+    def measure_qubit(self, q: int, u: float):
+        o = ctypes.c_int()
+        p = ctypes.c_double()
+        check(_lib.aqs_measure_qubit(self.h, q, u, ctypes.byref(o), ctypes.byref(p)))
+        return o.value, p.value
+
+    def collapse_to(self, i: int):
+        check(_lib.aqs_collapse_to(self.h, i))
+
+    def inner(self, other: "State") -> complex:
+        r = (ctypes.c_double * 2)()
+        check(_lib.aqs_inner(self.h, other.h, r))
+        return complex(r[0], r[1])
+
+    # timing
+    def profile(self, enable: bool):
+        check(_lib.aqs_profile_enable(self.h, int(enable)))
+
+    def profile_read(self):
+        ms = (ctypes.c_double * 8)()
+        cnt = (ctypes.c_uint64 * 8)()
+        check(_lib.aqs_profile_read(self.h, ms, cnt))
+        return {PROFILE_SLOTS[i]: (ms[i], cnt[i]) for i in range(8) if cnt[i]}
+
+    def timer_start(self):
+        check(_lib.aqs_timer_start(self.h))
+
+    def timer_stop(self) -> float:
+        r = ctypes.c_double()
+        check(_lib.aqs_timer_stop(self.h, ctypes.byref(r)))
+        return r.value
+
+
+class Schedule:
+    """A fused gate schedule (compile() output)."""
+
+    def __init__(self, ops, n: int, max_qubits: int = 5):
+        L = load()
+        arr, keep = pack_ops(ops)
+        h = ctypes.c_void_p()
+        check(L.aqs_fuse(arr, len(ops), n, max_qubits, ctypes.byref(h)))
+        del keep
+        self.h = h
+        self.n = n
+
+    def info(self):
+        p, s = ctypes.c_size_t(), ctypes.c_size_t()
+        check(_lib.aqs_schedule_info(self.h, ctypes.byref(p), ctypes.byref(s)))
+        return p.value, s.value
+
+    def passes(self):
+        p, _ = self.info()
+        out = []
+        for i in range(p):
+            op = Op()
+            check(_lib.aqs_schedule_get(self.h, i, ctypes.byref(op)))
+            out.append((op.kind, list(op.targets[:op.ntargets])))
+        return out
+
+    def close(self):
+        if self.h:
+            check(_lib.aqs_schedule_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def pack_ops(ops):
+    """List of op dicts -> ctypes array (keeps matrices alive in `keep`)."""
+    arr = (Op * max(1, len(ops)))()
+    keep = []
+    for i, o in enumerate(ops):
+        x = arr[i]
+        x.kind = o["kind"]
+        t = o["targets"]
+        x.ntargets = len(t)
+        for j, q in enumerate(t):
+            x.targets[j] = q
+        c = o.get("controls", ())
+        x.ncontrols = len(c)
+        for j, q in enumerate(c):
+            x.controls[j] = q
+        x.flags = o.get("flags", 0)
+        m = o.get("m")
+        if m is not None:
+            mm = np.asarray(m, dtype=np.complex128).reshape(-1)
+            for j in range(len(mm)):
+                x.m[2 * j] = mm[j].real
+                x.m[2 * j + 1] = mm[j].imag
+        u = o.get("u")
+        if u is not None:
+            uu = np.ascontiguousarray(u, dtype=np.complex128)
+            keep.append(uu)
+            x.u = uu.ctypes.data
+        else:
+            x.u = None
+    return arr, keep

@@ -1,0 +1,208 @@
+#!/usr/bin/env python3
+"""Benchmark: 28-qubit complex64 brickwork circuit (depth 20: U3 on every
+qubit, then CZ on alternating nearest-neighbour pairs) on one MI355X.
+
+A step is one simulate() of the whole circuit from |0..0> with the plain
+per-gate HIP kernels (830 gate passes over the 2 GiB state).  The JSON line
+reports gate-applications/s for that path, the HBM roofline of its dominant
+kernel (the one-qubit U3 pass), the compile()-fused form of the same circuit,
+infidelity against a complex128 run of the same circuit, and the FP64 CPU
+oracle timed on the host as a baseline.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from afquantumsim_amd import circuits, hip, ops  # noqa: E402
+
+N_QUBITS = 28
+DEPTH = 20
+SEED = 2
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(lowered, n):
+    return sum(16.0 * (1 << n) * ops.touched_fraction(o, n) for o in lowered)
+
+
+def cpu_baseline(gates, n, budget_s=12.0):
+    """The FP64 oracle's per-gate pass (OpenMP) on the same circuit, timed on
+    a bounded sample: gates from the start of the circuit until budget_s."""
+    import oracle
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = max(1, min(threads, 16))
+    sv = oracle.basis_state(n, 0)
+    done = 0
+    t0 = time.perf_counter()
+    for g in gates:
+        sv = oracle.run(sv, n, [g], threads=threads)
+        done += 1
+        if time.perf_counter() - t0 > budget_s and done >= 2:
+            break
+    dt = time.perf_counter() - t0
+    cpu = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": done / dt, "unit": "gate-apps/s", "cores": threads, "kind": "port",
+            "sample": f"first {done} of {len(gates)} gates of the {n}-qubit brickwork, FP64 oracle "
+                      f"(oracle/aqs_oracle.c, OpenMP), host CPU: {cpu}"}
+
+
+def load_traffic():
+    path = os.path.join(ROOT, "profiles", "traffic_u3_28q.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return None
+
+
+def run(args):
+    hip.init(0)
+    dev = hip.device_info()
+    log(f"device {dev}")
+    n = N_QUBITS
+    gates = circuits.brickwork(n, DEPTH, seed=SEED)
+    lowered = ops.lower_all(gates)
+    n_gates = len(lowered)
+    alg_bytes = algorithmic_bytes(lowered, n)
+
+    st = hip.State(n, hip.C64)
+
+    def step():
+        st.set_basis(0)
+        st.apply(lowered)
+
+    for _ in range(args.warmup):
+        step()
+    st.sync()
+    log("warmup done")
+
+    st.profile(True)
+    st.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    st.sync()
+    dt = time.perf_counter() - t0
+    prof = st.profile_read()
+    st.profile(False)
+    log(f"timed {args.steps} steps in {dt:.3f} s; profile {prof}")
+    ms_per_step = 1e3 * dt / args.steps
+    gate_apps = n_gates * args.steps / dt
+
+    # dominant kernel: the one-qubit U3 pass (MAT1)
+    mat1_ms, mat1_launches = prof.get("MAT1", (0.0, 0))
+    mat1_avg_ms = mat1_ms / max(1, mat1_launches)
+    mat1_bytes = 16.0 * (1 << n)
+    achieved = mat1_bytes / (mat1_avg_ms * 1e-3) / 1e9 if mat1_avg_ms > 0 else 0.0
+    traffic = load_traffic()
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic.get("bytes_per_launch") if traffic else None,
+                "kernel": "k_mat1_v4 (U3, 2x2 butterfly)", "avg_launch_ms": round(mat1_avg_ms, 4),
+                "algorithmic_bytes_per_launch": mat1_bytes}
+
+    # FP64 reference of the same circuit (complex128 path) -> infidelity
+    ref = hip.State(n, hip.C128)
+    ref.set_basis(0)
+    ref.apply(lowered)
+    st.set_basis(0)
+    st.apply(lowered)
+    ov = st.inner(ref)
+    n64 = st.norm2()
+    infid = max(0.0, 1.0 - abs(ov) ** 2 / n64)
+    log(f"infidelity plain c64 vs c128: {infid:.3e}")
+
+    # compile(): fused 5-qubit (and smaller) blocks on the same circuit
+    fused = {}
+    for kmax in (4, 5):
+        sch = hip.Schedule(lowered, n, kmax)
+        passes, _ = sch.info()
+        st.set_basis(0)
+        st.apply_schedule(sch)
+        st.sync()
+        t0 = time.perf_counter()
+        reps = max(1, args.steps)
+        for _ in range(reps):
+            st.set_basis(0)
+            st.apply_schedule(sch)
+        st.sync()
+        fdt = (time.perf_counter() - t0) / reps
+        ovf = st.inner(ref)
+        inf_f = max(0.0, 1.0 - abs(ovf) ** 2 / st.norm2())
+        fused[f"k{kmax}"] = {"passes": passes, "ms_per_step": round(1e3 * fdt, 3),
+                             "gate_apps_per_s": round(n_gates / fdt, 1), "infidelity": inf_f,
+                             "speedup_vs_plain": round((dt / args.steps) / fdt, 3)}
+        log(f"fused k={kmax}: {fused[f'k{kmax}']}")
+        sch.close()
+    ref.close()
+    st.close()
+
+    cpu = cpu_baseline(gates, n) if not args.no_cpu else None
+    if cpu:
+        log(f"cpu baseline {cpu}")
+
+    out = {
+        "metric": "gate-apps/s + HBM GB/s at 28q c64 vs plain; served p50/p99 at <=1e-5 infidelity",
+        "value": round(gate_apps, 1),
+        "unit": "gate-apps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded brickwork circuit, angles ~ U[0, 2pi))",
+        "config": {"workload": "brickwork-28q-depth20 (U3 per qubit + CZ alternating pairs), plain per-gate",
+                   "qubits": n, "depth": DEPTH, "gates": n_gates, "state_bytes": 8 << n,
+                   "algorithmic_bytes_per_step": alg_bytes, "seed": SEED},
+        "hbm_gbs_per_step": round(alg_bytes / (ms_per_step * 1e-3) / 1e9, 1),
+        "infidelity_vs_fp64": infid,
+        "roofline": roofline,
+        "fused": fused,
+        "cpu_baseline": cpu,
+        "device": dev,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def main():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    args = p.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the state-vector path here runs on a single MI355X)", file=sys.stderr)
+        sys.exit(2)
+    run(args)
+
+
+if __name__ == "__main__":
+    main()

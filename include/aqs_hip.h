@@ -1,0 +1,197 @@
+/*
+ * aqs_hip.h -- C ABI of the MI355X state-vector engine (libaqs_hip.so).
+ *
+ * This is the device boundary of the aqs:: API.  In afQuantumSim every
+ * operation below is an ArrayFire call on an af::array holding the state;
+ * here the state is a raw HBM buffer owned by an opaque aqs_state handle and
+ * every operation is a hand-written HIP kernel for gfx950.
+ *
+ * Conventions
+ *  - Qubit q of an n-qubit state is bit n-1-q of the amplitude index
+ *    (qubit 0 is the most significant bit), as in the reference
+ *    (include/quantum.h:102-106).
+ *  - Every entry point returns an aqs_status; C++ exceptions never cross this
+ *    boundary.  The aqs:: C++ layer rethrows AQS_ERR_* as the reference's
+ *    exception types (std::invalid_argument, std::out_of_range,
+ *    std::domain_error, std::runtime_error) with aqs_last_error() as message.
+ *  - Work is enqueued on the state's HIP stream; functions that return host
+ *    data synchronise that stream.
+ *  - Matrices cross the boundary in FP64 (row-major, interleaved re/im); the
+ *    complex64 kernels round them to FP32 once, at upload.
+ */
+#ifndef AQS_HIP_H
+#define AQS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum aqs_status {
+    AQS_OK = 0,
+    AQS_ERR_INVALID_ARGUMENT = 1, /* std::invalid_argument in the reference */
+    AQS_ERR_OUT_OF_RANGE = 2,     /* std::out_of_range */
+    AQS_ERR_DOMAIN = 3,           /* std::domain_error */
+    AQS_ERR_RUNTIME = 4,          /* std::runtime_error */
+    AQS_ERR_DEVICE = 5,           /* a HIP runtime error */
+    AQS_ERR_NO_MEMORY = 6         /* device allocation failed */
+} aqs_status;
+
+typedef enum aqs_precision {
+    AQS_C64 = 0,  /* complex64: the reference's c32 state (include/quantum.h:107-111) */
+    AQS_C128 = 1  /* complex128: FP64 state, used to measure the complex64 error */
+} aqs_precision;
+
+/* Maximum qubits per state.  The reference caps at 30
+ * (include/quantum.h:122); indices here are 64-bit, so the cap is memory. */
+#define AQS_MAX_QUBITS 34
+
+/* ---- gate operations ---------------------------------------------------- */
+
+typedef enum aqs_op_kind {
+    /* dense 2x2 matrix m on targets[0]: H, RotX, RotY, U3, Y, CY, CH, CRot*  */
+    AQS_OP_MAT1 = 1,
+    /* diagonal diag(m[0..1], m[6..7]) on targets[0]: RotZ, CRotZ           */
+    AQS_OP_DIAG1 = 2,
+    /* flip targets[0]: X, CX, CCNot, Or (with AQS_CTRL_ANY)               */
+    AQS_OP_X = 3,
+    /* exchange targets[0] and targets[1]: Swap, CSwap                      */
+    AQS_OP_SWAP = 4,
+    /* dense 2^k x 2^k matrix u, k = ntargets (1..5): fused blocks,
+     * Gate / ControlGate.  targets[0] is the most significant bit of the
+     * row/column index of u.                                               */
+    AQS_OP_MATK = 5,
+    /* multiply by the phase m[0..1] where every target and control bit is 1:
+     * Z, Phase, CZ, CPhase                                                  */
+    AQS_OP_PHASE = 6
+} aqs_op_kind;
+
+/* Control semantics: default = all control bits must be 1.  AQS_CTRL_ANY =
+ * at least one control bit is 1 (the reference's Or gate,
+ * src/quantum.cpp:1614-1653). */
+#define AQS_CTRL_ANY 1
+
+typedef struct aqs_op {
+    int32_t kind;        /* aqs_op_kind */
+    int32_t ntargets;    /* 1 (MAT1/DIAG1/X/PHASE), 2 (SWAP), 1..5 (MATK); PHASE may list up to 5 */
+    int32_t targets[5];  /* qubit indices */
+    int32_t ncontrols;   /* 0..3 */
+    int32_t controls[3];
+    int32_t flags;       /* AQS_CTRL_ANY */
+    double m[8];         /* 2x2 complex, row-major, interleaved */
+    const double* u;     /* MATK: 2^k x 2^k complex, row-major, interleaved */
+} aqs_op;
+
+typedef struct aqs_state_s* aqs_state;
+typedef struct aqs_schedule_s* aqs_schedule;
+
+/* ---- library ------------------------------------------------------------ */
+
+/* Last error message of the calling thread ("" when none). */
+const char* aqs_last_error(void);
+
+/* Select the device (replaces aqs::initialize's af::setDevice,
+ * src/quantum.cpp:69-86).  Idempotent. */
+aqs_status aqs_init(int device);
+
+/* Device name, compute-unit count and HBM bytes. */
+aqs_status aqs_device_info(char* name, size_t name_len, int* cu_count, size_t* hbm_bytes);
+
+/* ---- state lifetime (QSimulator ctors / copy, src/quantum.cpp:212-259) -- */
+
+aqs_status aqs_state_create(int nqubits, int precision, aqs_state* out);
+aqs_status aqs_state_destroy(aqs_state s);
+/* dst <- src (same qubit count; precisions may differ: converts). */
+aqs_status aqs_state_copy(aqs_state dst, aqs_state src);
+aqs_status aqs_state_qubits(aqs_state s, int* nqubits);
+aqs_status aqs_state_precision(aqs_state s, int* precision);
+/* Raw device pointer to the 2^n amplitudes (interleaved re, im). */
+aqs_status aqs_state_device_ptr(aqs_state s, void** ptr);
+/* Use an external stream (hipStream_t) for this state's work; NULL restores
+ * the library's stream. */
+aqs_status aqs_state_set_stream(aqs_state s, void* hip_stream);
+aqs_status aqs_state_sync(aqs_state s);
+
+/* |index> (the |0..0> / |1..1> fast paths, src/quantum.cpp:219-224). */
+aqs_status aqs_state_set_basis(aqs_state s, uint64_t index);
+/* Kronecker product of per-qubit 2-vectors, qubit 0 first
+ * (generate_statevector, src/quantum.cpp:261-275).  qubit_amps: n x 2
+ * complex, interleaved; evaluated in FP64. */
+aqs_status aqs_state_product(aqs_state s, const double* qubit_amps);
+/* Seeded Gaussian state (re, im ~ N(0,1), counter-based), normalised in FP64. */
+aqs_status aqs_state_gaussian(aqs_state s, uint64_t seed);
+/* Host -> device in the state's precision; normalize != 0 divides by the
+ * FP64 norm (the af::array ctor, src/quantum.cpp:242-259); a null vector is
+ * AQS_ERR_INVALID_ARGUMENT. */
+aqs_status aqs_state_upload(aqs_state s, const void* host, int normalize);
+/* Device -> host in the state's precision (statevector().host()). */
+aqs_status aqs_state_download(aqs_state s, void* host);
+
+/* ---- gate application (QSimulator::simulate, src/quantum.cpp:277-291) --- */
+
+/* Apply ops in order, one HIP kernel pass each. */
+aqs_status aqs_apply(aqs_state s, const aqs_op* ops, size_t count);
+
+/* ---- compile(): gate fusion (replaces QCircuit::compile, src/quantum.cpp:199-210) */
+
+/* Fuse ops into blocks of at most max_qubits (1..5) qubits.  Each block is
+ * a dense unitary built in FP64; runs of diagonal ops become phase passes. */
+aqs_status aqs_fuse(const aqs_op* ops, size_t count, int nqubits, int max_qubits,
+                    aqs_schedule* out);
+aqs_status aqs_schedule_destroy(aqs_schedule sch);
+/* Number of fused passes and the ops each came from. */
+aqs_status aqs_schedule_info(aqs_schedule sch, size_t* passes, size_t* source_ops);
+/* The i-th fused pass as an op (u stays owned by the schedule). */
+aqs_status aqs_schedule_get(aqs_schedule sch, size_t i, aqs_op* op);
+aqs_status aqs_apply_schedule(aqs_state s, aqs_schedule sch);
+
+/* ---- measurement (src/quantum.cpp:293-531) ------------------------------ */
+
+/* sum |a|^2 in FP64. */
+aqs_status aqs_norm2(aqs_state s, double* out);
+/* Divide by the FP64 norm. */
+aqs_status aqs_normalize(aqs_state s);
+/* P(qubit = 1) in FP64 (qubit_probability_true, src/quantum.cpp:372-391). */
+aqs_status aqs_prob1(aqs_state s, int qubit, double* out);
+/* P(q = 1) for every qubit q, one pass. out: n doubles. */
+aqs_status aqs_prob1_all(aqs_state s, double* out);
+/* <sum_q Z_q> in FP64. */
+aqs_status aqs_expect_sum_z(aqs_state s, double* out);
+/* |a_i|^2 as float for all i (probabilities(), src/quantum.cpp:404-414). out: host, 2^n floats. */
+aqs_status aqs_probabilities(aqs_state s, float* out);
+/* One amplitude (state(i), include/quantum.h:645-648). re_im: 2 doubles. */
+aqs_status aqs_amplitude(aqs_state s, uint64_t index, double* re_im);
+/* For each uniform u[k] in [0,1), the first index whose inclusive cumulative
+ * probability exceeds u[k] (peek_measure_all / profile_measure_all rule,
+ * src/quantum.cpp:344-359,467-501).  |a|^2 is formed in FP64 and summed
+ * exactly in 64-bit fixed point (quantum 2^-60), so the result does not
+ * depend on reduction order.  u at or above the total probability yields
+ * the last index of non-zero probability. */
+aqs_status aqs_sample(aqs_state s, const double* u, uint32_t shots, uint64_t* out_index);
+/* Measure one qubit with uniform u (outcome = u < P(1)) and collapse:
+ * the other half is zeroed and the kept half divided by sqrt(P)
+ * (measure(q), src/quantum.cpp:308-342). */
+aqs_status aqs_measure_qubit(aqs_state s, int qubit, double u, int* outcome, double* p1);
+/* Collapse to |index> (measure_all, src/quantum.cpp:361-370). */
+aqs_status aqs_collapse_to(aqs_state s, uint64_t index);
+/* <a|b> in FP64 for two states of equal qubit count (any precisions). */
+aqs_status aqs_inner(aqs_state a, aqs_state b, double* re_im);
+
+/* ---- timing ------------------------------------------------------------- */
+
+/* Per-kind kernel timing with HIP events on the state's stream.  While
+ * enabled every gate kernel launch is bracketed by an event pair. */
+aqs_status aqs_profile_enable(aqs_state s, int enable);
+/* kinds: AQS_OP_* index 0..7 -> total ms and launch count since enable. */
+aqs_status aqs_profile_read(aqs_state s, double* ms_by_kind, uint64_t* launches_by_kind);
+/* Event pair on the state's stream around arbitrary work. */
+aqs_status aqs_timer_start(aqs_state s);
+aqs_status aqs_timer_stop(aqs_state s, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AQS_HIP_H */

@@ -1,0 +1,363 @@
+"""Parity of the HIP path (called through the C ABI) with the FP64 oracle.
+
+Tolerances: complex64 states agree with the FP64 oracle to 2e-6 absolute
+per amplitude after a few gates (FP32 rounding of matrices and products),
+complex128 states to 1e-12; permutation circuits are bit-exact; sampled
+indices are bit-exact given the same amplitudes and uniforms.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from afquantumsim_amd import circuits, hip, ops
+from tests.helpers import (ALL_KINDS_4Q, cfequal, fixture_gates, init_qamps, kind_gate, load_kat,
+                           random_state)
+
+pytestmark = pytest.mark.gpu
+
+KAT = load_kat()
+TOL64 = 2e-6
+TOL128 = 1e-12
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    hip.init(0)
+    yield
+
+
+def gpu_run(n, sv, gates, precision=hip.C64, fused=None):
+    st = hip.State(n, precision)
+    st.upload(sv.astype(st.dtype))
+    lowered = ops.lower_all(gates)
+    if fused:
+        sch = hip.Schedule(lowered, n, fused)
+        st.apply_schedule(sch)
+    else:
+        st.apply(lowered)
+    out = st.download()
+    st.close()
+    return out
+
+
+@pytest.mark.parametrize("precision", [hip.C64, hip.C128], ids=["c64", "c128"])
+@pytest.mark.parametrize("case", KAT["gate_cases"], ids=lambda c: c["name"])
+def test_kat_gates_gpu(case, precision):
+    """test/tests.cpp:414-907 known answers, state built by aqs_state_product."""
+    n = case["n"]
+    st = hip.State(n, precision)
+    st.product(init_qamps(case["init"]))
+    st.apply(ops.lower_all(fixture_gates(case["gates"], n)))
+    for idx, re, im in case["expect"]:
+        got = st.amplitude(idx)
+        if case["tol"] == 0:
+            assert got == complex(re, im), (idx, got)
+        else:
+            assert cfequal(got, complex(re, im), case["tol"]), (idx, got, re, im)
+
+
+@pytest.mark.parametrize("eq", KAT["equalities"], ids=lambda c: c["name"])
+def test_kat_circuit_equalities_gpu(eq):
+    """test/tests.cpp:909-961 on the device: ControlGate/Gate (MATK passes)
+    equal their flattened circuits column by column."""
+    n = eq["n"]
+    lhs = fixture_gates(eq["lhs"], n)
+    rhs = fixture_gates(eq["rhs"], n)
+    for j in range(1 << n):
+        e = oracle.basis_state(n, j)
+        a = gpu_run(n, e, lhs)
+        b = gpu_run(n, e, rhs)
+        np.testing.assert_allclose(a, b, atol=TOL64)
+
+
+@pytest.mark.parametrize("case", KAT["ch_cases"], ids=lambda c: c["name"])
+def test_kat_ch_gpu(case):
+    n = case["n"]
+    st = hip.State(n, hip.C64)
+    st.product(init_qamps(case["init"]))
+    st.apply(ops.lower_all(fixture_gates(case["gates"], n)))
+    ref = oracle.product_state(n, init_qamps(case["expect_product"]))
+    np.testing.assert_allclose(st.download(), ref, atol=1e-6)
+
+
+@pytest.mark.parametrize("case", KAT["prob_cases"], ids=lambda c: c["name"])
+def test_kat_probabilities_gpu(case):
+    n = case["n"]
+    st = hip.State(n, hip.C64)
+    st.product(init_qamps(case["init"]))
+    for q, p in enumerate(case["p1"]):
+        got = st.prob1(q)
+        if case["tol"] == 0:
+            assert got == p
+        else:
+            assert abs(got - p) <= max(abs(got), p) * case["tol"]
+
+
+@pytest.mark.parametrize("case", KAT["measure_cases"], ids=lambda c: c["name"])
+def test_kat_peek_measure_all_gpu(case):
+    n = case["n"]
+    st = hip.State(n, hip.C64)
+    st.product(init_qamps(case["init"]))
+    u = np.array([0.0, 0.25, 0.5, 0.999999])
+    assert (st.sample(u) == case["peek_all"]).all()
+
+
+def _all_position_gates(n, rng):
+    """Every gate kind at random positions, including bit 0 targets/controls."""
+    out = []
+    for spec in ALL_KINDS_4Q:
+        g = kind_gate(spec)
+        k = spec[0]
+        arity = len(spec[1])
+        for _ in range(3):
+            q = rng.choice(n, size=arity, replace=False)
+            q = tuple(int(x) for x in q) + (0,) * (3 - arity)
+            if rng.random() < 0.3:
+                # put something on the least significant bit (qubit n-1)
+                q = (n - 1,) + q[1:] if arity >= 1 else q
+                if len(set(q[:arity])) < arity:
+                    continue
+            g2 = dict(g)
+            g2["q"] = q
+            out.append(g2)
+    return out
+
+
+@pytest.mark.parametrize("n", [3, 5, 8, 13])
+@pytest.mark.parametrize("precision", [hip.C64, hip.C128], ids=["c64", "c128"])
+def test_every_kind_every_position(n, precision):
+    rng = np.random.default_rng(100 + n)
+    gates = _all_position_gates(n, rng)
+    if n < 3:
+        gates = [g for g in gates if g["kind"] not in ("CSwap", "CCX", "Or")]
+    sv = random_state(n, 7 + n)
+    for g in gates:
+        if max(g["q"][: {"Swap": 2, "CX": 2, "CY": 2, "CZ": 2, "CH": 2, "CPhase": 2, "CRotX": 2,
+                          "CRotY": 2, "CRotZ": 2, "CSwap": 3, "CCX": 3, "Or": 3}.get(g["kind"], 1)]) >= n:
+            continue
+        want = oracle.run(sv, n, [g])
+        got = gpu_run(n, sv, [g], precision)
+        tol = TOL64 if precision == hip.C64 else TOL128
+        np.testing.assert_allclose(got, want, atol=tol, err_msg=str(g))
+
+
+@pytest.mark.parametrize("n", [4, 9, 14])
+def test_permutation_circuits_bit_exact(n):
+    """X / CX / CCX / Swap / CSwap / Or only: the complex64 result is the
+    exact permutation of the complex64 input."""
+    rng = np.random.default_rng(31 + n)
+    gates = []
+    for _ in range(60):
+        kind = str(rng.choice(["X", "CX", "CCX", "Swap", "CSwap", "Or"]))
+        q = rng.choice(n, size=3, replace=False)
+        gates.append(oracle.gate(kind, int(q[0]), int(q[1]), int(q[2])))
+    sv = random_state(n, 5).astype(np.complex64)
+    want = oracle.run(sv.astype(np.complex128), n, gates).astype(np.complex64)
+    got = gpu_run(n, sv.astype(np.complex128), gates)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+@pytest.mark.parametrize("n", [10, 16])
+def test_random_layered_circuit(n):
+    """The 20-qubit workload's gate vocabulary at smaller n, FP64 oracle."""
+    gates = circuits.random_layered(n, 12, seed=n)
+    sv = oracle.basis_state(n, 0)
+    want = oracle.run(sv, n, gates)
+    got = gpu_run(n, sv, gates)
+    assert oracle.infidelity(got, want) < 1e-9
+    got128 = gpu_run(n, sv, gates, hip.C128)
+    assert oracle.infidelity(got128, want) < 1e-20
+    np.testing.assert_allclose(got128, want, atol=1e-12)
+
+
+@pytest.mark.parametrize("kmax", [2, 3, 4, 5])
+@pytest.mark.parametrize("n", [9, 14])
+def test_fused_brickwork(n, kmax):
+    """compile() fusion: fused schedule == per-gate oracle."""
+    gates = circuits.brickwork(n, 6, seed=3)
+    sv = oracle.basis_state(n, 0)
+    want = oracle.run(sv, n, gates)
+    got = gpu_run(n, sv, gates, fused=kmax)
+    assert oracle.infidelity(got, want) < 1e-9
+    np.testing.assert_allclose(got, want, atol=5e-6)
+    got128 = gpu_run(n, sv, gates, hip.C128, fused=kmax)
+    np.testing.assert_allclose(got128, want, atol=1e-12)
+
+
+def _haar(k, rng):
+    z = (rng.normal(size=(1 << k, 1 << k)) + 1j * rng.normal(size=(1 << k, 1 << k))) / math.sqrt(2)
+    q, r = np.linalg.qr(z)
+    return q * (np.diag(r) / np.abs(np.diag(r)))
+
+
+@pytest.mark.parametrize("k", [2, 3, 4, 5])
+@pytest.mark.parametrize("begin", [0, 3, "low"])
+def test_matk_blocks(k, begin):
+    """Dense k-qubit blocks (MFMA for k = 4, 5) at high, middle and lowest bits."""
+    n = 12
+    rng = np.random.default_rng(k * 10 + (7 if begin == "low" else begin))
+    U = _haar(k, rng)
+    b = n - k if begin == "low" else begin
+    g = oracle.gate("Circuit", b, k=k, u=U)
+    sv = random_state(n, 3)
+    want = oracle.run(sv, n, [g])
+    got = gpu_run(n, sv, [g])
+    np.testing.assert_allclose(got, want, atol=TOL64)
+    gc = oracle.gate("ControlCircuit", (b - 1) % n if b > 0 else n - 1, b, k=k, u=U)
+    if gc["q"][0] >= b + k or gc["q"][0] < b:
+        want = oracle.run(sv, n, [gc])
+        np.testing.assert_allclose(gpu_run(n, sv, [gc]), want, atol=TOL64)
+
+
+@pytest.mark.parametrize("k", [4, 5])
+def test_mfma_scattered_targets(k):
+    """MFMA block on non-contiguous target bits, through the fuse path."""
+    n = 13
+    rng = np.random.default_rng(k)
+    targets = sorted(int(x) for x in rng.choice(n, size=k, replace=False))
+    U = _haar(k, rng)
+    op = {"kind": hip.OP_MATK, "targets": tuple(targets), "u": U}
+    sv = random_state(n, 11)
+    # oracle: embed via a permutation to contiguous qubits
+    st = hip.State(n, hip.C128)
+    st.upload(sv)
+    st.apply([op])
+    want = st.download()
+    st64 = hip.State(n, hip.C64)
+    st64.upload(sv.astype(np.complex64))
+    st64.apply([op])
+    np.testing.assert_allclose(st64.download(), want, atol=TOL64)
+    # independent check of the complex128 path against explicit algebra
+    idx = np.arange(1 << n)
+    ref = np.zeros_like(sv)
+    pos = [n - 1 - t for t in targets]
+    sub = np.zeros(1 << n, dtype=np.int64)
+    for j, p in enumerate(pos):
+        sub |= ((idx >> p) & 1) << (k - 1 - j)
+    rest = idx.copy()
+    for p in pos:
+        rest &= ~(1 << p)
+    for r in range(1 << k):
+        off = 0
+        for j, p in enumerate(pos):
+            if (r >> (k - 1 - j)) & 1:
+                off |= 1 << p
+        mask = sub == r
+        src = rest[mask]
+        acc = np.zeros(src.size, dtype=np.complex128)
+        for c in range(1 << k):
+            offc = 0
+            for j, p in enumerate(pos):
+                if (c >> (k - 1 - j)) & 1:
+                    offc |= 1 << p
+            acc += U[r, c] * sv[src | offc]
+        ref[src | off] = acc
+    np.testing.assert_allclose(want, ref, atol=1e-12)
+
+
+@pytest.mark.parametrize("n", [1, 3, 10, 17])
+def test_sampling_bit_exact(n):
+    rng = np.random.default_rng(n)
+    sv = random_state(n, 40 + n).astype(np.complex64)
+    st = hip.State(n, hip.C64)
+    st.upload(sv)
+    u = np.concatenate([rng.uniform(size=4000), [0.0, 1.0 - 2 ** -53, 1.0, 2.0]])
+    got = st.sample(u)
+    want = oracle.sample_c64(st.download(), n, u)
+    assert np.array_equal(got.astype(np.uint32), want)
+
+
+def test_sampling_sparse_and_clamp():
+    n = 12
+    sv = np.zeros(1 << n, dtype=np.complex64)
+    sv[5] = 0.6
+    sv[3000] = 0.8j
+    st = hip.State(n, hip.C64)
+    st.upload(sv)
+    u = np.array([0.0, 0.3599999, 0.3600001, 0.999, 1.0, 5.0])
+    assert list(st.sample(u)) == [5, 5, 3000, 3000, 3000, 3000]
+
+
+def test_measurement_reductions():
+    n = 11
+    sv = random_state(n, 77)
+    st = hip.State(n, hip.C128)
+    st.upload(sv)
+    assert abs(st.norm2() - oracle.norm2(sv, n)) < 1e-13
+    for q in range(n):
+        assert abs(st.prob1(q) - oracle.prob1(sv, n, q)) < 1e-13
+    np.testing.assert_allclose(st.prob1_all(), [oracle.prob1(sv, n, q) for q in range(n)], atol=1e-13)
+    assert abs(st.expect_sum_z() - oracle.expect_sum_z(sv, n)) < 1e-12
+    p = st.probabilities()
+    np.testing.assert_allclose(p, np.abs(sv) ** 2, rtol=1e-6, atol=1e-12)
+
+
+def test_measure_qubit_collapse():
+    n = 6
+    sv = random_state(n, 12)
+    for q in (0, 3, 5):
+        for u in (0.01, 0.99):
+            st = hip.State(n, hip.C128)
+            st.upload(sv)
+            outcome, p1 = st.measure_qubit(q, u)
+            assert abs(p1 - oracle.prob1(sv, n, q)) < 1e-13
+            assert outcome == int(u < p1)
+            m = 1 << (n - 1 - q)
+            idx = np.arange(1 << n)
+            keep = ((idx & m) != 0) == bool(outcome)
+            ref = np.where(keep, sv, 0) / math.sqrt(p1 if outcome else 1 - p1)
+            np.testing.assert_allclose(st.download(), ref, atol=1e-13)
+
+
+def test_state_preparation():
+    n = 9
+    qa = np.stack([np.array([math.cos(0.1 * q), np.exp(1j * q) * math.sin(0.1 * q)]) for q in range(n)])
+    st = hip.State(n, hip.C128)
+    st.product(qa)
+    np.testing.assert_allclose(st.download(), oracle.product_state(n, qa), atol=1e-14)
+    st.gaussian(2024)
+    np.testing.assert_allclose(st.download(), oracle.gaussian_state(n, 2024), atol=1e-13)
+    st.set_basis(17)
+    e = st.download()
+    assert e[17] == 1 and np.count_nonzero(e) == 1
+    with pytest.raises(hip.InvalidArgument):
+        st.upload(np.zeros(1 << n), normalize=True)
+
+
+def test_errors_match_reference_types():
+    st = hip.State(4, hip.C64)
+    with pytest.raises(hip.OutOfRange):
+        st.apply([{"kind": hip.OP_X, "targets": (4,)}])
+    with pytest.raises(hip.InvalidArgument):
+        st.apply([{"kind": hip.OP_X, "targets": (1,), "controls": (1,)}])
+    with pytest.raises(hip.InvalidArgument):
+        st.apply([{"kind": hip.OP_SWAP, "targets": (2, 2)}])
+    with pytest.raises(hip.OutOfRange):
+        st.prob1(9)
+
+
+@pytest.mark.parametrize("n", [26])
+def test_large_state_round_trip(n):
+    """Size-independent properties at a large state: a brickwork layer
+    followed by its inverse restores the input; permutations are exact."""
+    st = hip.State(n, hip.C64)
+    st.gaussian(5)
+    ref = st.download()
+    gates = circuits.brickwork(n, 2, seed=9)
+    inv = []
+    for g in reversed(gates):
+        if g["kind"] == "U3":
+            inv.append(circuits.gate("U3", g["q"][0], angle=-g["angle"], phi=-g["lam"], lam=-g["phi"]))
+        else:
+            inv.append(g)
+    st.apply(ops.lower_all(gates))
+    st.apply(ops.lower_all(inv))
+    out = st.download()
+    assert oracle.infidelity(out, ref) < 1e-9
+    perm = [circuits.gate("CX", 0, n - 1), circuits.gate("Swap", 3, n - 2), circuits.gate("X", n - 1),
+            circuits.gate("CCX", 1, 2, 20)]
+    st.apply(ops.lower_all(perm))
+    st.apply(ops.lower_all(list(reversed(perm))))
+    assert np.array_equal(st.download().view(np.uint32), out.view(np.uint32))

@@ -76,11 +76,27 @@ def build_cpp() -> str:
     return AQS_LIB
 
 
+CPP_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "aqs_tests.cpp")
+CPP_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "aqs_tests")
+
+
+def build_cpp_tests() -> str:
+    """The C++ suite mirroring the reference's test/tests.cpp (runs on a GPU)."""
+    if not os.path.exists(CPP_TEST_SRC):
+        return ""
+    tmp = CPP_TEST_BIN + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", INCLUDE, "-o", tmp, CPP_TEST_SRC, "-L", PKG, "-laqs",
+          "-laqs_hip", "-Wl,-rpath,$ORIGIN/../../afquantumsim_amd"])
+    os.replace(tmp, CPP_TEST_BIN)
+    return CPP_TEST_BIN
+
+
 def build_all() -> None:
     if not os.path.exists(HIPCC) and not shutil.which("hipcc"):
         raise RuntimeError("hipcc not found")
     build_hip()
     build_cpp()
+    build_cpp_tests()
 
 
 if __name__ == "__main__":

@@ -62,7 +62,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const T2* __restrict__ a, uin
         const double p = prob_of(a[i]);
         if (MODE == 0) acc += p;
         if (MODE == 1) acc += (i & mask) ? p : 0.0;
-        if (MODE == 2) acc += p * (double)(n - 2 * __popcll(i));
+        if (MODE == 2) acc += p * (double)(n - 2 * (int)__popcll(i));
     }
     double r = block_reduce_sum(acc, sh);
     if (threadIdx.x == 0) partial[blockIdx.x] = r;

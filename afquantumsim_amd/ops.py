@@ -1,11 +1,16 @@
 """Lowering of the reference's gate classes to aqs_op passes.
 
 Each reference gate (include/quantum.h:857-1589) becomes one op of the C ABI
-(include/aqs_hip.h).  Matrix entries are computed the way the reference
-computes them -- float32 cos/sin of the float angle (e.g. RotX,
-src/quantum.cpp:683-688) and the float constant 0.70710678118f for H
-(src/quantum.cpp:46-49) -- and carried as doubles across the boundary.
-U3 has no reference counterpart; its matrix is built in FP64.
+(include/aqs_hip.h).  Two matrix flavours:
+
+* ``exact=False`` (default, the complex64 product path): entries computed
+  the way the reference computes them -- float32 cos/sin of the float angle
+  (e.g. RotX, src/quantum.cpp:683-688) and the float constant 0.70710678118f
+  for H (src/quantum.cpp:46-49);
+* ``exact=True`` (the complex128 reference path): FP64 trigonometry of the
+  same float angles and 1/sqrt(2) in double, i.e. the exact operator.
+
+U3 has no reference counterpart; its matrix is built in FP64 in both cases.
 """
 from __future__ import annotations
 
@@ -17,14 +22,21 @@ from . import hip
 
 F = np.float32
 H_F = float(F(0.70710678118))
+H_D = 0.70710678118654752440
 
 
-def _cs_half(angle):
+def _cs_half(angle, exact):
+    if exact:
+        a = float(F(angle)) / 2.0
+        return math.cos(a), math.sin(a)
     a = F(angle) / F(2.0)
     return float(np.cos(a)), float(np.sin(a))
 
 
-def _eiphi(angle):
+def _eiphi(angle, exact):
+    if exact:
+        a = float(F(angle))
+        return complex(math.cos(a), math.sin(a))
     a = F(angle)
     return complex(float(np.cos(a)), float(np.sin(a)))
 
@@ -47,18 +59,18 @@ def _x(t, controls=(), flags=0):
     return {"kind": hip.OP_X, "targets": (t,), "controls": tuple(controls), "flags": flags}
 
 
-def matrix_rotx(angle):
-    c, s = _cs_half(angle)
+def matrix_rotx(angle, exact=False):
+    c, s = _cs_half(angle, exact)
     return [c, -1j * s, -1j * s, c]
 
 
-def matrix_roty(angle):
-    c, s = _cs_half(angle)
+def matrix_roty(angle, exact=False):
+    c, s = _cs_half(angle, exact)
     return [c, -s, s, c]
 
 
-def matrix_rotz_diag(angle):
-    c, s = _cs_half(angle)
+def matrix_rotz_diag(angle, exact=False):
+    c, s = _cs_half(angle, exact)
     return complex(c, -s), complex(c, s)
 
 
@@ -69,11 +81,15 @@ def matrix_u3(theta, phi, lam):
             complex(math.cos(p + l), math.sin(p + l)) * c]
 
 
-H_MAT = [H_F, H_F, H_F, -H_F]
+def matrix_h(exact=False):
+    h = H_D if exact else H_F
+    return [h, h, h, -h]
+
+
 Y_MAT = [0, -1j, 1j, 0]
 
 
-def lower(g) -> list:
+def lower(g, exact: bool = False) -> list:
     """One gate dict (see afquantumsim_amd.circuits.gate) -> list of ops."""
     k = g["kind"]
     q0, q1, q2 = g["q"]
@@ -87,17 +103,17 @@ def lower(g) -> list:
     if k == "Z":
         return [_phase((q0,), -1.0)]
     if k == "H":
-        return [_mat1(q0, H_MAT)]
+        return [_mat1(q0, matrix_h(exact))]
     if k == "Phase":
-        return [_phase((q0,), _eiphi(a))]
+        return [_phase((q0,), _eiphi(a, exact))]
     if k == "Swap":
         return [{"kind": hip.OP_SWAP, "targets": (q0, q1)}]
     if k == "RotX":
-        return [_mat1(q0, matrix_rotx(a))]
+        return [_mat1(q0, matrix_rotx(a, exact))]
     if k == "RotY":
-        return [_mat1(q0, matrix_roty(a))]
+        return [_mat1(q0, matrix_roty(a, exact))]
     if k == "RotZ":
-        return [_diag1(q0, *matrix_rotz_diag(a))]
+        return [_diag1(q0, *matrix_rotz_diag(a, exact))]
     if k == "U3":
         return [_mat1(q0, matrix_u3(a, g.get("phi", 0.0), g.get("lam", 0.0)))]
     if k == "CX":
@@ -107,17 +123,17 @@ def lower(g) -> list:
     if k == "CZ":
         return [_phase((q1,), -1.0, (q0,))]
     if k == "CH":
-        return [_mat1(q1, H_MAT, (q0,))]
+        return [_mat1(q1, matrix_h(exact), (q0,))]
     if k == "CPhase":
-        return [_phase((q1,), _eiphi(a), (q0,))]
+        return [_phase((q1,), _eiphi(a, exact), (q0,))]
     if k == "CSwap":
         return [{"kind": hip.OP_SWAP, "targets": (q1, q2), "controls": (q0,)}]
     if k == "CRotX":
-        return [_mat1(q1, matrix_rotx(a), (q0,))]
+        return [_mat1(q1, matrix_rotx(a, exact), (q0,))]
     if k == "CRotY":
-        return [_mat1(q1, matrix_roty(a), (q0,))]
+        return [_mat1(q1, matrix_roty(a, exact), (q0,))]
     if k == "CRotZ":
-        return [_diag1(q1, *matrix_rotz_diag(a), controls=(q0,))]
+        return [_diag1(q1, *matrix_rotz_diag(a, exact), controls=(q0,))]
     if k == "CCX":
         ctl = (q0,) if q0 == q1 else (q0, q1)
         return [_x(q2, ctl)]
@@ -133,10 +149,10 @@ def lower(g) -> list:
     raise ValueError(f"unknown gate kind {k}")
 
 
-def lower_all(gates) -> list:
+def lower_all(gates, exact: bool = False) -> list:
     out = []
     for g in gates:
-        out.extend(lower(g))
+        out.extend(lower(g, exact))
     return out
 
 

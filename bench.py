@@ -78,6 +78,33 @@ def load_traffic():
     return None
 
 
+def probe(args):
+    """Short fixed sequence for counter collection: 3 plain copies of the
+    2 GiB state (calibration), 3 U3 passes, 3 CZ passes, 3 fused 5-qubit
+    passes, each kind in order, on the 28-qubit state."""
+    hip.init(0)
+    n = N_QUBITS
+    a = hip.State(n, hip.C64)
+    b = hip.State(n, hip.C64)
+    a.gaussian(1)
+    gates = circuits.brickwork(n, 2, seed=SEED)
+    lowered = ops.lower_all(gates)
+    u3 = [o for o in lowered if o["kind"] == hip.OP_MAT1][:3]
+    cz = [o for o in lowered if o["kind"] == hip.OP_PHASE][:3]
+    sch = hip.Schedule(lowered, n, 5)
+    for _ in range(3):
+        b.copy_from(a)
+    b.sync()
+    a.apply(u3)
+    a.sync()
+    a.apply(cz)
+    a.sync()
+    for _ in range(1):
+        a.apply_schedule(sch)
+    a.sync()
+    log(f"probe done: copy x3 ({8 << n} B each), U3 x3, CZ x3, fused schedule ({sch.info()[0]} passes)")
+
+
 def run(args):
     hip.init(0)
     dev = hip.device_info()
@@ -127,7 +154,7 @@ def run(args):
     # FP64 reference of the same circuit (complex128 path) -> infidelity
     ref = hip.State(n, hip.C128)
     ref.set_basis(0)
-    ref.apply(lowered)
+    ref.apply(ops.lower_all(gates, exact=True))
     st.set_basis(0)
     st.apply(lowered)
     ov = st.inner(ref)
@@ -196,7 +223,12 @@ def main():
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    p.add_argument("--probe", action="store_true",
+                   help="run a short fixed kernel sequence for hardware-counter collection and exit")
     args = p.parse_args()
+    if args.probe:
+        probe(args)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the state-vector path here runs on a single MI355X)", file=sys.stderr)

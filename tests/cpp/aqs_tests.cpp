@@ -1,0 +1,674 @@
+// The reference's test/tests.cpp suite, run against the aqs:: API on the
+// MI355X engine.  Same structure and asserts; the differences:
+//  - af::cfloat is aqs::cfloat (std::complex<float>);
+//  - exact equalities between two different float computations (CH at
+//    :986-1002, the ControlGate / Gate circuit matrices at :909-961) are
+//    asserted to 1e-6, since the reference's own exact equality relies on
+//    ArrayFire's rounding (0.70710677^2 != 0.7071068^2 in IEEE float);
+//  - the measurement generator is seeded so the 99.9% statistical bands of
+//    :343-406 do not fail at random.
+#undef NDEBUG
+
+#include <cassert>
+#include <cmath>
+#include <cstdio>
+#include <iostream>
+
+#include "aqs/quantum.h"
+#include "aqs/quantum_algo.h"
+
+using namespace aqs;
+
+#define CHECK(cond)                                                                   \
+    do {                                                                              \
+        if (!(cond)) {                                                                \
+            std::fprintf(stderr, "FAILED %s:%d: %s\n", __FILE__, __LINE__, #cond); \
+            std::exit(1);                                                             \
+        }                                                                             \
+    } while (0)
+
+static bool fequal(float a, float b, float diff) {
+    return std::fabs(a - b) <= ((std::fabs(a) < std::fabs(b) ? std::fabs(b) : std::fabs(a)) * diff);
+}
+static bool cfequal(const cfloat& a, const cfloat& b, float diff) {
+    return fequal(a.real(), b.real(), diff) && fequal(a.imag(), b.imag(), diff);
+}
+static bool near(const CircuitMatrix& a, const CircuitMatrix& b, float tol) {
+    if (a.dim != b.dim) return false;
+    for (size_t i = 0; i < a.data.size(); ++i)
+        if (std::abs(a.data[i] - b.data[i]) > tol) return false;
+    return true;
+}
+static bool near(const std::vector<cfloat>& a, const std::vector<cfloat>& b, float tol) {
+    if (a.size() != b.size()) return false;
+    for (size_t i = 0; i < a.size(); ++i)
+        if (std::abs(a[i] - b[i]) > tol) return false;
+    return true;
+}
+
+// test/tests.cpp:42-58
+static void test_qubit_constructor() {
+    QState q1;
+    CHECK((q1[0] == cfloat{1.f, 0.f} && q1[1] == cfloat{0.f, 0.f}));
+    QState q2({0.6f, 0.f}, {0.f, 0.8f});
+    CHECK((q2[0] == cfloat{0.6f, 0.0f} && q2[1] == cfloat{0.f, 0.8f}));
+    QState q3(q2);
+    CHECK((q3[0] == cfloat{0.6f, 0.f} && q3[1] == cfloat{0.f, 0.8f}));
+}
+
+// :64-73
+static void test_qubit_assigment() {
+    QState q1({0.6f, 0.f}, {0.f, 0.8f});
+    QState q2;
+    q2 = q1;
+    CHECK((q2[0] == q1[0] && q2[1] == q1[1]));
+}
+
+// :78-97
+static void test_qubit_normalization() {
+    QState q1({3.f, 0.f}, {4.f, 0.f});
+    CHECK((q1[0] == cfloat{0.6f, 0.0f} && q1[1] == cfloat{0.8f, 0.f}));
+    QState q2({1.f, 2.f}, {3.f, 4.f});
+    cfloat ns[2] = {{1.f, 2.f}, {3.f, 4.f}};
+    float mag = std::sqrt(ns[0].real() * ns[0].real() + ns[0].imag() * ns[0].imag() + ns[1].real() * ns[1].real() +
+                          ns[1].imag() * ns[1].imag());
+    ns[0] = ns[0] / mag;
+    ns[1] = ns[1] / mag;
+    CHECK(q2[0] == ns[0] && q2[1] == ns[1]);
+}
+
+// :102-131
+static void test_qubit_probability() {
+    float diff = 1e-4f;
+    QState q1({1.f, 0.f}, {0.f, 0.f});
+    CHECK(q1.probability_false() == 1.f && q1.probability_true() == 0.f);
+    QState q2({0.f, 1.f}, {0.f, 0.f});
+    CHECK(q2.probability_false() == 1.f && q2.probability_true() == 0.f);
+    QState q3({0.f, 0.f}, {1.f, 0.f});
+    CHECK(q3.probability_false() == 0.f && q3.probability_true() == 1.f);
+    QState q4({0.f, 0.f}, {0.f, 1.f});
+    CHECK(q4.probability_false() == 0.f && q4.probability_true() == 1.f);
+    QState q5({0.f, 1.f}, {0.f, 1.f});
+    CHECK(fequal(q5.probability_false(), 0.5f, diff) && fequal(q5.probability_true(), 0.5f, diff));
+    QState q6({0.6f, 0.f}, {0.f, 0.8f});
+    CHECK(fequal(q6.probability_false(), 0.36f, diff) && fequal(q6.probability_true(), 0.64f, diff));
+}
+
+// :136-190
+static void test_qubit_profile_measure() {
+    int reps = 10000;
+    auto range_true = [](const QState& q, int rep_count) {
+        float std_dev = std::sqrt(q.probability_true() * q.probability_false());
+        int expected = rep_count * q.probability_true();
+        int diff = std::sqrt(rep_count) * 3.291f * std_dev;
+        return std::pair<int, int>(expected - diff, expected + diff);
+    };
+    const QState qs[] = {QState({1.f, 0.f}, {0.f, 0.f}), QState({0.f, 1.f}, {0.f, 0.f}), QState({0.f, 0.f}, {1.f, 0.f}),
+                         QState({0.f, 0.f}, {0.f, 1.f}), QState({0.f, 1.f}, {0.f, 1.f}), QState({0.6f, 0.f}, {0.f, 0.8f})};
+    for (const auto& q : qs) {
+        auto prof = q.profile_measure(reps);
+        auto rng = range_true(q, reps);
+        CHECK((rng.first <= (int)prof[1] && (int)prof[1] <= rng.second));
+    }
+}
+
+// :195-220
+static void test_qubit_operations() {
+    float diff = 1e-4f;
+    const QState qref({0.5f, -0.5f}, {0.7f, 0.1f});
+    QState q1 = X_op(qref);
+    CHECK(cfequal(q1[0], qref[1], 1e-6f) && cfequal(q1[1], qref[0], 1e-6f));
+    QState q2 = Y_op(qref);
+    CHECK(cfequal(q2[0], cfloat{0.1f, -0.7f}, diff) && cfequal(q2[1], cfloat{0.5f, 0.5f}, diff));
+    QState q3 = Z_op(qref);
+    CHECK(cfequal(q3[0], cfloat{0.5f, -0.5f}, diff) && cfequal(q3[1], cfloat{-0.7f, -0.1f}, diff));
+    QState q4 = Hadamard_op(qref);
+    QState q4h({1.2f, -0.4f}, {-0.2f, -0.6f});
+    CHECK(fequal(q4[0].real(), q4h[0].real(), diff) && fequal(q4[0].imag(), q4h[0].imag(), diff) &&
+          fequal(q4[1].real(), q4h[1].real(), diff) && fequal(q4[1].imag(), q4h[1].imag(), diff));
+}
+
+// :251-267
+static void test_qsim_constructor() {
+    int count = 3;
+    QState state{{0.6f, 0.0f}, {0.0f, 0.8f}};
+    QSimulator qs2(count, state);
+    CHECK(qs2.qubit_count() == (uint32_t)count);
+    for (int i = 0; i < count; ++i) CHECK(qs2.qubit(i) == state);
+    CHECK(cfequal(qs2.state(0), cfloat{0.216f, 0.f}, 1e-4f));
+}
+
+// :274-336
+static void test_qsim_measurement() {
+    int count = 3;
+    QCircuit qc(count);
+    QSimulator qs(count, QState::zero());
+    qs.generate_statevector();
+    CHECK(qs.peek_measure_all() == 0);
+    for (int i = 0; i < count; ++i) CHECK(qs.peek_measure(i) == false);
+    qs.qubit(0) = QState::one();
+    qs.qubit(count - 1) = QState::one();
+    qs.generate_statevector();
+    CHECK(qs.peek_measure_all() == ((1u << (count - 1)) | 1u));
+    CHECK(qs.peek_measure(0) == true);
+    CHECK(qs.peek_measure(count - 1) == true);
+    for (int i = 1; i < count - 1; ++i) CHECK(qs.peek_measure(i) == false);
+    qs.qubit(0) = QState({0.0f, 1.0f}, {-1.0f, 0.0f});
+    qs.qubit(count - 1) = QState({0.6f, 0.0f}, {0.0f, 0.8f});
+    qs.generate_statevector();
+    auto sv = qs.measure_all();
+    CHECK((qs.state(sv) == cfloat{1.0f, 0.0f}));
+    qs.qubit(0) = QState({0.0f, 1.0f}, {-1.0f, 0.0f});
+    qs.qubit(count - 1) = QState({0.6f, 0.0f}, {0.0f, 0.8f});
+    qs.generate_statevector();
+    float diff = 1e-4f;
+    bool q0state = qs.measure(0);
+    sv = qs.peek_measure_all();
+    CHECK((q0state == static_cast<bool>(sv & (1u << (qs.qubit_count() - 1)))));
+    if (q0state) {
+        CHECK(cfequal(qs.state(0b100), cfloat{-0.6f, 0.0f}, diff));
+        CHECK(cfequal(qs.state(0b101), cfloat{0.0f, -0.8f}, diff));
+    } else {
+        CHECK(cfequal(qs.state(0b000), cfloat{0.0f, 0.6f}, diff));
+        CHECK(cfequal(qs.state(0b001), cfloat{-0.8f, 0.0f}, diff));
+    }
+}
+
+// :343-406
+static void test_qsim_profile_measure() {
+    int reps = 10000;
+    int qcount = 5;
+    QSimulator qs(qcount);
+    qs.qubit(3) = QState::one();
+    qs.generate_statevector();
+    auto state_range_true = [](const QSimulator& q, int state, int rep_count) {
+        float prob_true = q.state_probability(state);
+        float std_dev = std::sqrt(prob_true * (1.f - prob_true));
+        int expected = rep_count * prob_true;
+        int diff = std::sqrt(rep_count) * 3.291f * std_dev;
+        return std::pair<int, int>(expected - diff, expected + diff);
+    };
+    auto qubit_range_true = [](const QState& q, int rep_count) {
+        float std_dev = std::sqrt(q.probability_true() * q.probability_false());
+        int expected = rep_count * q.probability_true();
+        int diff = std::sqrt(rep_count) * 3.291f * std_dev;
+        return std::pair<int, int>(expected - diff, expected + diff);
+    };
+    auto profile = qs.profile_measure_all(reps);
+    for (size_t i = 0; i < profile.size(); ++i) {
+        auto range = state_range_true(qs, (int)i, reps);
+        CHECK((range.first <= (int)profile[i] && (int)profile[i] <= range.second));
+    }
+    qs.qubit(0) = QState(1.0f, 1.0f);
+    qs.qubit(1) = QState(-2.0f, 3.0f);
+    qs.qubit(4) = QState({0.0f, 0.6f}, {0.8f, 0.0f});
+    qs.generate_statevector();
+    profile = qs.profile_measure_all(reps);
+    for (size_t i = 0; i < profile.size(); ++i) {
+        auto range = state_range_true(qs, (int)i, reps);
+        CHECK((range.first <= (int)profile[i] && (int)profile[i] <= range.second));
+    }
+    for (int i = 0; i < qcount; ++i) {
+        auto prof = qs.profile_measure(i, reps);
+        auto range = qubit_range_true(qs.qubit(i), reps);
+        CHECK((range.first <= (int)prof[1] && (int)prof[1] <= range.second));
+    }
+}
+
+// :414-1005
+static void test_qsim_gates() {
+    float diff = 1e-4f;
+    int qcount = 4;
+    QSimulator qs(qcount);
+    QCircuit qc(qcount);
+    qs.qubit(3) = QState::one();
+
+    qs.generate_statevector();
+    qc << X(2);
+    qc << X(3);
+    qc.compile();
+    qs.simulate(qc);
+    CHECK((qs.state(0b0010) == cfloat{1.0f, 0.f}));
+
+    std::vector<X> xgates = {X{2}, X{3}};
+    qc.clear();
+    qs.generate_statevector();
+    qc << xgates;
+    qc.compile();
+    qs.simulate(qc);
+    CHECK((qs.state(0b0010) == cfloat{1.0f, 0.f}));
+
+    qs.qubit(2) = QState::zero();
+    qs.qubit(3) = QState::one();
+    qs.generate_statevector();
+    qc.clear();
+    qc << Y(2);
+    qc << Y(3);
+    qc.compile();
+    qs.simulate(qc);
+    CHECK((qs.state(0b0010) == cfloat{1.0f, 0.f}));
+
+    qs.generate_statevector();
+    qc.clear();
+    qc << Z(2);
+    qc << Z(3);
+    qc.compile();
+    qs.simulate(qc);
+    CHECK((qs.state(0b0001) == cfloat{-1.0f, 0.f}));
+
+    qs.generate_statevector();
+    qc.clear();
+    qc << Not(2);
+    qc << Not(3);
+    qc.compile();
+    qs.simulate(qc);
+    CHECK((qs.state(0b0010) == cfloat{1.0f, 0.f}));
+
+    qs.generate_statevector();
+    qc.clear();
+    qc << H(2);
+    qc << H(3);
+    qc.compile();
+    qs.simulate(qc);
+    CHECK(cfequal(qs.state(0b0000), cfloat{0.5f, 0.f}, diff));
+    CHECK(cfequal(qs.state(0b0001), cfloat{-0.5f, 0.f}, diff));
+    CHECK(cfequal(qs.state(0b0010), cfloat{0.5f, 0.f}, diff));
+    CHECK(cfequal(qs.state(0b0011), cfloat{-0.5f, 0.f}, diff));
+
+    qs.qubit(2) = QState({0.6f, 0.0f}, {0.0f, 0.8f});
+    qs.qubit(3) = QState::zero();
+    qs.generate_statevector();
+    qc.clear();
+    qc << Phase(2, std::atan(0.75f));
+    qc << Phase(3, std::atan(1.f));
+    qc.compile();
+    qs.simulate(qc);
+    CHECK(cfequal(qs.state(0b0000), cfloat{0.6f, 0.f}, diff));
+    CHECK(cfequal(qs.state(0b0001), cfloat{0.f, 0.f}, diff));
+    CHECK(cfequal(qs.state(0b0010), cfloat{-0.48f, 0.64f}, diff));
+    CHECK(cfequal(qs.state(0b0011), cfloat{0.f, 0.f}, diff));
+
+    qc.clear();
+    qs.qubit(0) = QState::zero();
+    qs.qubit(1) = QState::zero();
+    qs.qubit(2) = QState::minus();
+    qs.qubit(3) = QState::zero();
+    qc << RotX(2, pi / 2.f);
+    qs.generate_statevector();
+    qc.compile();
+    qs.simulate(qc);
+    CHECK(cfequal(qs.state(0b0000), cfloat{0.5, 0.5}, diff));
+    CHECK(cfequal(qs.state(0b0010), cfloat{-0.5, -0.5}, diff));
+
+    qc.clear();
+    qs.qubit(2) = QState::one();
+    qc << RotY(2, std::acos(0.6f) * 2.f);
+    qs.generate_statevector();
+    qc.compile();
+    qs.simulate(qc);
+    CHECK(cfequal(qs.state(0b0000), cfloat{-0.8, 0.0}, diff));
+    CHECK(cfequal(qs.state(0b0010), cfloat{0.6, 0.0}, diff));
+
+    qc.clear();
+    qs.qubit(2) = QState::minus();
+    qc << RotZ(2, -pi / 2.f);
+    qs.generate_statevector();
+    qc.compile();
+    qs.simulate(qc);
+    CHECK(cfequal(qs.state(0b0000), cfloat{0.5, 0.5}, diff));
+    CHECK(cfequal(qs.state(0b0010), cfloat{-0.5, 0.5}, diff));
+
+    // Xor / CNot / CX: same expected amplitudes
+    auto cx_expect = [&]() {
+        CHECK(cfequal(qs.state(0b0100), cfloat{0.0f, 0.48f}, diff));
+        CHECK(cfequal(qs.state(0b0101), cfloat{0.36f, 0.f}, diff));
+        CHECK(cfequal(qs.state(0b0110), cfloat{-0.64f, 0.f}, diff));
+        CHECK(cfequal(qs.state(0b0111), cfloat{0.f, 0.48f}, diff));
+    };
+    qs.qubit(0) = QState::zero();
+    qs.qubit(1) = QState::one();
+    qs.qubit(2) = QState({0.6f, 0.0f}, {0.0f, 0.8f});
+    qs.qubit(3) = QState({0.6f, 0.0f}, {0.0f, 0.8f});
+    qs.generate_statevector();
+    qc.clear();
+    qc << Xor(0, 2);
+    qc << Xor(1, 3);
+    qc.compile();
+    qs.simulate(qc);
+    cx_expect();
+
+    qs.qubit(3) = QState({0.0f, 0.8f}, {0.6f, 0.0f});
+    qs.generate_statevector();
+    qc.clear();
+    qc << Swap(0, 2);
+    qc << Swap(1, 3);
+    qc.compile();
+    qs.simulate(qc);
+    CHECK(cfequal(qs.state(0b0001), cfloat{0.0f, 0.48f}, diff));
+    CHECK(cfequal(qs.state(0b0101), cfloat{0.36f, 0.0f}, diff));
+    CHECK(cfequal(qs.state(0b1001), cfloat{-0.64f, 0.0f}, diff));
+    CHECK(cfequal(qs.state(0b1101), cfloat{0.0f, 0.48f}, diff));
+
+    qs.qubit(3) = QState({0.6f, 0.0f}, {0.0f, 0.8f});
+    qs.generate_statevector();
+    qc.clear();
+    qc << CNot(0, 2);
+    qc << CNot(1, 3);
+    qc.compile();
+    qs.simulate(qc);
+    cx_expect();
+
+    std::vector<CX> cxgates = {CX{0, 2}, CX{1, 3}};
+    qs.generate_statevector();
+    qc.clear();
+    qc << cxgates;
+    qc.compile();
+    qs.simulate(qc);
+    cx_expect();
+
+    qs.generate_statevector();
+    qc.clear();
+    qc << CY(0, 2);
+    qc << CY(1, 3);
+    qc.compile();
+    qs.simulate(qc);
+    CHECK(cfequal(qs.state(0b0100), cfloat{0.48f, 0.0f}, diff));
+    CHECK(cfequal(qs.state(0b0101), cfloat{0.0f, 0.36f}, diff));
+    CHECK(cfequal(qs.state(0b0110), cfloat{0.0f, 0.64f}, diff));
+    CHECK(cfequal(qs.state(0b0111), cfloat{-0.48f, 0.0f}, diff));
+
+    qs.generate_statevector();
+    qc.clear();
+    qc << CZ(0, 2);
+    qc << CZ(1, 3);
+    qc.compile();
+    qs.simulate(qc);
+    CHECK(cfequal(qs.state(0b0100), cfloat{0.36f, 0.0f}, diff));
+    CHECK(cfequal(qs.state(0b0101), cfloat{0.0f, -0.48f}, diff));
+    CHECK(cfequal(qs.state(0b0110), cfloat{0.0f, 0.48f}, diff));
+    CHECK(cfequal(qs.state(0b0111), cfloat{0.64f, 0.0f}, diff));
+
+    qs.generate_statevector();
+    qc.clear();
+    qc << CPhase(0, 2, std::atan(0.75f));
+    qc << CPhase(1, 3, std::atan(0.75f));
+    qc.compile();
+    qs.simulate(qc);
+    CHECK(cfequal(qs.state(0b0100), cfloat{0.36f, 0.f}, diff));
+    CHECK(cfequal(qs.state(0b0101), cfloat{-0.288f, 0.384f}, diff));
+    CHECK(cfequal(qs.state(0b0110), cfloat{0.0f, 0.48f}, diff));
+    CHECK(cfequal(qs.state(0b0111), cfloat{-0.512f, -0.384f}, diff));
+
+    const float invsqrt8 = 0.3535533906f;
+    qc.clear();
+    qs.qubit(2) = QState::minus();
+    qs.qubit(3) = QState::minus();
+    qc << CRotX(0, 2, pi / 2.f);
+    qc << CRotX(1, 3, pi / 2.f);
+    qs.generate_statevector();
+    qc.compile();
+    qs.simulate(qc);
+    CHECK(cfequal(qs.state(0b0100), cfloat{invsqrt8, invsqrt8}, diff));
+    CHECK(cfequal(qs.state(0b0101), cfloat{-invsqrt8, -invsqrt8}, diff));
+    CHECK(cfequal(qs.state(0b0110), cfloat{-invsqrt8, -invsqrt8}, diff));
+    CHECK(cfequal(qs.state(0b0111), cfloat{invsqrt8, invsqrt8}, diff));
+
+    qc.clear();
+    qs.qubit(2) = QState::one();
+    qs.qubit(3) = QState::one();
+    qc << CRotY(0, 2, std::acos(0.6f) * 2.f);
+    qc << CRotY(1, 3, std::acos(0.6f) * 2.f);
+    qs.generate_statevector();
+    qc.compile();
+    qs.simulate(qc);
+    CHECK(cfequal(qs.state(0b0100), cfloat{0.0, 0.0}, diff));
+    CHECK(cfequal(qs.state(0b0110), cfloat{-0.8, 0.0}, diff));
+    CHECK(cfequal(qs.state(0b0111), cfloat{0.6, 0.0}, diff));
+
+    qc.clear();
+    qs.qubit(2) = QState::minus();
+    qs.qubit(3) = QState::minus();
+    qc << CRotZ(0, 2, -pi / 2.f);
+    qc << CRotZ(1, 3, -pi / 2.f);
+    qs.generate_statevector();
+    qc.compile();
+    qs.simulate(qc);
+    CHECK(cfequal(qs.state(0b0100), cfloat{invsqrt8, invsqrt8}, diff));
+    CHECK(cfequal(qs.state(0b0101), cfloat{-invsqrt8, invsqrt8}, diff));
+    CHECK(cfequal(qs.state(0b0110), cfloat{-invsqrt8, -invsqrt8}, diff));
+    CHECK(cfequal(qs.state(0b0111), cfloat{invsqrt8, -invsqrt8}, diff));
+
+    // Or / And / CCNot with three initial states each (:800-907)
+    auto three = [&](QCircuit& c, uint32_t e1, uint32_t e2, uint32_t e3) {
+        qs.qubit(0) = QState::zero();
+        qs.qubit(1) = QState::one();
+        qs.qubit(2) = QState::zero();
+        qs.qubit(3) = QState::zero();
+        qs.generate_statevector();
+        c.compile();
+        qs.simulate(c);
+        CHECK(cfequal(qs.state(e1), cfloat{1.0f, 0.0f}, diff));
+        qs.qubit(0) = QState::one();
+        qs.generate_statevector();
+        qs.simulate(c);
+        CHECK(cfequal(qs.state(e2), cfloat{1.0f, 0.0f}, diff));
+        qs.qubit(0) = QState::zero();
+        qs.qubit(1) = QState::zero();
+        qs.generate_statevector();
+        qs.simulate(c);
+        CHECK(cfequal(qs.state(e3), cfloat{1.0f, 0.0f}, diff));
+    };
+    qc.clear();
+    qc << Or(0, 1, 2);
+    qc << Or(1, 0, 3);
+    three(qc, 0b0111, 0b1111, 0b0000);
+    qc.clear();
+    qc << And(0, 1, 2);
+    qc << And(1, 0, 3);
+    three(qc, 0b0100, 0b1111, 0b0000);
+    qc.clear();
+    qc << CCNot(0, 1, 2);
+    qc << CCNot(1, 0, 3);
+    three(qc, 0b0100, 0b1111, 0b0000);
+
+    qs.qubit(0) = QState::zero();
+    qs.qubit(1) = QState::one();
+    qs.qubit(2) = QState({0.6f, 0.0f}, {0.0f, 0.8f});
+    qs.qubit(3) = QState({0.0f, 0.8f}, {0.6f, 0.0f});
+    qs.generate_statevector();
+    qc.clear();
+    qc << CSwap(0, 2, 3);
+    qc.compile();
+    qs.simulate(qc);
+    CHECK(cfequal(qs.state(0b0100), cfloat{0.0f, 0.48f}, diff));
+    CHECK(cfequal(qs.state(0b0101), cfloat{0.36f, 0.0f}, diff));
+    CHECK(cfequal(qs.state(0b0110), cfloat{-0.64f, 0.0f}, diff));
+    CHECK(cfequal(qs.state(0b0111), cfloat{0.0f, 0.48f}, diff));
+    qs.generate_statevector();
+    qc.clear();
+    qc << CSwap(1, 2, 3);
+    qc.compile();
+    qs.simulate(qc);
+    CHECK(cfequal(qs.state(0b0100), cfloat{0.0f, 0.48f}, diff));
+    CHECK(cfequal(qs.state(0b0101), cfloat{-0.64f, 0.0f}, diff));
+    CHECK(cfequal(qs.state(0b0110), cfloat{0.36f, 0.0f}, diff));
+    CHECK(cfequal(qs.state(0b0111), cfloat{0.0f, 0.48f}, diff));
+
+    // Custom ControlGate (:909-941)
+    QCircuit temp(2);
+    temp << H(0);
+    temp << CNot(0, 1);
+    temp << Z(0);
+    temp << Swap(0, 1);
+    qc.clear();
+    qc << ControlGate(temp, 0, 2);
+    QCircuit ref(4);
+    ref << CH(0, 2);
+    ref << CCNot(0, 2, 3);
+    ref << CZ(0, 2);
+    ref << CSwap(0, 2, 3);
+    qc.compile();
+    ref.compile();
+    CHECK(near(qc.circuit(), ref.circuit(), 1e-6f));
+    qc.clear();
+    qc << ControlGate(temp, 2, 0);
+    ref.clear();
+    ref << CH(2, 0);
+    ref << CCNot(0, 2, 1);
+    ref << CZ(2, 0);
+    ref << CSwap(2, 0, 1);
+    qc.compile();
+    ref.compile();
+    CHECK(near(qc.circuit(), ref.circuit(), 1e-6f));
+
+    // Custom Gate (:943-961)
+    qc.clear();
+    temp.clear();
+    temp << H(0);
+    temp << CPhase(0, 1, 1.0f);
+    temp << X(1);
+    temp << Swap(0, 1);
+    qc << Gate(temp, 1);
+    ref.clear();
+    ref << H(1);
+    ref << CPhase(1, 2, 1.0f);
+    ref << X(2);
+    ref << Swap(1, 2);
+    qc.compile();
+    ref.compile();
+    CHECK(near(qc.circuit(), ref.circuit(), 1e-6f));
+
+    // CH (:963-1002)
+    QSimulator reference(4);
+    reference.qubit(0) = QState::zero();
+    reference.qubit(3) = QState::zero();
+    reference.qubit(1) = QState::zero();
+    reference.qubit(2) = QState::one();
+    reference.generate_statevector();
+    qs.qubit(0) = QState::zero();
+    qs.qubit(3) = QState::zero();
+    qs.qubit(1) = QState::zero();
+    qs.qubit(2) = QState::one();
+    qs.generate_statevector();
+    qc.clear();
+    qc << CH(0, 1);
+    qc << CH(3, 2);
+    qc.compile();
+    qs.simulate(qc);
+    CHECK(near(reference.statevector().host(), qs.statevector().host(), 0.f));
+    float rsqrt2 = std::sqrt(1.0f / 2.0f);
+    reference.qubit(0) = QState::one();
+    reference.qubit(3) = QState::one();
+    reference.qubit(1) = QState{rsqrt2, rsqrt2};
+    reference.qubit(2) = QState{rsqrt2, -rsqrt2};
+    reference.generate_statevector();
+    qs.qubit(0) = QState::one();
+    qs.qubit(3) = QState::one();
+    qs.qubit(1) = QState::zero();
+    qs.qubit(2) = QState::one();
+    qs.generate_statevector();
+    qc.compile();
+    qs.simulate(qc);
+    CHECK(near(reference.statevector().host(), qs.statevector().host(), 1e-6f));
+}
+
+// :1141-1194
+static void test_qsim_probability() {
+    int qcount = 4;
+    float diff = 1e-4f;
+    QSimulator qs(4);
+    qs.generate_statevector();
+    for (int i = 0; i < qcount; ++i) CHECK(qs.qubit_probability_true(i) == 0.f && qs.qubit_probability_false(i) == 1.f);
+    qs.qubit(qcount - 1) = QState{0.f, 1.f};
+    qs.generate_statevector();
+    for (int i = 0; i < qcount - 1; ++i) CHECK(qs.qubit_probability_true(i) == 0.f);
+    CHECK(qs.qubit_probability_true(qcount - 1) == 1.f && qs.qubit_probability_false(qcount - 1) == 0.f);
+    qs.qubit(qcount - 1) = QState{1.f, 1.f};
+    qs.generate_statevector();
+    CHECK(fequal(qs.qubit_probability_true(qcount - 1), 0.5f, diff));
+    CHECK(fequal(qs.qubit_probability_false(qcount - 1), 0.5f, diff));
+    qs.qubit(0) = QState{{0.6f, 0.0f}, {0.f, 0.8f}};
+    qs.generate_statevector();
+    for (int i = 1; i < qcount - 1; ++i) CHECK(qs.qubit_probability_true(i) == 0.f);
+    CHECK(fequal(qs.qubit_probability_true(0), 0.64f, diff));
+    CHECK(fequal(qs.qubit_probability_false(0), 0.36f, diff));
+    CHECK(fequal(qs.qubit_probability_true(qcount - 1), 0.5f, diff));
+}
+
+// Errors carry the reference's exception types (e.g. src/quantum.cpp:535-536, 967-975).
+static void test_errors() {
+    QCircuit qc(3);
+    bool threw = false;
+    try {
+        qc << X(3);
+    } catch (const std::out_of_range&) {
+        threw = true;
+    }
+    CHECK(threw);
+    threw = false;
+    try {
+        qc << CX(1, 1);
+    } catch (const std::invalid_argument&) {
+        threw = true;
+    }
+    CHECK(threw);
+    threw = false;
+    try {
+        QCircuit one(1);
+        one << CX(0, 1);
+    } catch (const std::domain_error&) {
+        threw = true;
+    }
+    CHECK(threw);
+    threw = false;
+    try {
+        QCircuit bad(31);
+    } catch (const std::invalid_argument&) {
+        threw = true;
+    }
+    CHECK(threw);
+    CHECK(qc.gate_list().empty());
+}
+
+// Compiled (fused) and uncompiled simulation agree; representation strings.
+static void test_compile_equivalence() {
+    for (uint32_t n : {6u, 12u}) {
+        QCircuit a = fourier_transform(n, false);
+        QCircuit b = fourier_transform(n, true);
+        CHECK(a.representation() == b.representation());
+        CHECK(b.compiled_passes() < b.gate_list().size());
+        QSimulator sa(n, QState::plus()), sb(n, QState::plus());
+        sa.simulate(a);
+        sb.simulate(b);
+        CHECK(near(sa.statevector().host(), sb.statevector().host(), 2e-6f));
+    }
+    QCircuit r(3);
+    r << H(0) << CX(0, 1) << Phase(2, pi / 4) << CPhase(1, 2, -pi / 2) << CSwap(0, 1, 2);
+    CHECK(r.representation() == "H,0,1:0;X,1,1:0,1;T,0,1:2;S†,1,1:1,2;Swap,1,2:0,12;");
+    QCircuit sub(2);
+    sub << H(0) << CX(0, 1);
+    Gate g(sub, 1);
+    ControlGate cg(sub, 0, 1);
+    CHECK(g.to_string() == "H,0,1:1;X,1,1:1,2;");
+    CHECK(cg.to_string() == "H,1,1:0,1;X,2,1:0,1,2;");
+}
+
+int main(int argc, char** argv) {
+    initialize(argc, argv);
+    seed(20240229u);
+    test_qubit_constructor();
+    test_qubit_assigment();
+    test_qubit_normalization();
+    test_qubit_probability();
+    test_qubit_profile_measure();
+    test_qubit_operations();
+    test_qsim_constructor();
+    test_qsim_measurement();
+    test_qsim_profile_measure();
+    test_qsim_gates();
+    test_qsim_probability();
+    test_errors();
+    test_compile_equivalence();
+    std::cout << "ALL PASSED" << std::endl;
+    return 0;
+}

@@ -31,7 +31,9 @@ def _device():
 def gpu_run(n, sv, gates, precision=hip.C64, fused=None):
     st = hip.State(n, precision)
     st.upload(sv.astype(st.dtype))
-    lowered = ops.lower_all(gates)
+    # complex128 runs use the exact FP64 operator; complex64 runs use the
+    # reference's float32 matrix entries
+    lowered = ops.lower_all(gates, exact=(precision == hip.C128))
     if fused:
         sch = hip.Schedule(lowered, n, fused)
         st.apply_schedule(sch)
@@ -168,7 +170,7 @@ def test_random_layered_circuit(n):
     got = gpu_run(n, sv, gates)
     assert oracle.infidelity(got, want) < 1e-9
     got128 = gpu_run(n, sv, gates, hip.C128)
-    assert oracle.infidelity(got128, want) < 1e-20
+    assert oracle.infidelity(got128, want) < 1e-24
     np.testing.assert_allclose(got128, want, atol=1e-12)
 
 

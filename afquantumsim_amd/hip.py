@@ -79,6 +79,8 @@ EXPORTS = [
     "aqs_norm2", "aqs_normalize", "aqs_prob1", "aqs_prob1_all", "aqs_expect_sum_z",
     "aqs_probabilities", "aqs_amplitude", "aqs_sample", "aqs_measure_qubit", "aqs_collapse_to",
     "aqs_inner", "aqs_profile_enable", "aqs_profile_read", "aqs_timer_start", "aqs_timer_stop",
+    "aqs_batch_create", "aqs_batch_autotune", "aqs_batch_run", "aqs_batch_state", "aqs_batch_info",
+    "aqs_batch_destroy",
 ]
 
 _lib = None
@@ -131,6 +133,13 @@ def load(path: str = LIB_PATH):
         "aqs_profile_read": [vp, dp, ctypes.POINTER(ctypes.c_uint64)],
         "aqs_timer_start": [vp],
         "aqs_timer_stop": [vp, dp],
+        "aqs_batch_create": [i32, ctypes.POINTER(i32), ctypes.POINTER(Op), ctypes.POINTER(sz), ctypes.POINTER(vp)],
+        "aqs_batch_autotune": [vp, ctypes.POINTER(i32), ctypes.POINTER(i32), dp],
+        "aqs_batch_run": [vp, dp, dp],
+        "aqs_batch_state": [vp, i32, vp],
+        "aqs_batch_info": [vp, ctypes.POINTER(sz), ctypes.POINTER(sz), ctypes.POINTER(i32), ctypes.POINTER(i32),
+                           ctypes.POINTER(i32)],
+        "aqs_batch_destroy": [vp],
     }
     for name, args in sigs.items():
         f = getattr(L, name)
@@ -332,6 +341,61 @@ class Schedule:
     def close(self):
         if self.h:
             check(_lib.aqs_schedule_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many small independent states run in one persistent launch."""
+
+    def __init__(self, circuits_ops):
+        """circuits_ops: list of (nqubits, ops) pairs."""
+        L = load()
+        self.n = [int(n) for n, _ in circuits_ops]
+        allops = []
+        offs = [0]
+        for _, o in circuits_ops:
+            allops.extend(o)
+            offs.append(len(allops))
+        arr, keep = pack_ops(allops)
+        nq = (ctypes.c_int * len(self.n))(*self.n)
+        off = (ctypes.c_size_t * len(offs))(*offs)
+        h = ctypes.c_void_p()
+        check(L.aqs_batch_create(len(self.n), nq, arr, off, ctypes.byref(h)))
+        del keep
+        self.h = h
+
+    def autotune(self):
+        b, lq, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        check(_lib.aqs_batch_autotune(self.h, ctypes.byref(b), ctypes.byref(lq), ctypes.byref(ms)))
+        return {"block": b.value, "lds_qubits": lq.value, "ms": ms.value}
+
+    def run(self):
+        out = np.zeros(len(self.n), dtype=np.float64)
+        ms = ctypes.c_double()
+        check(_lib.aqs_batch_run(self.h, _dp(out), ctypes.byref(ms)))
+        return out, ms.value
+
+    def state(self, i):
+        out = np.empty(1 << self.n[i], dtype=np.complex64)
+        check(_lib.aqs_batch_state(self.h, i, out.ctypes.data))
+        return out
+
+    def info(self):
+        p, s = ctypes.c_size_t(), ctypes.c_size_t()
+        b, lq, g = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(_lib.aqs_batch_info(self.h, ctypes.byref(p), ctypes.byref(s), ctypes.byref(b), ctypes.byref(lq),
+                                  ctypes.byref(g)))
+        return {"passes": p.value, "source_ops": s.value, "block": b.value, "lds_qubits": lq.value, "grid": g.value}
+
+    def close(self):
+        if self.h:
+            check(_lib.aqs_batch_destroy(self.h))
             self.h = None
 
     def __del__(self):

@@ -179,6 +179,27 @@ aqs_status aqs_collapse_to(aqs_state s, uint64_t index);
 /* <a|b> in FP64 for two states of equal qubit count (any precisions). */
 aqs_status aqs_inner(aqs_state a, aqs_state b, double* re_im);
 
+/* ---- batched small states (no reference counterpart: the reference runs
+ * one state per QSimulator) ----------------------------------------------- */
+
+typedef struct aqs_batch_s* aqs_batch;
+
+/* count states; state i has nqubits[i] (1..20) qubits and the ops
+ * ops[op_offsets[i] .. op_offsets[i+1]) (MAT1, DIAG1, PHASE, X, SWAP).
+ * Every run starts each state from |0..0>. */
+aqs_status aqs_batch_create(int count, const int* nqubits, const aqs_op* ops, const size_t* op_offsets,
+                            aqs_batch* out);
+/* Times the (workgroup size, LDS tile) candidates and keeps the fastest. */
+aqs_status aqs_batch_autotune(aqs_batch b, int* block, int* lds_qubits, double* best_ms);
+/* One persistent launch (replayed from a hipGraph) over all states;
+ * expect_sum_z (host, count doubles, may be NULL) receives <sum_q Z_q> of
+ * every final state in FP64; ms the device time of the launch. */
+aqs_status aqs_batch_run(aqs_batch b, double* expect_sum_z, double* ms);
+/* Final complex64 amplitudes of state `index` (2^n values). */
+aqs_status aqs_batch_state(aqs_batch b, int index, void* host);
+aqs_status aqs_batch_info(aqs_batch b, size_t* passes, size_t* source_ops, int* block, int* lds_qubits, int* grid);
+aqs_status aqs_batch_destroy(aqs_batch b);
+
 /* ---- timing ------------------------------------------------------------- */
 
 /* Per-kind kernel timing with HIP events on the state's stream.  While

@@ -170,7 +170,8 @@ def test_random_layered_circuit(n):
     got = gpu_run(n, sv, gates)
     assert oracle.infidelity(got, want) < 1e-9
     got128 = gpu_run(n, sv, gates, hip.C128)
-    assert oracle.infidelity(got128, want) < 1e-24
+    # 1 - |<a|b>|^2 itself carries ~1e-16 of rounding in FP64
+    assert oracle.infidelity(got128, want) < 1e-14
     np.testing.assert_allclose(got128, want, atol=1e-12)
 
 

@@ -1,21 +1,25 @@
 // Dense k-qubit blocks (k = 4, 5) on CDNA4 matrix cores, complex64 state.
 //
 // A fused block is a 2^k x 2^k complex unitary U applied to every group of
-// 2^k amplitudes that differ only in the k target bits.  With the groups as
-// columns, the pass is the complex GEMM  Y = U X  (X: 2^k x 2^(n-k)).  It is
-// done as one real GEMM on the stacked operands
+// 2^k amplitudes that differ only in the k target bits; with the groups as
+// columns the pass is the complex GEMM  Y = U X.
 //
-//     [Yr]   [Ur  -Ui] [Xr]
-//     [Yi] = [Ui   Ur] [Xi]        (2K x 2K real times 2K x columns)
+// Tiling.  A tile is the 2^(k+5) amplitudes spanned by the k target bits and
+// the 5 lowest non-target bits (32 columns).  Bits 0..4 of the amplitude
+// index always belong to that set, so a tile is a union of contiguous runs
+// of at least 32 amplitudes (256 B) whatever the targets are: every HBM
+// access below is a coalesced 16-byte-per-lane run.  One wavefront owns a
+// tile at a time: it streams the tile into a private LDS region (padded by
+// 16 B every 256 B), computes from LDS, writes the result back to LDS and
+// streams it out.  The next tile's global loads are issued before the
+// matrix work of the current one.
 //
-// with v_mfma_f32_32x32x2_f32 (FP32 products, FP32 accumulation).  One
-// wavefront owns a tile of 32 columns: the stacked U (64 x 64 real for
-// k = 5, 32 x 32 for k = 4) lives in VGPRs for the whole kernel, every lane
-// loads the 16 (k = 5) or 8 (k = 4) complex amplitudes its B operands need
-// and writes the complex results straight from the accumulators, so no LDS
-// round trip is needed.  The 32 columns of a tile are consecutive in the
-// non-target bits; when the targets are not the lowest bits each row of a
-// tile is one 256-byte run of memory.
+// Arithmetic (FP32 MFMA, FP32 accumulation), v_mfma_f32_32x32x2_f32:
+//   k = 5: three real products (Gauss): T1 = Ur Xr, T2 = Ui Xi,
+//          T3 = (Ur + Ui)(Xr + Xi);  Yr = T1 - T2, Yi = T3 - T1 - T2
+//          -> 48 MFMAs per 32 columns instead of 64.
+//   k = 4: the stacked [Ur; Ui] (32 x 16) times Xr and times Xi
+//          -> 16 MFMAs per 32 columns.
 //
 // Operand layouts of v_mfma_f32_32x32x2_f32 (wave64):
 //   A (32 x 2):  lane l holds A[l % 32][l / 32]
@@ -34,154 +38,152 @@ namespace {
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
-constexpr int kWaves = 4;  // wavefronts per workgroup
-constexpr int kBlock = 64 * kWaves;
-
 struct MfmaArgs {
-    float2* sv;
-    uint64_t tiles;     // number of 32-column tiles
-    int nfixed;
-    int fixed[5];       // sorted target bit positions
-    uint32_t off[32];   // amplitude offset of sub-index r
-    const float* apack; // packed stacked operand
+    float4* sv4;
+    uint64_t tiles;
+    int ntile_bits;       // k + 5
+    int tpos[10];         // sorted amplitude bit positions of the tile
+    uint32_t gj[8];       // float4 offset of the j-th 64-lane chunk: pdep(128 j) / 2
+    uint32_t glane[64];   // float4 offset of lane l inside a chunk: pdep(2 l) / 2
+    uint32_t tr[32];      // tile-index bits of target sub-index r
+    uint32_t tc[32];      // tile-index bits of column c
+    const float* apack;
 };
 
-__device__ __forceinline__ uint64_t insert_zeros5(uint64_t t, const MfmaArgs& a) {
+__device__ __forceinline__ uint32_t padded(uint32_t t) { return t + ((t >> 5) << 1); }
+
+__device__ __forceinline__ uint64_t tile_base(uint64_t t, const MfmaArgs& a) {
 #pragma unroll
-    for (int j = 0; j < 5; ++j) {
-        if (j >= a.nfixed) break;
-        const uint64_t p = (uint64_t)a.fixed[j];
+    for (int j = 0; j < 10; ++j) {
+        if (j >= a.ntile_bits) break;
+        const uint64_t p = (uint64_t)a.tpos[j];
         t = ((t >> p) << (p + 1)) | (t & ((1ull << p) - 1ull));
     }
-    return t;
+    return t;  // amplitude index of the tile's first element (even)
 }
 
-// k = 5: K = 32 complex rows, stacked real 64 x 64.
-__global__ __launch_bounds__(kBlock) void k_matk32_mfma(MfmaArgs a) {
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+template <int K>
+__global__ __launch_bounds__(64) void k_matk_mfma(MfmaArgs a) {
+    constexpr int TILE = 32 * K;            // amplitudes per tile
+    constexpr int CHUNKS = TILE / 128;      // 64 lanes x float4 = 128 amplitudes
+    constexpr int LDS_F2 = TILE + TILE / 16;
+    __shared__ __attribute__((aligned(16))) float2 lds[LDS_F2];
+    const int lane = threadIdx.x;
     const int col = lane & 31;
     const int half = lane >> 5;
 
-    // A operand: [rt][s] for rt = 0,1 (real / imag output rows), s = 0..31.
-    float A0[32], A1[32];
+    // A operands for the whole kernel
+    constexpr int NA = (K == 32) ? 48 : 8;
+    float A[NA];
 #pragma unroll
-    for (int s = 0; s < 32; ++s) {
-        A0[s] = a.apack[(0 * 32 + s) * 64 + lane];
-        A1[s] = a.apack[(1 * 32 + s) * 64 + lane];
-    }
-    uint32_t loff[16], soff[16];
-#pragma unroll
-    for (int s = 0; s < 16; ++s) loff[s] = a.off[2 * s + half];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) soff[r] = a.off[8 * (r / 4) + 4 * half + (r % 4)];
+    for (int i = 0; i < NA; ++i) A[i] = a.apack[i * 64 + lane];
 
-    const uint64_t wstride = (uint64_t)gridDim.x * kWaves;
-    for (uint64_t tile = (uint64_t)blockIdx.x * kWaves + wave; tile < a.tiles; tile += wstride) {
-        const uint64_t base = insert_zeros5(tile * 32 + col, a);
-        float2 x[16];
+    const uint32_t glane = a.glane[lane];
+    const uint32_t rd_lane = a.tr[half] | a.tc[col];        // B rows 2s + half
+    const uint32_t wr_lane = a.tr[4 * half] | a.tc[col];    // D rows 8(r/4) + 4 half + r%4
+
+    uint64_t t = blockIdx.x;
+    const uint64_t stride = gridDim.x;
+    float4 pf[CHUNKS];
+    if (t < a.tiles) {
+        const uint64_t b4 = tile_base(t, a) >> 1;
 #pragma unroll
-        for (int s = 0; s < 16; ++s) x[s] = ld_nt(&a.sv[base + loff[s]]);
-        floatx16 acc0 = {0}, acc1 = {0};
+        for (int j = 0; j < CHUNKS; ++j) pf[j] = ld_nt(&a.sv4[b4 + a.gj[j] + glane]);
+    }
+    while (t < a.tiles) {
+        const uint64_t b4 = tile_base(t, a) >> 1;
+        // tile -> LDS (memory order)
 #pragma unroll
-        for (int s = 0; s < 16; ++s) {
-            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0[s], x[s].x, acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1[s], x[s].x, acc1, 0, 0, 0);
+        for (int j = 0; j < CHUNKS; ++j) {
+            const uint32_t tau = 2u * (uint32_t)(j * 64 + lane);
+            *reinterpret_cast<float4*>(&lds[padded(tau)]) = pf[j];
         }
+        __syncthreads();
+        // prefetch the next tile
+        const uint64_t tn = t + stride;
+        if (tn < a.tiles) {
+            const uint64_t n4 = tile_base(tn, a) >> 1;
 #pragma unroll
-        for (int s = 0; s < 16; ++s) {
-            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0[16 + s], x[s].y, acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1[16 + s], x[s].y, acc1, 0, 0, 0);
+            for (int j = 0; j < CHUNKS; ++j) pf[j] = ld_nt(&a.sv4[n4 + a.gj[j] + glane]);
         }
+        if (K == 32) {
+            floatx16 t1 = {0}, t2 = {0}, t3 = {0};
 #pragma unroll
-        for (int r = 0; r < 16; ++r)
-            st_nt(make_float2(acc0[r], acc1[r]), &a.sv[base + soff[r]]);
+            for (int s = 0; s < 16; ++s) {
+                const float2 x = lds[padded(a.tr[2 * s] | rd_lane)];
+                t1 = __builtin_amdgcn_mfma_f32_32x32x2f32(A[s], x.x, t1, 0, 0, 0);
+                t2 = __builtin_amdgcn_mfma_f32_32x32x2f32(A[16 + s], x.y, t2, 0, 0, 0);
+                t3 = __builtin_amdgcn_mfma_f32_32x32x2f32(A[32 + s], x.x + x.y, t3, 0, 0, 0);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const uint32_t tau = a.tr[8 * (r / 4) + (r % 4)] | wr_lane;
+                lds[padded(tau)] = make_float2(t1[r] - t2[r], t3[r] - t1[r] - t2[r]);
+            }
+        } else {
+            floatx16 p = {0}, q = {0};
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                const float2 x = lds[padded(a.tr[2 * s] | rd_lane)];
+                p = __builtin_amdgcn_mfma_f32_32x32x2f32(A[s], x.x, p, 0, 0, 0);
+                q = __builtin_amdgcn_mfma_f32_32x32x2f32(A[s], x.y, q, 0, 0, 0);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                const uint32_t tau = a.tr[8 * (r / 4) + (r % 4)] | wr_lane;
+                lds[padded(tau)] = make_float2(p[r] - q[r + 8], p[r + 8] + q[r]);
+            }
+        }
+        __syncthreads();
+        // LDS -> tile (memory order)
+#pragma unroll
+        for (int j = 0; j < CHUNKS; ++j) {
+            const uint32_t tau = 2u * (uint32_t)(j * 64 + lane);
+            st_nt(*reinterpret_cast<const float4*>(&lds[padded(tau)]), &a.sv4[b4 + a.gj[j] + glane]);
+        }
+        __syncthreads();
+        t = tn;
     }
 }
 
-// k = 4: K = 16 complex rows, stacked real 32 x 32; two column tiles per
-// iteration keep 16 loads in flight per lane.
-__global__ __launch_bounds__(kBlock) void k_matk16_mfma(MfmaArgs a) {
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int col = lane & 31;
-    const int half = lane >> 5;
-
-    float A[16];
-#pragma unroll
-    for (int s = 0; s < 16; ++s) A[s] = a.apack[s * 64 + lane];
-    uint32_t loff[8], soff[8];
-#pragma unroll
-    for (int s = 0; s < 8; ++s) loff[s] = a.off[2 * s + half];
-#pragma unroll
-    for (int r = 0; r < 8; ++r) soff[r] = a.off[8 * (r / 4) + 4 * half + (r % 4)];
-
-    const uint64_t pairs = (a.tiles + 1) / 2;
-    const uint64_t wstride = (uint64_t)gridDim.x * kWaves;
-    for (uint64_t p = (uint64_t)blockIdx.x * kWaves + wave; p < pairs; p += wstride) {
-        const uint64_t t0 = 2 * p, t1 = 2 * p + 1;
-        const bool has1 = t1 < a.tiles;
-        const uint64_t b0 = insert_zeros5(t0 * 32 + col, a);
-        const uint64_t b1 = has1 ? insert_zeros5(t1 * 32 + col, a) : b0;
-        float2 x0[8], x1[8];
-#pragma unroll
-        for (int s = 0; s < 8; ++s) x0[s] = ld_nt(&a.sv[b0 + loff[s]]);
-        if (has1) {
-#pragma unroll
-            for (int s = 0; s < 8; ++s) x1[s] = ld_nt(&a.sv[b1 + loff[s]]);
-        }
-        floatx16 acc0 = {0}, acc1 = {0};
-#pragma unroll
-        for (int s = 0; s < 8; ++s) acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(A[s], x0[s].x, acc0, 0, 0, 0);
-#pragma unroll
-        for (int s = 0; s < 8; ++s) acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(A[8 + s], x0[s].y, acc0, 0, 0, 0);
-        if (has1) {
-#pragma unroll
-            for (int s = 0; s < 8; ++s) acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(A[s], x1[s].x, acc1, 0, 0, 0);
-#pragma unroll
-            for (int s = 0; s < 8; ++s)
-                acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(A[8 + s], x1[s].y, acc1, 0, 0, 0);
-        }
-#pragma unroll
-        for (int r = 0; r < 8; ++r)
-            st_nt(make_float2(acc0[r], acc0[r + 8]), &a.sv[b0 + soff[r]]);
-        if (has1) {
-#pragma unroll
-            for (int r = 0; r < 8; ++r)
-                st_nt(make_float2(acc1[r], acc1[r + 8]), &a.sv[b1 + soff[r]]);
-        }
-    }
+// pdep of `v` into the bit positions listed in `pos` (ascending).
+uint64_t deposit(uint64_t v, const std::vector<int>& pos) {
+    uint64_t r = 0;
+    for (size_t j = 0; j < pos.size(); ++j)
+        if ((v >> j) & 1ull) r |= 1ull << pos[j];
+    return r;
 }
 
 }  // namespace
 
 namespace aqs_hip {
 
-// Stacked real operand in per-lane order.
-//   k = 5: apack[(rt*32 + s)*64 + l] = S[32*rt + l%32][2s + l/32]   (S: 64 x 64)
-//   k = 4: apack[s*64 + l]           = S[l%32][2s + l/32]           (S: 32 x 32)
-// with S[R][C] = Re/Im blocks of U as in the header comment.
+// Per-lane operand order.
+//   k = 5: [Ur | Ui | Ur + Ui] x 16 steps: apack[(m*16 + s)*64 + l] = M_m[l % 32][2s + l/32]
+//   k = 4: apack[s*64 + l] = [Ur; Ui][l % 32][2s + l/32], s < 8
 std::vector<float> pack_mfma_operand(int k, const double* u) {
     const int K = 1 << k;
-    auto S = [&](int R, int C) -> double {
-        const int r = R % K, c = C % K;
-        const double ur = u[2 * (r * K + c)], ui = u[2 * (r * K + c) + 1];
-        const bool rim = R >= K, cim = C >= K;
-        if (!rim && !cim) return ur;
-        if (!rim && cim) return -ui;
-        if (rim && !cim) return ui;
-        return ur;
-    };
+    auto ur = [&](int r, int c) { return u[2 * (r * K + c)]; };
+    auto ui = [&](int r, int c) { return u[2 * (r * K + c) + 1]; };
     std::vector<float> out;
     if (k == 5) {
-        out.resize(2 * 32 * 64);
-        for (int rt = 0; rt < 2; ++rt)
-            for (int s = 0; s < 32; ++s)
-                for (int l = 0; l < 64; ++l) out[(rt * 32 + s) * 64 + l] = (float)S(32 * rt + l % 32, 2 * s + l / 32);
+        out.resize(3 * 16 * 64);
+        for (int m = 0; m < 3; ++m)
+            for (int s = 0; s < 16; ++s)
+                for (int l = 0; l < 64; ++l) {
+                    const int r = l % 32, c = 2 * s + l / 32;
+                    const double v = m == 0 ? ur(r, c) : (m == 1 ? ui(r, c) : ur(r, c) + ui(r, c));
+                    out[(m * 16 + s) * 64 + l] = (float)v;
+                }
     } else if (k == 4) {
-        out.resize(16 * 64);
-        for (int s = 0; s < 16; ++s)
-            for (int l = 0; l < 64; ++l) out[s * 64 + l] = (float)S(l % 32, 2 * s + l / 32);
+        out.resize(8 * 64);
+        for (int s = 0; s < 8; ++s)
+            for (int l = 0; l < 64; ++l) {
+                const int R = l % 32, c = 2 * s + l / 32;
+                out[s * 64 + l] = (float)(R < 16 ? ur(R, c) : ui(R - 16, c));
+            }
     }
     return out;
 }
@@ -189,31 +191,58 @@ std::vector<float> pack_mfma_operand(int k, const double* u) {
 bool apply_matk_mfma(aqs_state s, const aqs_op& op, const float* dev_apack) {
     const int k = op.ntargets;
     if (s->precision != AQS_C64 || op.ncontrols != 0 || (k != 4 && k != 5)) return false;
-    if (s->n > 32 || s->n < k + 5) return false;  // 32-bit offsets; at least one 32-column tile
+    if (s->n < k + 5 || s->n > 36) return false;
+    const int n = s->n;
+    // target bit positions (op order: targets[0] = most significant sub-index bit)
+    std::vector<int> tbits;
+    for (int j = 0; j < k; ++j) tbits.push_back(qpos(n, op.targets[j]));
+    std::vector<int> sorted_t = tbits;
+    std::sort(sorted_t.begin(), sorted_t.end());
+    std::vector<int> freeb;
+    for (int b = 0; b < n && (int)freeb.size() < 5; ++b)
+        if (std::find(tbits.begin(), tbits.end(), b) == tbits.end()) freeb.push_back(b);
+    std::vector<int> T = sorted_t;
+    T.insert(T.end(), freeb.begin(), freeb.end());
+    std::sort(T.begin(), T.end());
+    // the tile covers bits 0..4, so float4 pairs and 256-byte runs hold
+    if (T[0] != 0 || T.size() != (size_t)(k + 5)) return false;
+    auto tidx = [&](int bit) { return (int)(std::find(T.begin(), T.end(), bit) - T.begin()); };
+
     MfmaArgs a{};
-    a.sv = (float2*)s->amps;
-    std::vector<int> pos;
-    for (int j = 0; j < k; ++j) pos.push_back(qpos(s->n, op.targets[j]));
-    std::sort(pos.begin(), pos.end());
-    a.nfixed = k;
-    for (int j = 0; j < k; ++j) a.fixed[j] = pos[j];
+    a.sv4 = (float4*)s->amps;
+    a.tiles = s->N >> (k + 5);
+    a.ntile_bits = k + 5;
+    for (int j = 0; j < k + 5; ++j) a.tpos[j] = T[j];
+    const int chunks = (32 << k) / 128;
+    for (int j = 0; j < chunks; ++j) a.gj[j] = (uint32_t)(deposit((uint64_t)(128 * j), T) >> 1);
+    for (int l = 0; l < 64; ++l) a.glane[l] = (uint32_t)(deposit((uint64_t)(2 * l), T) >> 1);
     const int K = 1 << k;
-    for (int r = 0; r < K; ++r) {
-        uint32_t o = 0;
-        for (int j = 0; j < k; ++j)
-            if ((r >> (k - 1 - j)) & 1) o |= (uint32_t)qbit(s->n, op.targets[j]);
-        a.off[r] = o;
+    for (int r = 0; r < 32; ++r) {
+        uint32_t v = 0;
+        if (r < K)
+            for (int j = 0; j < k; ++j)
+                if ((r >> (k - 1 - j)) & 1) v |= 1u << tidx(tbits[j]);
+        a.tr[r] = v;
     }
-    a.tiles = (s->N >> k) / 32;
+    for (int c = 0; c < 32; ++c) {
+        uint32_t v = 0;
+        for (int j = 0; j < 5; ++j)
+            if ((c >> j) & 1) v |= 1u << tidx(freeb[j]);
+        a.tc[c] = v;
+    }
     a.apack = dev_apack;
-    const uint64_t work = (k == 5) ? a.tiles : (a.tiles + 1) / 2;
-    uint64_t blocks = (work + kWaves - 1) / kWaves;
-    blocks = std::min<uint64_t>(std::max<uint64_t>(blocks, 1), 8192);
+    hipDeviceProp_t prop;
+    static int cus = 0;
+    if (!cus) {
+        AQS_HIP(hipGetDeviceProperties(&prop, current_device()));
+        cus = prop.multiProcessorCount;
+    }
+    const uint64_t blocks = std::min<uint64_t>(a.tiles, (uint64_t)cus * 16);
     ProfScope ps(s, kSlotMfma);
     if (k == 5)
-        hipLaunchKernelGGL(k_matk32_mfma, dim3((unsigned)blocks), dim3(kBlock), 0, s->stream, a);
+        hipLaunchKernelGGL(k_matk_mfma<32>, dim3((unsigned)blocks), dim3(64), 0, s->stream, a);
     else
-        hipLaunchKernelGGL(k_matk16_mfma, dim3((unsigned)blocks), dim3(kBlock), 0, s->stream, a);
+        hipLaunchKernelGGL(k_matk_mfma<16>, dim3((unsigned)blocks), dim3(64), 0, s->stream, a);
     AQS_HIP(hipGetLastError());
     return true;
 }

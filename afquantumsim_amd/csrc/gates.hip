@@ -78,24 +78,42 @@ struct Mat1V4Args {
     float m[8];
 };
 
-__global__ __launch_bounds__(kBlock) void k_mat1_v4(Mat1V4Args a) {
-    const uint64_t base = (uint64_t)blockIdx.x * (kBlock * kUnroll) + threadIdx.x;
-    float4 lo[kUnroll], hi[kUnroll];
-    uint64_t idx[kUnroll];
+template <typename IT>
+__device__ __forceinline__ IT insert_zeros_t(IT t, const Fixed& f) {
 #pragma unroll
-    for (int j = 0; j < kUnroll; ++j) {
-        const uint64_t u = base + (uint64_t)j * kBlock;
-        idx[j] = (u < a.units) ? (insert_zeros(u, a.fixed) | a.cmask) : ~0ull;
-        if (idx[j] != ~0ull) {
+    for (int j = 0; j < 8; ++j) {
+        if (j >= f.n) break;
+        const int p = f.pos[j];
+        t = ((t >> p) << (p + 1)) | (t & ((((IT)1) << p) - (IT)1));
+    }
+    return t;
+}
+
+// U independent units per thread; IT = index type (32-bit when the float4
+// index space fits).  The variant is chosen per (qubits, target bit) by
+// aqs_autotune.
+template <int U, typename IT>
+__global__ __launch_bounds__(kBlock) void k_mat1_v4(Mat1V4Args a) {
+    const IT units = (IT)a.units, stride = (IT)a.stride, cmask = (IT)a.cmask;
+    const IT base = (IT)blockIdx.x * (IT)(kBlock * U) + (IT)threadIdx.x;
+    float4 lo[U], hi[U];
+    IT idx[U];
+    bool ok[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+        const IT u = base + (IT)j * kBlock;
+        ok[j] = u < units;
+        idx[j] = insert_zeros_t<IT>(u, a.fixed) | cmask;
+        if (ok[j]) {
             lo[j] = ld_nt(&a.sv[idx[j]]);
-            hi[j] = ld_nt(&a.sv[idx[j] + a.stride]);
+            hi[j] = ld_nt(&a.sv[idx[j] + stride]);
         }
     }
     const float m0r = a.m[0], m0i = a.m[1], m1r = a.m[2], m1i = a.m[3];
     const float m2r = a.m[4], m2i = a.m[5], m3r = a.m[6], m3i = a.m[7];
 #pragma unroll
-    for (int j = 0; j < kUnroll; ++j) {
-        if (idx[j] == ~0ull) continue;
+    for (int j = 0; j < U; ++j) {
+        if (!ok[j]) continue;
         float4 x = lo[j], y = hi[j], nx, ny;
         // amplitude pair (x.xy, y.xy)
         nx.x = m0r * x.x - m0i * x.y + m1r * y.x - m1i * y.y;
@@ -108,8 +126,43 @@ __global__ __launch_bounds__(kBlock) void k_mat1_v4(Mat1V4Args a) {
         ny.z = m2r * x.z - m2i * x.w + m3r * y.z - m3i * y.w;
         ny.w = m2r * x.w + m2i * x.z + m3r * y.w + m3i * y.z;
         st_nt(nx, &a.sv[idx[j]]);
-        st_nt(ny, &a.sv[idx[j] + a.stride]);
+        st_nt(ny, &a.sv[idx[j] + stride]);
     }
+}
+
+// Launch-shape variants of the float4 MAT1 pass.
+constexpr int kMat1Variants = 4;
+
+void launch_mat1_v4(const Mat1V4Args& a, int variant, uint64_t n_float4, hipStream_t st) {
+    const bool fits32 = n_float4 <= (1ull << 32);
+    if (!fits32 && (variant == 0 || variant == 3)) variant = 1;
+    switch (variant) {
+        case 0:
+            hipLaunchKernelGGL((k_mat1_v4<1, uint32_t>), dim3(blocks_for(a.units, 1)), dim3(kBlock), 0, st, a);
+            break;
+        case 1:
+            hipLaunchKernelGGL((k_mat1_v4<2, uint64_t>), dim3(blocks_for(a.units, 2)), dim3(kBlock), 0, st, a);
+            break;
+        case 2:
+            hipLaunchKernelGGL((k_mat1_v4<4, uint64_t>), dim3(blocks_for(a.units, 4)), dim3(kBlock), 0, st, a);
+            break;
+        default:
+            hipLaunchKernelGGL((k_mat1_v4<2, uint32_t>), dim3(blocks_for(a.units, 2)), dim3(kBlock), 0, st, a);
+            break;
+    }
+}
+
+// Tuned variant per (device, qubits, target bit); -1 = not tuned (variant 0).
+std::mutex g_tune_mutex;
+int g_tune[8][AQS_MAX_QUBITS + 1][AQS_MAX_QUBITS];
+bool g_tune_init = false;
+
+int tuned_variant(int n, int t) {
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    if (!g_tune_init) return 0;
+    const int d = current_device() & 7;
+    const int v = g_tune[d][n][t];
+    return v < 0 ? 0 : v;
 }
 
 // MAT1, complex64, target bit 0 (controls >= 1): the pair is one float4.
@@ -519,7 +572,7 @@ void apply_c64_fast_or_generic(aqs_state s, const aqs_op& op) {
                     a.stride = 1ull << (t - 1);
                     a.units = s->N >> (2 + cb.cpos.size());
                     a.fixed = make_fixed(fp);
-                    hipLaunchKernelGGL(k_mat1_v4, dim3(blocks_for(a.units, kUnroll)), dim3(kBlock), 0, s->stream, a);
+                    launch_mat1_v4(a, tuned_variant(n, t), s->N / 2, s->stream);
                 } else {
                     a.units = s->N >> (1 + cb.cpos.size());
                     a.fixed = make_fixed(fp);
@@ -760,5 +813,65 @@ extern "C" aqs_status aqs_apply(aqs_state s, const aqs_op* ops, size_t count) {
             if (ops[i].kind == AQS_OP_MATK && i > 0) AQS_HIP(hipStreamSynchronize(s->stream));
             apply_op(s, ops[i]);
         }
+    });
+}
+
+// Launch-shape autotuner for the one-qubit pass: for every target bit of the
+// state's size, times each variant with the identity matrix (the state is
+// left unchanged) and keeps the fastest for that (qubits, target bit).
+extern "C" aqs_status aqs_autotune(aqs_state s, int reps, int* variant_by_target) {
+    return guard([&] {
+        if (!s || !s->amps) fail(AQS_ERR_INVALID_ARGUMENT, "null state handle");
+        if (s->precision != AQS_C64) fail(AQS_ERR_INVALID_ARGUMENT, "autotune runs on a complex64 state");
+        const int n = s->n;
+        if (reps < 1) reps = 3;
+        {
+            std::lock_guard<std::mutex> lock(g_tune_mutex);
+            if (!g_tune_init) {
+                for (auto& d : g_tune)
+                    for (auto& q : d)
+                        for (auto& t : q) t = -1;
+                g_tune_init = true;
+            }
+        }
+        hipEvent_t e0, e1;
+        AQS_HIP(hipEventCreate(&e0));
+        AQS_HIP(hipEventCreate(&e1));
+        const int d = current_device() & 7;
+        for (int t = 0; t < n; ++t) {
+            int best = 0;
+            if (t >= 1 && n >= 2) {
+                Mat1V4Args a{};
+                a.sv = (float4*)s->amps;
+                a.fixed = make_fixed({t - 1});
+                a.stride = 1ull << (t - 1);
+                a.units = s->N >> 2;
+                a.m[0] = 1.f;
+                a.m[6] = 1.f;
+                float best_ms = 1e30f;
+                for (int v = 0; v < kMat1Variants; ++v) {
+                    launch_mat1_v4(a, v, s->N / 2, s->stream);
+                    AQS_HIP(hipEventRecord(e0, s->stream));
+                    for (int r = 0; r < reps; ++r) launch_mat1_v4(a, v, s->N / 2, s->stream);
+                    AQS_HIP(hipEventRecord(e1, s->stream));
+                    AQS_HIP(hipEventSynchronize(e1));
+                    float ms = 0.f;
+                    AQS_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best = v;
+                    }
+                }
+                AQS_HIP(hipGetLastError());
+            }
+            {
+                std::lock_guard<std::mutex> lock(g_tune_mutex);
+                g_tune[d][n][t] = best;
+            }
+            // qubit q has target bit n-1-q
+            if (variant_by_target) variant_by_target[n - 1 - t] = best;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
     });
 }

@@ -80,7 +80,7 @@ EXPORTS = [
     "aqs_probabilities", "aqs_amplitude", "aqs_sample", "aqs_measure_qubit", "aqs_collapse_to",
     "aqs_inner", "aqs_profile_enable", "aqs_profile_read", "aqs_timer_start", "aqs_timer_stop",
     "aqs_batch_create", "aqs_batch_autotune", "aqs_batch_run", "aqs_batch_state", "aqs_batch_info",
-    "aqs_batch_destroy",
+    "aqs_batch_destroy", "aqs_autotune",
 ]
 
 _lib = None
@@ -140,6 +140,7 @@ def load(path: str = LIB_PATH):
         "aqs_batch_info": [vp, ctypes.POINTER(sz), ctypes.POINTER(sz), ctypes.POINTER(i32), ctypes.POINTER(i32),
                            ctypes.POINTER(i32)],
         "aqs_batch_destroy": [vp],
+        "aqs_autotune": [vp, i32, ctypes.POINTER(i32)],
     }
     for name, args in sigs.items():
         f = getattr(L, name)
@@ -235,6 +236,12 @@ class State:
         arr, keep = pack_ops(ops)
         check(_lib.aqs_apply(self.h, arr, len(ops)))
         del keep
+
+    def autotune(self, reps: int = 3):
+        """Pick the fastest one-qubit launch shape per target qubit (state unchanged)."""
+        out = (ctypes.c_int * self.n)()
+        check(_lib.aqs_autotune(self.h, reps, out))
+        return list(out)
 
     def apply_schedule(self, sch: "Schedule"):
         check(_lib.aqs_apply_schedule(self.h, sch.h))

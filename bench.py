@@ -168,6 +168,9 @@ def run(args):
     alg_bytes = algorithmic_bytes(lowered, n)
 
     st = hip.State(n, hip.C64)
+    t0 = time.perf_counter()
+    tuned = st.autotune(reps=3)
+    log(f"autotuned one-qubit launch shapes in {time.perf_counter() - t0:.2f} s: {tuned}")
 
     def step():
         st.set_basis(0)

@@ -134,6 +134,13 @@ aqs_status aqs_state_download(aqs_state s, void* host);
 /* Apply ops in order, one HIP kernel pass each. */
 aqs_status aqs_apply(aqs_state s, const aqs_op* ops, size_t count);
 
+/* Launch-shape autotuner for the one-qubit pass at this state's size: times
+ * every variant for every target qubit with the identity matrix (the state
+ * is unchanged) and keeps the fastest per (qubit count, target) for later
+ * launches on this device.  variant_by_qubit (n ints, may be NULL) receives
+ * the choice per target qubit. */
+aqs_status aqs_autotune(aqs_state s, int reps, int* variant_by_qubit);
+
 /* ---- compile(): gate fusion (replaces QCircuit::compile, src/quantum.cpp:199-210) */
 
 /* Fuse ops into blocks of at most max_qubits (1..5) qubits.  Each block is

@@ -341,6 +341,22 @@ def test_errors_match_reference_types():
         st.prob1(9)
 
 
+def test_autotune_keeps_state_and_results():
+    """The launch-shape tuner runs the identity and leaves the state as is;
+    every tuned variant then gives the oracle's answer."""
+    n = 14
+    sv = random_state(n, 90).astype(np.complex64)
+    st = hip.State(n, hip.C64)
+    st.upload(sv)
+    variants = st.autotune(reps=2)
+    assert len(variants) == n and all(0 <= v < 4 for v in variants)
+    np.testing.assert_array_equal(st.download(), sv)
+    gates = [oracle.gate("U3", q, angle=0.3 + q, phi=0.1 * q, lam=-0.2) for q in range(n)]
+    want = oracle.run(sv.astype(np.complex128), n, gates)
+    st.apply(ops.lower_all(gates))
+    np.testing.assert_allclose(st.download(), want, atol=TOL64)
+
+
 @pytest.mark.parametrize("n", [26])
 def test_large_state_round_trip(n):
     """Size-independent properties at a large state: a brickwork layer

@@ -80,8 +80,23 @@ EXPORTS = [
     "aqs_probabilities", "aqs_amplitude", "aqs_sample", "aqs_measure_qubit", "aqs_collapse_to",
     "aqs_inner", "aqs_profile_enable", "aqs_profile_read", "aqs_timer_start", "aqs_timer_stop",
     "aqs_batch_create", "aqs_batch_autotune", "aqs_batch_run", "aqs_batch_state", "aqs_batch_info",
-    "aqs_batch_destroy", "aqs_autotune",
+    "aqs_batch_destroy", "aqs_autotune", "aqs_server_create", "aqs_server_set_starvation_limit",
+    "aqs_server_submit", "aqs_server_wait", "aqs_server_stats", "aqs_server_destroy",
 ]
+
+
+class RequestResult(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("status", ctypes.c_int32), ("nqubits", ctypes.c_int32),
+                ("shots", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("passes", ctypes.c_uint64),
+                ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double),
+                ("norm2", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("admitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("rejected", ctypes.c_uint64),
+                ("evictions", ctypes.c_uint64), ("pending", ctypes.c_uint64), ("running", ctypes.c_int32),
+                ("max_concurrent", ctypes.c_int32), ("resident_bytes", ctypes.c_uint64),
+                ("peak_resident_bytes", ctypes.c_uint64), ("budget_bytes", ctypes.c_uint64)]
 
 _lib = None
 
@@ -141,6 +156,12 @@ def load(path: str = LIB_PATH):
                            ctypes.POINTER(i32)],
         "aqs_batch_destroy": [vp],
         "aqs_autotune": [vp, i32, ctypes.POINTER(i32)],
+        "aqs_server_create": [sz, i32, i32, ctypes.POINTER(vp)],
+        "aqs_server_set_starvation_limit": [vp, ctypes.c_double],
+        "aqs_server_submit": [vp, u64, i32, ctypes.POINTER(Op), sz, dp, ctypes.c_uint32, i32],
+        "aqs_server_wait": [vp, u64, i32, ctypes.POINTER(RequestResult), ctypes.POINTER(ctypes.c_uint64), vp],
+        "aqs_server_stats": [vp, ctypes.POINTER(ServerStats)],
+        "aqs_server_destroy": [vp],
     }
     for name, args in sigs.items():
         f = getattr(L, name)
@@ -403,6 +424,57 @@ class Batch:
     def close(self):
         if self.h:
             check(_lib.aqs_batch_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching request server on one device."""
+
+    def __init__(self, budget_bytes: int, workers: int = 4, fuse_qubits: int = 5):
+        L = load()
+        h = ctypes.c_void_p()
+        check(L.aqs_server_create(int(budget_bytes), workers, fuse_qubits, ctypes.byref(h)))
+        self.h = h
+        self._shots = {}
+        self._n = {}
+        self._keep = {}
+
+    def set_starvation_limit(self, seconds: float):
+        check(_lib.aqs_server_set_starvation_limit(self.h, seconds))
+
+    def submit(self, rid: int, n: int, ops, uniforms, keep_state: bool = False):
+        arr, keep = pack_ops(ops)
+        u = np.ascontiguousarray(uniforms, dtype=np.float64)
+        check(_lib.aqs_server_submit(self.h, rid, n, arr, len(ops), _dp(u), len(u), int(keep_state)))
+        del keep
+        self._shots[rid] = len(u)
+        self._n[rid] = n
+        self._keep[rid] = keep_state
+
+    def wait(self, rid: int, timeout_ms: int = -1):
+        res = RequestResult()
+        idx = np.zeros(max(1, self._shots[rid]), dtype=np.uint64)
+        state = np.zeros(1 << self._n[rid], dtype=np.complex64) if self._keep[rid] else None
+        check(_lib.aqs_server_wait(self.h, rid, timeout_ms, ctypes.byref(res),
+                                   idx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                   state.ctypes.data if state is not None else None))
+        out = {f: getattr(res, f) for f, _ in RequestResult._fields_}
+        return out, idx[: self._shots.pop(rid)], state
+
+    def stats(self):
+        s = ServerStats()
+        check(_lib.aqs_server_stats(self.h, ctypes.byref(s)))
+        return {f: getattr(s, f) for f, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.h:
+            check(_lib.aqs_server_destroy(self.h))
             self.h = None
 
     def __del__(self):

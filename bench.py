@@ -122,6 +122,95 @@ def batched(args):
     return out
 
 
+def served_requests(count, seed):
+    """Brickwork requests: n log-uniform over 12..28 (P(n) ~ 1/n on the
+    integers), depth ~ U{10..100}, 1000 shots with seeded uniforms."""
+    rng = np.random.default_rng(seed)
+    ns = np.arange(12, 29)
+    p = (1.0 / ns) / np.sum(1.0 / ns)
+    out = []
+    for i in range(count):
+        n = int(rng.choice(ns, p=p))
+        depth = int(rng.integers(10, 101))
+        gates = circuits.brickwork(n, depth, seed=50_000 + i)
+        out.append({"id": i, "n": n, "depth": depth, "ops": ops.lower_all(gates), "gates": gates,
+                    "u": rng.uniform(size=1000)})
+    return out
+
+
+def served(args):
+    """Poisson arrivals of mixed-size brickwork requests into the
+    continuous-batching server; p50/p99 latency (arrival -> sampled
+    indices), throughput, and the same requests one at a time."""
+    import threading
+    budget = 64 << 30
+    workers = 4
+    # service-time calibration: requests alone, one after another
+    calib = served_requests(24, seed=77)
+    srv = hip.Server(budget, workers=1)
+    t0 = time.perf_counter()
+    for r in calib:
+        srv.submit(r["id"], r["n"], r["ops"], r["u"])
+        srv.wait(r["id"])
+    alone_s = (time.perf_counter() - t0) / len(calib)
+    srv.close()
+    lam = 0.7 / alone_s
+    reqs = served_requests(args.serve_requests, seed=78)
+    rng = np.random.default_rng(79)
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, size=len(reqs)))
+    srv = hip.Server(budget, workers=workers)
+    lat = {}
+    submitted = threading.Semaphore(0)
+
+    def collector():
+        for r in reqs:
+            submitted.acquire()
+            res, idx, _ = srv.wait(r["id"])
+            lat[r["id"]] = (res["t_done"] - res["t_submit"], res["t_admit"] - res["t_submit"])
+
+    th = threading.Thread(target=collector)
+    th.start()
+    start = time.perf_counter()
+    for r, at in zip(reqs, arrivals):
+        delay = start + at - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        srv.submit(r["id"], r["n"], r["ops"], r["u"])
+        submitted.release()
+    th.join()
+    wall = time.perf_counter() - start
+    stats = srv.stats()
+    srv.close()
+    l = np.array([lat[r["id"]][0] for r in reqs])
+    q = np.array([lat[r["id"]][1] for r in reqs])
+    gates = sum(len(r["ops"]) for r in reqs)
+    # accuracy: the served result of a few requests against complex128
+    worst = 0.0
+    chk = hip.Server(budget, workers=2)
+    for r in [x for x in reqs if x["n"] <= 22][:6]:
+        chk.submit(10_000 + r["id"], r["n"], r["ops"], r["u"][:10], keep_state=True)
+        _, _, psi = chk.wait(10_000 + r["id"])
+        ref = hip.State(r["n"], hip.C128)
+        ref.apply(ops.lower_all(r["gates"], exact=True))
+        x = ref.download()
+        ov = np.vdot(x, psi.astype(np.complex128))
+        worst = max(worst, 1.0 - abs(ov) ** 2 / (np.vdot(psi, psi).real * np.vdot(x, x).real))
+        ref.close()
+    chk.close()
+    out = {"requests": len(reqs), "arrival_rate_per_s": round(lam, 2), "workers": workers,
+           "hbm_budget_bytes": budget, "p50_ms": round(1e3 * float(np.percentile(l, 50)), 2),
+           "p99_ms": round(1e3 * float(np.percentile(l, 99)), 2),
+           "mean_queue_ms": round(1e3 * float(q.mean()), 2),
+           "throughput_req_per_s": round(len(reqs) / wall, 2),
+           "gate_apps_per_s": round(gates / wall, 1),
+           "alone_mean_service_ms": round(1e3 * alone_s, 2),
+           "max_concurrent": stats["max_concurrent"], "peak_resident_bytes": stats["peak_resident_bytes"],
+           "max_infidelity_checked": worst, "shots_per_request": 1000,
+           "qubits": "log-uniform 12..28", "depth": "U{10..100}"}
+    log(f"served: {out}")
+    return out
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "traffic_u3_28q.json")
     if os.path.exists(path):
@@ -243,6 +332,7 @@ def run(args):
     st.close()
 
     batch = batched(args) if not args.no_batch else None
+    serve = served(args) if args.serve_requests > 0 else None
 
     cpu = cpu_baseline(gates, n) if not args.no_cpu else None
     if cpu:
@@ -269,6 +359,7 @@ def run(args):
         "roofline": roofline,
         "fused": fused,
         "batched": batch,
+        "served": serve,
         "cpu_baseline": cpu,
         "device": dev,
     }
@@ -282,6 +373,8 @@ def main():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     p.add_argument("--no-batch", action="store_true", help="skip the batched-states section")
+    p.add_argument("--serve-requests", type=int, default=120,
+                   help="requests in the served-stream section (0 skips it)")
     p.add_argument("--probe", action="store_true",
                    help="run a short fixed kernel sequence for hardware-counter collection and exit")
     args = p.parse_args()

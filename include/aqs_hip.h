@@ -207,6 +207,50 @@ aqs_status aqs_batch_state(aqs_batch b, int index, void* host);
 aqs_status aqs_batch_info(aqs_batch b, size_t* passes, size_t* source_ops, int* block, int* lds_qubits, int* grid);
 aqs_status aqs_batch_destroy(aqs_batch b);
 
+/* ---- request server (no reference counterpart) ---------------------------
+ * Continuous batching of circuits of mixed size: W worker threads, each with
+ * its own HIP stream, run admitted requests concurrently; a request is
+ * admitted when a worker is free and the resident device states plus its
+ * own fit the HBM budget; idle states stay resident for reuse. */
+
+typedef struct aqs_server_s* aqs_server;
+
+typedef struct aqs_request_result {
+    uint64_t id;
+    int32_t status;      /* aqs_status of the request */
+    int32_t nqubits;
+    uint32_t shots;
+    uint32_t reserved;
+    uint64_t passes;     /* fused device passes it ran */
+    double t_submit;     /* steady-clock seconds */
+    double t_admit;
+    double t_done;
+    double norm2;        /* FP64 norm^2 of the final state */
+} aqs_request_result;
+
+typedef struct aqs_server_stats_t {
+    uint64_t admitted, completed, rejected, evictions, pending;
+    int32_t running, max_concurrent;
+    uint64_t resident_bytes, peak_resident_bytes, budget_bytes;
+} aqs_server_stats_t;
+
+aqs_status aqs_server_create(size_t hbm_budget_bytes, int workers, int fuse_qubits, aqs_server* out);
+/* A request waiting longer than this blocks later requests from jumping
+ * ahead of it (default 0.25 s). */
+aqs_status aqs_server_set_starvation_limit(aqs_server s, double seconds);
+/* Queue a circuit (ops and uniforms are copied).  Fails with
+ * AQS_ERR_INVALID_ARGUMENT when the request can never fit the budget. */
+aqs_status aqs_server_submit(aqs_server s, uint64_t id, int nqubits, const aqs_op* ops, size_t nops,
+                             const double* uniforms, uint32_t shots, int keep_state);
+/* Wait for request id (timeout_ms < 0: no limit); indices receives `shots`
+ * sampled basis indices, state_host the final complex64 state when the
+ * request was submitted with keep_state. */
+aqs_status aqs_server_wait(aqs_server s, uint64_t id, int timeout_ms, aqs_request_result* res, uint64_t* indices,
+                           void* state_host);
+aqs_status aqs_server_stats(aqs_server s, aqs_server_stats_t* out);
+/* Finishes queued work, then frees every resident state. */
+aqs_status aqs_server_destroy(aqs_server s);
+
 /* ---- timing ------------------------------------------------------------- */
 
 /* Per-kind kernel timing with HIP events on the state's stream.  While

@@ -1,26 +1,36 @@
-// compile(): gate fusion into dense k-qubit blocks (host side, FP64).
+// compile(): commutation-aware gate fusion (host side, FP64).
 //
 // The reference's QCircuit::compile (src/quantum.cpp:199-210) multiplies
-// every gate into one 2^n x 2^n matrix.  This pass instead groups the gate
-// list into blocks of at most max_qubits (<= 5) qubits:
+// every gate into one 2^n x 2^n matrix.  This pass turns the gate list into
+// device passes of three kinds:
 //
-//   - open blocks have disjoint qubit sets, so they commute and can grow in
-//     any order;
-//   - an op whose qubits meet open blocks is merged with them when the union
-//     fits in max_qubits, otherwise those blocks are closed (emitted) first
-//     and the op starts a new block;
-//   - an op wider than max_qubits is emitted unfused.
+//   dense blocks   <= max_qubits (<= 5) qubits, an FP64 unitary applied with
+//                  MFMA (4, 5 qubits) or the small-block kernel;
+//   diagonal table one sweep applying many deferred phase terms (diag.hip);
+//   plain ops      permutations that fit no open block, ops wider than a block.
 //
-// Each block's unitary is accumulated in FP64 (op matrices are built by
-// applying the op to the block's basis vectors) and rounded to FP32 once,
-// when the schedule is uploaded for a complex64 state.  One-qubit blocks are
-// emitted as MAT1 / DIAG1 passes, wider blocks as MATK passes (MFMA for
-// 4 and 5 qubits on complex64 states).
+// Rules, in program order:
+//   - open dense blocks have disjoint qubit sets, so they commute and grow in
+//     any order; an op that meets open blocks merges with them when the union
+//     fits, else those blocks are emitted first;
+//   - a run of consecutive diagonal ops (they commute) is reordered: ops that
+//     add no new qubit to the open blocks first, then by how soon a
+//     non-diagonal op next touches the qubits they would add;
+//   - a diagonal op that does not fit is deferred as phase terms instead of
+//     closing blocks; a later non-diagonal op touching a deferred term's
+//     qubits first emits the open blocks that meet the terms, then the terms
+//     (every gate added to those blocks after the term commutes with it: it is
+//     diagonal or acts on other qubits);
+//   - a Swap that fits no open block becomes a plain pass (it moves half the
+//     state; a dense block would move all of it).
+// Each block's unitary is accumulated in FP64 and rounded to FP32 once, at
+// upload for a complex64 state.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <complex>
 #include <cstring>
+#include <map>
 #include <vector>
 
 #include "internal.h"
@@ -28,16 +38,29 @@
 using namespace aqs_hip;
 using cd = std::complex<double>;
 
+namespace {
+
+struct DiagPass {
+    std::vector<int> required;            // amplitude bit positions
+    std::vector<int> anchor;              // amplitude bit positions (<= 5)
+    std::vector<std::vector<int>> chunks; // amplitude bit positions (<= 6 x 5)
+    std::vector<cd> table;                // [2^|anchor|][chunks][32]
+};
+
+}  // namespace
+
 struct aqs_schedule_s {
     int n = 0;
     size_t source_ops = 0;
     std::vector<aqs_op> passes;
     std::vector<std::vector<double>> mats;  // owned FP64 matrices (MATK passes)
+    std::vector<int> diag_index;            // -1 or index into diags
+    std::vector<DiagPass> diags;
     // device copies, built at first use for a given precision
     void* dev = nullptr;
     int dev_precision = -1;
-    std::vector<size_t> dev_off;   // byte offset of each pass's matrix
-    std::vector<char> dev_mfma;    // pass uses the MFMA packed layout
+    std::vector<size_t> dev_off;
+    std::vector<char> dev_mfma;
 };
 
 namespace {
@@ -125,10 +148,18 @@ std::vector<int> op_qubits(const aqs_op& op) {
     return q;
 }
 
+bool is_diagonal(const aqs_op& op) {
+    if (op.flags & AQS_CTRL_ANY) return false;
+    return op.kind == AQS_OP_PHASE || op.kind == AQS_OP_DIAG1;
+}
+
+// A swap touches half of the amplitudes as a plain pass but all of them as a
+// dense block, so it only joins blocks it can merge into.
+bool is_permutation(const aqs_op& op) { return op.kind == AQS_OP_SWAP && !(op.flags & AQS_CTRL_ANY); }
+
 struct Block {
     std::vector<int> qubits;  // local qubit j = bit k-1-j of the block index
     std::vector<cd> U;        // row-major 2^k x 2^k
-    size_t sources = 0;
 };
 
 int local_index(const std::vector<int>& qs, int q) {
@@ -137,7 +168,6 @@ int local_index(const std::vector<int>& qs, int q) {
     return -1;
 }
 
-// The op's matrix over the qubit list `qs` (all op qubits are in qs).
 std::vector<cd> op_matrix(const aqs_op& op, const std::vector<int>& qs) {
     const int k = (int)qs.size();
     const uint64_t K = 1ull << k;
@@ -155,7 +185,6 @@ std::vector<cd> op_matrix(const aqs_op& op, const std::vector<int>& qs) {
     return M;
 }
 
-// Re-express a block's matrix over a larger qubit list (block qubits subset).
 std::vector<cd> embed(const Block& b, const std::vector<int>& qs) {
     aqs_op op{};
     op.kind = AQS_OP_MATK;
@@ -181,40 +210,380 @@ std::vector<cd> matmul(const std::vector<cd>& A, const std::vector<cd>& B, uint6
     return C;
 }
 
-bool is_diagonal(const std::vector<cd>& U, uint64_t K) {
+bool diagonal_matrix(const std::vector<cd>& U, uint64_t K) {
     for (uint64_t i = 0; i < K; ++i)
         for (uint64_t j = 0; j < K; ++j)
             if (i != j && U[i * K + j] != cd(0)) return false;
     return true;
 }
 
-void emit_block(aqs_schedule_s* sch, Block&& b) {
-    const int k = (int)b.qubits.size();
-    const uint64_t K = 1ull << k;
-    aqs_op op{};
-    if (k == 1) {
-        op.ntargets = 1;
-        op.targets[0] = b.qubits[0];
-        op.kind = is_diagonal(b.U, 2) ? AQS_OP_DIAG1 : AQS_OP_MAT1;
-        for (int i = 0; i < 4; ++i) {
-            op.m[2 * i] = b.U[i].real();
-            op.m[2 * i + 1] = b.U[i].imag();
+// A deferred phase term: factor `phase` on amplitudes whose qubits in `q` are all 1.
+struct Term {
+    std::vector<int> q;
+    cd phase;
+};
+
+class Fuser {
+   public:
+    Fuser(aqs_schedule_s* sch, int n, int kmax) : sch_(sch), n_(n), kmax_(kmax) {}
+
+    void run(const aqs_op* ops, size_t count) {
+        // non-diagonal uses per qubit, for the look-ahead of diagonal runs
+        std::vector<std::vector<size_t>> uses(n_);
+        for (size_t i = 0; i < count; ++i)
+            if (!is_diagonal(ops[i]))
+                for (int q : op_qubits(ops[i])) uses[q].push_back(i);
+        size_t i = 0;
+        while (i < count) {
+            if (!is_diagonal(ops[i])) {
+                nondiag(ops[i]);
+                ++i;
+                continue;
+            }
+            size_t e = i;
+            while (e < count && is_diagonal(ops[e])) ++e;
+            std::vector<size_t> run;
+            for (size_t j = i; j < e; ++j) run.push_back(j);
+            // qubits already in open blocks at the start of the run
+            std::vector<char> inblock(n_, 0);
+            for (auto& b : open_)
+                for (int q : b.qubits) inblock[q] = 1;
+            auto next_use = [&](int q) {
+                auto it = std::lower_bound(uses[q].begin(), uses[q].end(), e);
+                return it == uses[q].end() ? (size_t)-1 : *it;
+            };
+            struct Key {
+                int fresh;
+                size_t soon;
+                size_t idx;
+            };
+            std::vector<Key> keys;
+            for (size_t j : run) {
+                int fresh = 0;
+                size_t soon = (size_t)-1;
+                for (int q : op_qubits(ops[j]))
+                    if (!inblock[q]) {
+                        ++fresh;
+                        soon = std::min(soon, next_use(q));
+                    }
+                keys.push_back({fresh, soon, j});
+            }
+            std::stable_sort(keys.begin(), keys.end(), [](const Key& a, const Key& b) {
+                if (a.fresh != b.fresh) return a.fresh < b.fresh;
+                return a.soon < b.soon;
+            });
+            for (auto& k : keys) diag(ops[k.idx]);
+            i = e;
         }
-        sch->passes.push_back(op);
-        sch->mats.emplace_back();
-        return;
+        flush_terms();
+        while (!open_.empty()) close_block(0);
     }
-    op.kind = AQS_OP_MATK;
-    op.ntargets = k;
-    for (int j = 0; j < k; ++j) op.targets[j] = b.qubits[j];
-    std::vector<double> u(2 * K * K);
-    for (uint64_t i = 0; i < K * K; ++i) {
-        u[2 * i] = b.U[i].real();
-        u[2 * i + 1] = b.U[i].imag();
+
+   private:
+    aqs_schedule_s* sch_;
+    int n_, kmax_;
+    std::vector<Block> open_;
+    std::vector<Term> terms_;
+
+    std::vector<size_t> touching(const std::vector<int>& S) {
+        std::vector<size_t> hit;
+        for (size_t b = 0; b < open_.size(); ++b)
+            for (int q : S)
+                if (local_index(open_[b].qubits, q) >= 0) {
+                    hit.push_back(b);
+                    break;
+                }
+        return hit;
     }
-    sch->passes.push_back(op);
-    sch->mats.push_back(std::move(u));
-}
+
+    std::vector<int> union_with(const std::vector<int>& S, const std::vector<size_t>& hit) {
+        std::vector<int> U = S;
+        for (size_t b : hit)
+            for (int q : open_[b].qubits)
+                if (std::find(U.begin(), U.end(), q) == U.end()) U.push_back(q);
+        std::sort(U.begin(), U.end());
+        return U;
+    }
+
+    void merge(const aqs_op& op, const std::vector<size_t>& hit, const std::vector<int>& U) {
+        const uint64_t K = 1ull << U.size();
+        std::vector<cd> M(K * K);
+        for (uint64_t d = 0; d < K; ++d) M[d * K + d] = 1.0;
+        for (size_t b : hit) M = matmul(embed(open_[b], U), M, K);
+        M = matmul(op_matrix(op, U), M, K);
+        for (size_t j = hit.size(); j-- > 0;) open_.erase(open_.begin() + hit[j]);
+        Block nb;
+        nb.qubits = U;
+        nb.U = std::move(M);
+        open_.push_back(std::move(nb));
+    }
+
+    void close_block(size_t idx) {
+        Block b = std::move(open_[idx]);
+        open_.erase(open_.begin() + idx);
+        const int k = (int)b.qubits.size();
+        const uint64_t K = 1ull << k;
+        aqs_op op{};
+        if (k == 1) {
+            op.ntargets = 1;
+            op.targets[0] = b.qubits[0];
+            op.kind = diagonal_matrix(b.U, 2) ? AQS_OP_DIAG1 : AQS_OP_MAT1;
+            for (int i = 0; i < 4; ++i) {
+                op.m[2 * i] = b.U[i].real();
+                op.m[2 * i + 1] = b.U[i].imag();
+            }
+            push(op, {});
+            return;
+        }
+        op.kind = AQS_OP_MATK;
+        op.ntargets = k;
+        for (int j = 0; j < k; ++j) op.targets[j] = b.qubits[j];
+        std::vector<double> u(2 * K * K);
+        for (uint64_t i = 0; i < K * K; ++i) {
+            u[2 * i] = b.U[i].real();
+            u[2 * i + 1] = b.U[i].imag();
+        }
+        push(op, std::move(u));
+    }
+
+    void push(const aqs_op& op, std::vector<double> mat) {
+        sch_->passes.push_back(op);
+        sch_->mats.push_back(std::move(mat));
+        sch_->diag_index.push_back(-1);
+    }
+
+    void close_touching(const std::vector<int>& S) {
+        auto hit = touching(S);
+        for (size_t j = hit.size(); j-- > 0;) close_block(hit[j]);
+    }
+
+    void nondiag(const aqs_op& op) {
+        std::vector<int> S = op_qubits(op);
+        // deferred phase terms on these qubits go first
+        bool need = false;
+        for (auto& t : terms_)
+            for (int q : t.q)
+                if (std::find(S.begin(), S.end(), q) != S.end()) need = true;
+        if (need) flush_terms();
+        auto hit = touching(S);
+        if ((int)S.size() > kmax_) {
+            close_touching(S);
+            raw(op);
+            return;
+        }
+        std::vector<int> U = union_with(S, hit);
+        if ((int)U.size() <= kmax_ && (!hit.empty() || !is_permutation(op))) {
+            merge(op, hit, U);
+            return;
+        }
+        close_touching(S);
+        if (is_permutation(op)) {
+            raw(op);
+            return;
+        }
+        std::sort(S.begin(), S.end());
+        Block nb;
+        nb.qubits = S;
+        nb.U = op_matrix(op, S);
+        open_.push_back(std::move(nb));
+    }
+
+    void diag(const aqs_op& op) {
+        std::vector<int> S = op_qubits(op);
+        auto hit = touching(S);
+        std::vector<int> U = union_with(S, hit);
+        if ((int)U.size() <= kmax_) {
+            merge(op, hit, U);
+            return;
+        }
+        // Defer only ops that would pull a qubit no open block holds yet into
+        // a full block (the long controlled-phase ladders); an op between
+        // qubits that are all in blocks closes them, as a non-diagonal op does.
+        bool fresh = false;
+        for (int q : S) {
+            bool held = false;
+            for (auto& b : open_)
+                if (local_index(b.qubits, q) >= 0) held = true;
+            if (!held) fresh = true;
+        }
+        if (!fresh || (int)S.size() > 5) {
+            close_touching(S);
+            if ((int)S.size() > kmax_) {
+                raw(op);
+                return;
+            }
+            std::sort(S.begin(), S.end());
+            Block nb;
+            nb.qubits = S;
+            nb.U = op_matrix(op, S);
+            open_.push_back(std::move(nb));
+            return;
+        }
+        // deferred as phase terms (DIAG1 = global d0 and d1/d0 on the target)
+        std::vector<int> ctl;
+        for (int j = 0; j < op.ncontrols; ++j) ctl.push_back(op.controls[j]);
+        if (op.kind == AQS_OP_PHASE) {
+            std::vector<int> q = ctl;
+            for (int j = 0; j < op.ntargets; ++j) q.push_back(op.targets[j]);
+            terms_.push_back({q, cd(op.m[0], op.m[1])});
+        } else {
+            const cd d0(op.m[0], op.m[1]), d1(op.m[6], op.m[7]);
+            terms_.push_back({ctl, d0});
+            std::vector<int> q = ctl;
+            q.push_back(op.targets[0]);
+            terms_.push_back({q, d1 / d0});
+        }
+    }
+
+    void raw(const aqs_op& op) {
+        std::vector<double> u;
+        if (op.kind == AQS_OP_MATK) {
+            const size_t K = (size_t)1 << op.ntargets;
+            u.assign(op.u, op.u + 2 * K * K);
+        }
+        push(op, std::move(u));
+    }
+
+    // Emit the deferred terms: close the open blocks they meet, then table passes.
+    void flush_terms() {
+        if (terms_.empty()) return;
+        std::vector<int> Q;
+        for (auto& t : terms_)
+            for (int q : t.q)
+                if (std::find(Q.begin(), Q.end(), q) == Q.end()) Q.push_back(q);
+        close_touching(Q);
+        cd global = 1.0;
+        std::vector<Term> left;
+        for (auto& t : terms_) {
+            if (t.q.empty())
+                global *= t.phase;
+            else
+                left.push_back(t);
+        }
+        terms_.clear();
+        while (!left.empty() || global != cd(1.0)) {
+            // anchor group: the (up to) 5 qubits in the most terms
+            std::map<int, int> deg;
+            for (auto& t : left)
+                for (int q : t.q) deg[q]++;
+            std::vector<std::pair<int, int>> byd(deg.begin(), deg.end());
+            std::stable_sort(byd.begin(), byd.end(), [](auto& a, auto& b) { return a.second > b.second; });
+            std::vector<int> G;
+            for (auto& p : byd)
+                if ((int)G.size() < 5) G.push_back(p.first);
+            std::vector<Term> covered, rest;
+            std::vector<int> outside;
+            for (auto& t : left) {
+                std::vector<int> o;
+                for (int q : t.q)
+                    if (std::find(G.begin(), G.end(), q) == G.end()) o.push_back(q);
+                bool ok = o.size() <= 1;
+                if (ok && o.size() == 1 && std::find(outside.begin(), outside.end(), o[0]) == outside.end()) {
+                    if (outside.size() >= 30)
+                        ok = false;
+                    else
+                        outside.push_back(o[0]);
+                }
+                (ok ? covered : rest).push_back(t);
+            }
+            if (covered.empty() && global == cd(1.0)) {
+                // cannot factorise: emit each remaining term as its own phase pass
+                for (auto& t : rest) {
+                    aqs_op op{};
+                    op.kind = AQS_OP_PHASE;
+                    op.ntargets = std::min<int>(5, (int)t.q.size());
+                    for (int j = 0; j < op.ntargets; ++j) op.targets[j] = t.q[j];
+                    for (size_t j = op.ntargets; j < t.q.size(); ++j) op.controls[op.ncontrols++] = t.q[j];
+                    op.m[0] = t.phase.real();
+                    op.m[1] = t.phase.imag();
+                    push(op, {});
+                }
+                break;
+            }
+            emit_table(G, outside, covered, global);
+            global = 1.0;
+            left = std::move(rest);
+        }
+    }
+
+    int pos(int q) const { return n_ - 1 - q; }
+
+    void emit_table(std::vector<int> G, std::vector<int> outside, const std::vector<Term>& terms, cd global) {
+        // drop anchor qubits no covered term uses
+        std::vector<int> used;
+        for (int g : G)
+            for (auto& t : terms)
+                if (std::find(t.q.begin(), t.q.end(), g) != t.q.end()) {
+                    used.push_back(g);
+                    break;
+                }
+        G = used;
+        // required qubits: in every term
+        std::vector<int> R;
+        if (global == cd(1.0) && !terms.empty())
+            for (int q : terms[0].q) {
+                bool all = true;
+                for (auto& t : terms)
+                    if (std::find(t.q.begin(), t.q.end(), q) == t.q.end()) all = false;
+                if (all) R.push_back(q);
+            }
+        std::sort(outside.begin(), outside.end(), [&](int a, int b) { return pos(a) < pos(b); });
+        DiagPass d;
+        for (int q : R) d.required.push_back(pos(q));
+        for (int q : G) d.anchor.push_back(pos(q));
+        std::vector<std::vector<int>> chunk_q;
+        for (size_t j = 0; j < outside.size(); j += 5) {
+            std::vector<int> c(outside.begin() + j, outside.begin() + std::min(outside.size(), j + 5));
+            chunk_q.push_back(c);
+        }
+        if (chunk_q.empty()) chunk_q.push_back({});
+        for (auto& c : chunk_q) {
+            std::vector<int> p;
+            for (int q : c) p.push_back(pos(q));
+            d.chunks.push_back(p);
+        }
+        const size_t ng = (size_t)1 << G.size(), nc = chunk_q.size();
+        d.table.assign(ng * nc * 32, cd(1.0));
+        for (size_t g = 0; g < ng; ++g)
+            for (size_t v = 0; v < 32; ++v) d.table[(g * nc + 0) * 32 + v] *= global;
+        for (auto& t : terms) {
+            uint32_t gmask = 0;
+            int oq = -1;
+            for (int q : t.q) {
+                int gi = local_index(G, q);
+                if (gi >= 0)
+                    gmask |= 1u << gi;
+                else
+                    oq = q;
+            }
+            size_t c = 0;
+            uint32_t vbit = 0;
+            if (oq >= 0) {
+                for (size_t ci = 0; ci < nc; ++ci) {
+                    int b = local_index(chunk_q[ci], oq);
+                    if (b >= 0) {
+                        c = ci;
+                        vbit = 1u << b;
+                    }
+                }
+            }
+            for (uint32_t g = 0; g < ng; ++g) {
+                if ((g & gmask) != gmask) continue;
+                for (uint32_t v = 0; v < 32; ++v)
+                    if ((v & vbit) == vbit) d.table[(g * nc + c) * 32 + v] *= t.phase;
+            }
+        }
+        aqs_op op{};
+        op.kind = AQS_OP_DIAGT;
+        op.ntargets = std::max(1, (int)G.size());
+        for (size_t j = 0; j < G.size(); ++j) op.targets[j] = G[j];
+        if (G.empty()) op.targets[0] = outside.empty() ? 0 : outside[0];
+        sch_->passes.push_back(op);
+        sch_->mats.emplace_back();
+        sch_->diag_index.push_back((int)sch_->diags.size());
+        sch_->diags.push_back(std::move(d));
+    }
+};
 
 }  // namespace
 
@@ -227,73 +596,13 @@ aqs_status aqs_fuse(const aqs_op* ops, size_t count, int nqubits, int max_qubits
         if (count && !ops) fail(AQS_ERR_INVALID_ARGUMENT, "null op array");
         if (nqubits < 1 || nqubits > AQS_MAX_QUBITS) fail(AQS_ERR_INVALID_ARGUMENT, "bad qubit count");
         if (max_qubits < 1 || max_qubits > 5) fail(AQS_ERR_INVALID_ARGUMENT, "max_qubits must be 1..5");
+        for (size_t i = 0; i < count; ++i) validate_op(ops[i], nqubits);
         auto* sch = new aqs_schedule_s();
         sch->n = nqubits;
         sch->source_ops = count;
         try {
-            std::vector<Block> open;
-            auto close_block = [&](size_t idx) {
-                emit_block(sch, std::move(open[idx]));
-                open.erase(open.begin() + idx);
-            };
-            for (size_t i = 0; i < count; ++i) {
-                const aqs_op& op = ops[i];
-                validate_op(op, nqubits);
-                std::vector<int> S = op_qubits(op);
-                std::vector<size_t> hit;
-                for (size_t b = 0; b < open.size(); ++b)
-                    for (int q : S)
-                        if (local_index(open[b].qubits, q) >= 0) {
-                            hit.push_back(b);
-                            break;
-                        }
-                std::vector<int> U = S;
-                for (size_t b : hit)
-                    for (int q : open[b].qubits)
-                        if (std::find(U.begin(), U.end(), q) == U.end()) U.push_back(q);
-                if ((int)S.size() > max_qubits) {
-                    // too wide to fuse: close what it touches, emit as is
-                    for (size_t j = hit.size(); j-- > 0;) close_block(hit[j]);
-                    aqs_op cp = op;
-                    std::vector<double> u;
-                    if (op.kind == AQS_OP_MATK) {
-                        const size_t K = (size_t)1 << op.ntargets;
-                        u.assign(op.u, op.u + 2 * K * K);
-                    }
-                    sch->passes.push_back(cp);
-                    sch->mats.push_back(std::move(u));
-                    continue;
-                }
-                if ((int)U.size() <= max_qubits) {
-                    // merge op and every touched block into one block over U
-                    std::sort(U.begin(), U.end());
-                    const uint64_t K = 1ull << U.size();
-                    std::vector<cd> M(K * K);
-                    for (uint64_t d = 0; d < K; ++d) M[d * K + d] = 1.0;
-                    size_t src = 1;
-                    for (size_t b : hit) {
-                        M = matmul(embed(open[b], U), M, K);
-                        src += open[b].sources;
-                    }
-                    M = matmul(op_matrix(op, U), M, K);
-                    for (size_t j = hit.size(); j-- > 0;) open.erase(open.begin() + hit[j]);
-                    Block nb;
-                    nb.qubits = U;
-                    nb.U = std::move(M);
-                    nb.sources = src;
-                    open.push_back(std::move(nb));
-                } else {
-                    for (size_t j = hit.size(); j-- > 0;) close_block(hit[j]);
-                    std::sort(S.begin(), S.end());
-                    Block nb;
-                    nb.qubits = S;
-                    nb.U = op_matrix(op, S);
-                    nb.sources = 1;
-                    open.push_back(std::move(nb));
-                }
-            }
-            while (!open.empty()) close_block(0);
-            // MATK passes point at their owned matrices
+            Fuser f(sch, nqubits, max_qubits);
+            f.run(ops, count);
             for (size_t p = 0; p < sch->passes.size(); ++p)
                 if (sch->passes[p].kind == AQS_OP_MATK) sch->passes[p].u = sch->mats[p].data();
         } catch (...) {
@@ -334,9 +643,9 @@ aqs_status aqs_apply_schedule(aqs_state s, aqs_schedule sch) {
         if (!sch) fail(AQS_ERR_INVALID_ARGUMENT, "null schedule");
         if (sch->n != s->n) fail(AQS_ERR_INVALID_ARGUMENT, "schedule and state qubit counts differ");
         if (sch->dev_precision != s->precision) {
-            // Build the device copy of every MATK matrix in the layout its
-            // kernel reads: MFMA-packed FP32 (complex64, k = 4, 5), plain FP32
-            // (complex64, k <= 3) or plain FP64 (complex128).
+            // Device copies of the pass data, in the layout each kernel reads:
+            // MFMA-packed FP32 (complex64, k = 4, 5), plain FP32 / FP64
+            // matrices, and diagonal tables as float2 / double2.
             if (sch->dev) {
                 AQS_HIP(hipDeviceSynchronize());
                 AQS_HIP(hipFree(sch->dev));
@@ -345,13 +654,30 @@ aqs_status aqs_apply_schedule(aqs_state s, aqs_schedule sch) {
             std::vector<char> host;
             sch->dev_off.assign(sch->passes.size(), 0);
             sch->dev_mfma.assign(sch->passes.size(), 0);
+            auto append = [&](const void* p, size_t bytes) {
+                size_t at = (host.size() + 255) & ~(size_t)255;
+                host.resize(at + bytes);
+                std::memcpy(host.data() + at, p, bytes);
+                return at;
+            };
             for (size_t p = 0; p < sch->passes.size(); ++p) {
                 const aqs_op& op = sch->passes[p];
+                if (sch->diag_index[p] >= 0) {
+                    const auto& d = sch->diags[sch->diag_index[p]];
+                    if (s->precision == AQS_C64) {
+                        std::vector<float> t(2 * d.table.size());
+                        for (size_t i = 0; i < d.table.size(); ++i) {
+                            t[2 * i] = (float)d.table[i].real();
+                            t[2 * i + 1] = (float)d.table[i].imag();
+                        }
+                        sch->dev_off[p] = append(t.data(), t.size() * sizeof(float));
+                    } else {
+                        sch->dev_off[p] = append(d.table.data(), d.table.size() * sizeof(cd));
+                    }
+                    continue;
+                }
                 if (op.kind != AQS_OP_MATK) continue;
                 const size_t K = (size_t)1 << op.ntargets;
-                size_t at = (host.size() + 255) & ~(size_t)255;
-                host.resize(at);
-                sch->dev_off[p] = at;
                 if (s->precision == AQS_C64) {
                     std::vector<float> v;
                     if ((op.ntargets == 4 || op.ntargets == 5) && op.ncontrols == 0) {
@@ -361,11 +687,9 @@ aqs_status aqs_apply_schedule(aqs_state s, aqs_schedule sch) {
                         v.resize(2 * K * K);
                         for (size_t i = 0; i < v.size(); ++i) v[i] = (float)op.u[i];
                     }
-                    host.resize(at + v.size() * sizeof(float));
-                    std::memcpy(host.data() + at, v.data(), v.size() * sizeof(float));
+                    sch->dev_off[p] = append(v.data(), v.size() * sizeof(float));
                 } else {
-                    host.resize(at + 2 * K * K * sizeof(double));
-                    std::memcpy(host.data() + at, op.u, 2 * K * K * sizeof(double));
+                    sch->dev_off[p] = append(op.u, 2 * K * K * sizeof(double));
                 }
             }
             if (!host.empty()) {
@@ -376,16 +700,19 @@ aqs_status aqs_apply_schedule(aqs_state s, aqs_schedule sch) {
         }
         for (size_t p = 0; p < sch->passes.size(); ++p) {
             const aqs_op& op = sch->passes[p];
+            const void* d = (const char*)sch->dev + sch->dev_off[p];
+            if (sch->diag_index[p] >= 0) {
+                const auto& dp = sch->diags[sch->diag_index[p]];
+                apply_diag_table(s, dp.required, dp.anchor, dp.chunks, d);
+                continue;
+            }
             if (op.kind != AQS_OP_MATK) {
                 apply_op(s, op);
                 continue;
             }
-            const void* d = (const char*)sch->dev + sch->dev_off[p];
             if (sch->dev_mfma[p] && apply_matk_mfma(s, op, (const float*)d)) continue;
             if (sch->dev_mfma[p]) {
-                // MFMA shape not usable for this state size: plain matrix path
-                aqs_op cp = op;
-                apply_op(s, cp);
+                apply_op(s, op);  // MFMA shape not usable at this size: plain matrix path
                 continue;
             }
             validate_op(op, s->n);

@@ -130,6 +130,11 @@ void apply_matk_device(aqs_state s, const aqs_op& op, const void* dev_u);
 // Pack an FP64 complex matrix (2^k x 2^k) into the MFMA operand layout.
 std::vector<float> pack_mfma_operand(int k, const double* u);
 
+// Diagonal table pass (diag.hip): bit positions in amplitude-index space;
+// dev_tab = [2^|anchor|][chunks][32] complex in the state's precision.
+void apply_diag_table(aqs_state s, const std::vector<int>& required, const std::vector<int>& anchor,
+                      const std::vector<std::vector<int>>& chunks, const void* dev_tab);
+
 // Validation shared by apply and fuse.
 void validate_op(const aqs_op& op, int nqubits);
 

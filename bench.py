@@ -122,6 +122,105 @@ def batched(args):
     return out
 
 
+def random20(args, n=20, depth=40, shots=1000):
+    """20-qubit random layered circuit (H/X/RZ/RY per qubit, CX on a random
+    matching; 1200 gates), simulate() from |0..0>, then 1000 sampled shots
+    (profile_measure_all); plain per-gate and compile()-fused."""
+    gates = circuits.random_layered(n, depth, seed=1)
+    lowered = ops.lower_all(gates)
+    u = np.random.default_rng(12345).uniform(size=shots)
+    st = hip.State(n, hip.C64)
+    out = {"qubits": n, "depth": depth, "gates": len(lowered), "shots": shots}
+    reps = max(3, args.steps)
+    for label, sch in (("plain", None), ("fused_k5", hip.Schedule(lowered, n, 5))):
+        def run_once():
+            st.set_basis(0)
+            if sch is None:
+                st.apply(lowered)
+            else:
+                st.apply_schedule(sch)
+            return st.sample(u)
+        run_once()
+        st.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            idx = run_once()
+        dt = (time.perf_counter() - t0) / reps
+        entry = {"ms_per_circuit_with_sampling": round(1e3 * dt, 3),
+                 "gate_apps_per_s": round(len(lowered) / dt, 1)}
+        if sch is not None:
+            entry["passes"] = sch.info()[0]
+        out[label] = entry
+    # sampled indices against the oracle's rule on the same amplitudes
+    import oracle
+    psi = st.download()
+    out["samples_match_oracle"] = bool(np.array_equal(idx.astype(np.uint32), oracle.sample_c64(psi, n, u)))
+    ref = hip.State(n, hip.C128)
+    ref.apply(ops.lower_all(gates, exact=True))
+    ov = st.inner(ref)
+    out["infidelity_vs_fp64"] = max(0.0, 1.0 - abs(ov) ** 2 / (st.norm2() * ref.norm2()))
+    ref.close()
+    st.close()
+    log(f"random20: {out}")
+    return out
+
+
+def qft30(args, n=30, seed=30):
+    """30-qubit QFT (fourier_transform + final swaps) on a seeded Gaussian
+    complex64 state (normalised in FP64): unfused per-gate path, compile()
+    fused into <= 5-qubit MFMA blocks, and a complex128 run as reference."""
+    gates = circuits.fourier_transform(n, final_swaps=True)
+    lowered = ops.lower_all(gates)
+    alg = algorithmic_bytes(lowered, n)
+    st = hip.State(n, hip.C64)
+    ref = hip.State(n, hip.C128)
+    ref.gaussian(seed)
+    ref.apply(ops.lower_all(gates, exact=True))
+    out = {"qubits": n, "gates": len(lowered), "algorithmic_bytes_unfused": alg}
+
+    def infid():
+        ov = st.inner(ref)
+        return max(0.0, 1.0 - abs(ov) ** 2 / (st.norm2() * ref.norm2()))
+
+    st.gaussian(seed)
+    st.sync()
+    t0 = time.perf_counter()
+    st.apply(lowered)
+    st.sync()
+    dt = time.perf_counter() - t0
+    out["unfused"] = {"ms": round(1e3 * dt, 2), "gate_apps_per_s": round(len(lowered) / dt, 1),
+                      "hbm_gbs": round(alg / dt / 1e9, 1), "infidelity": infid()}
+    for kmax in (5,):
+        t0 = time.perf_counter()
+        sch = hip.Schedule(lowered, n, kmax)
+        fuse_s = time.perf_counter() - t0
+        passes, _ = sch.info()
+        st.gaussian(seed)
+        st.apply_schedule(sch)  # uploads the block matrices once
+        st.gaussian(seed)
+        st.sync()
+        st.profile(True)
+        t0 = time.perf_counter()
+        st.apply_schedule(sch)
+        st.sync()
+        dt = time.perf_counter() - t0
+        prof = st.profile_read()
+        st.profile(False)
+        mf_ms, mf_n = prof.get("MATK_MFMA", (0.0, 0))
+        out[f"fused_k{kmax}"] = {"passes": passes, "ms": round(1e3 * dt, 2),
+                                 "gate_apps_per_s": round(len(lowered) / dt, 1), "infidelity": infid(),
+                                 "fuse_host_s": round(fuse_s, 3), "mfma_launches": mf_n,
+                                 "mfma_avg_ms": round(mf_ms / max(1, mf_n), 4),
+                                 "mfma_tflops": round(8 * 32 * (1 << n) / (mf_ms / max(1, mf_n) * 1e-3) / 1e12, 1)
+                                 if mf_n else None,
+                                 "speedup_vs_unfused": round(out["unfused"]["ms"] / (1e3 * dt), 2)}
+        sch.close()
+    st.close()
+    ref.close()
+    log(f"qft30: {out}")
+    return out
+
+
 def served_requests(count, seed):
     """Brickwork requests: n log-uniform over 12..28 (P(n) ~ 1/n on the
     integers), depth ~ U{10..100}, 1000 shots with seeded uniforms."""
@@ -332,6 +431,8 @@ def run(args):
     st.close()
 
     batch = batched(args) if not args.no_batch else None
+    r20 = random20(args)
+    qft = qft30(args) if not args.no_qft else None
     serve = served(args) if args.serve_requests > 0 else None
 
     cpu = cpu_baseline(gates, n) if not args.no_cpu else None
@@ -359,6 +460,8 @@ def run(args):
         "roofline": roofline,
         "fused": fused,
         "batched": batch,
+        "random20": r20,
+        "qft30": qft,
         "served": serve,
         "cpu_baseline": cpu,
         "device": dev,
@@ -373,6 +476,7 @@ def main():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     p.add_argument("--no-batch", action="store_true", help="skip the batched-states section")
+    p.add_argument("--no-qft", action="store_true", help="skip the 30-qubit QFT section")
     p.add_argument("--serve-requests", type=int, default=120,
                    help="requests in the served-stream section (0 skips it)")
     p.add_argument("--probe", action="store_true",

@@ -65,7 +65,11 @@ typedef enum aqs_op_kind {
     AQS_OP_MATK = 5,
     /* multiply by the phase m[0..1] where every target and control bit is 1:
      * Z, Phase, CZ, CPhase                                                  */
-    AQS_OP_PHASE = 6
+    AQS_OP_PHASE = 6,
+    /* only in fused schedules (aqs_schedule_get): a diagonal table pass that
+     * applies many deferred phase terms in one sweep; targets lists its
+     * anchor qubits.  Not accepted by aqs_apply.                           */
+    AQS_OP_DIAGT = 7
 } aqs_op_kind;
 
 /* Control semantics: default = all control bits must be 1.  AQS_CTRL_ANY =

@@ -189,6 +189,31 @@ def test_fused_brickwork(n, kmax):
     np.testing.assert_allclose(got128, want, atol=1e-12)
 
 
+@pytest.mark.parametrize("kmax", [3, 4, 5])
+@pytest.mark.parametrize("precision", [hip.C64, hip.C128], ids=["c64", "c128"])
+def test_fused_diagonal_deferral(kmax, precision):
+    """Circuits dominated by phase gates (QFT ladders, RZ, Phase, CZ, CPhase,
+    CRotZ with its DIAG1 global phase) exercise deferred terms and the
+    diagonal table passes of compile()."""
+    n = 13
+    rng = np.random.default_rng(kmax)
+    gates = circuits.fourier_transform(n, final_swaps=False)
+    for _ in range(120):
+        kind = str(rng.choice(["RotZ", "Phase", "CZ", "CPhase", "CRotZ", "Z", "H", "RotY", "Swap"],
+                              p=[0.15, 0.15, 0.15, 0.2, 0.15, 0.05, 0.05, 0.05, 0.05]))
+        q = [int(x) for x in rng.choice(n, size=3, replace=False)]
+        gates.append(circuits.gate(kind, q[0], q[1], q[2], angle=float(rng.uniform(-3, 3))))
+    gates += circuits.inverse_fourier_transform(n)
+    sv = random_state(n, 19)
+    want = oracle.run(sv, n, gates)
+    sch = hip.Schedule(ops.lower_all(gates, exact=(precision == hip.C128)), n, kmax)
+    kinds = {k for k, _ in sch.passes()}
+    assert hip.OP_DIAGT in kinds
+    got = gpu_run(n, sv, gates, precision, fused=kmax)
+    tol = 5e-6 if precision == hip.C64 else 1e-12
+    np.testing.assert_allclose(got, want, atol=tol)
+
+
 def _haar(k, rng):
     z = (rng.normal(size=(1 << k, 1 << k)) + 1j * rng.normal(size=(1 << k, 1 << k))) / math.sqrt(2)
     q, r = np.linalg.qr(z)
@@ -339,6 +364,25 @@ def test_errors_match_reference_types():
         st.apply([{"kind": hip.OP_SWAP, "targets": (2, 2)}])
     with pytest.raises(hip.OutOfRange):
         st.prob1(9)
+
+
+@pytest.mark.parametrize("fused", [None, 4, 5])
+def test_qft_closed_form_gpu(fused):
+    """fourier_transform(n) + swaps on a Gaussian state = bitrev(ifft(bitrev(v))) sqrt(N)."""
+    n = 16
+    N = 1 << n
+    st = hip.State(n, hip.C64)
+    st.gaussian(77)
+    v = oracle.gaussian_state(n, 77)
+    gates = circuits.fourier_transform(n, final_swaps=True)
+    lowered = ops.lower_all(gates)
+    if fused:
+        st.apply_schedule(hip.Schedule(lowered, n, fused))
+    else:
+        st.apply(lowered)
+    rev = np.array([int(format(i, f"0{n}b")[::-1], 2) for i in range(N)])
+    ref = np.fft.ifft(v[rev])[rev] * math.sqrt(N)
+    assert oracle.infidelity(st.download(), ref) < 1e-9
 
 
 def test_autotune_keeps_state_and_results():

@@ -34,8 +34,14 @@ struct DiagTArgs {
     int nc;
     int cpos[6][5];
     int clen[6];
+    // bit groups whose positions are consecutive ascending are read with
+    // one shift and mask: start bit, or -1 when the group is scattered
+    int gstart;
+    int cstart[6];
     const void* tab;      // [2^ng][nc][32] complex (float2 or double2)
 };
+
+__device__ __forceinline__ uint32_t group_bits(uint64_t i, int start, const int* pos, int len);
 
 __device__ __forceinline__ uint32_t pext_list(uint64_t i, const int* pos, int len) {
     uint32_t v = 0;
@@ -45,6 +51,11 @@ __device__ __forceinline__ uint32_t pext_list(uint64_t i, const int* pos, int le
         v |= (uint32_t)((i >> pos[b]) & 1ull) << b;
     }
     return v;
+}
+
+__device__ __forceinline__ uint32_t group_bits(uint64_t i, int start, const int* pos, int len) {
+    if (start >= 0) return (uint32_t)(i >> start) & ((1u << len) - 1u);
+    return pext_list(i, pos, len);
 }
 
 template <typename T2, typename R>
@@ -66,10 +77,10 @@ __global__ __launch_bounds__(kBlock) void k_diagt(DiagTArgs a) {
             i = ((i >> p) << (p + 1)) | (i & ((1ull << p) - 1ull));
         }
         i |= a.reqmask;
-        const uint32_t g = pext_list(i, a.gpos, a.ng);
+        const uint32_t g = group_bits(i, a.gstart, a.gpos, a.ng);
         R pr = 1, pi = 0;
         for (int c = 0; c < a.nc; ++c) {
-            const T2 f = tab[(g * a.nc + c) * 32 + pext_list(i, a.cpos[c], a.clen[c])];
+            const T2 f = tab[(g * a.nc + c) * 32 + group_bits(i, a.cstart[c], a.cpos[c], a.clen[c])];
             const R nr = pr * f.x - pi * f.y;
             pi = pr * f.y + pi * f.x;
             pr = nr;
@@ -102,13 +113,20 @@ void apply_diag_table(aqs_state s, const std::vector<int>& required, const std::
         a.reqmask |= 1ull << req[j];
     }
     a.units = s->N >> req.size();
+    auto consecutive = [](const std::vector<int>& p) {
+        for (size_t j = 1; j < p.size(); ++j)
+            if (p[j] != p[0] + (int)j) return -1;
+        return p.empty() ? 0 : p[0];
+    };
     a.ng = (int)anchor.size();
     for (size_t j = 0; j < anchor.size(); ++j) a.gpos[j] = anchor[j];
+    a.gstart = consecutive(anchor);
     a.nc = (int)chunks.size();
     for (size_t c = 0; c < chunks.size(); ++c) {
         if (chunks[c].size() > 5) fail(AQS_ERR_INVALID_ARGUMENT, "diagonal chunk too wide");
         a.clen[c] = (int)chunks[c].size();
         for (size_t j = 0; j < chunks[c].size(); ++j) a.cpos[c][j] = chunks[c][j];
+        a.cstart[c] = consecutive(chunks[c]);
     }
     a.tab = dev_tab;
     const size_t entries = ((size_t)1 << a.ng) * std::max(1, a.nc) * 32;

@@ -518,6 +518,8 @@ class Fuser {
                     break;
                 }
         G = used;
+        // ascending bit positions let the kernel read the groups with one shift
+        std::sort(G.begin(), G.end(), [&](int a, int b) { return pos(a) < pos(b); });
         // required qubits: in every term
         std::vector<int> R;
         if (global == cd(1.0) && !terms.empty())

@@ -33,8 +33,8 @@ HIP_FLAGS = [
     "-I", INCLUDE, "-I", CSRC,
 ]
 
-HIP_SOURCES = ["state.hip", "gates.hip", "mfma.hip", "diag.hip", "measure.hip", "batch.hip", "fusion.cpp",
-               "serve.cpp"]
+HIP_SOURCES = ["state.hip", "gates.hip", "mfma.hip", "diag.hip", "perm.hip", "measure.hip", "batch.hip",
+               "fusion.cpp", "serve.cpp"]
 CPP_SOURCES = ["quantum.cpp", "quantum_algo.cpp"]
 
 

@@ -56,6 +56,8 @@ struct aqs_schedule_s {
     std::vector<std::vector<double>> mats;  // owned FP64 matrices (MATK passes)
     std::vector<int> diag_index;            // -1 or index into diags
     std::vector<DiagPass> diags;
+    std::vector<std::pair<int, int>> perm_index;  // (pass, index into perms)
+    std::vector<std::vector<int>> perms;          // bit involutions (AQS_OP_PERMB passes)
     // device copies, built at first use for a given precision
     void* dev = nullptr;
     int dev_precision = -1;
@@ -227,7 +229,24 @@ class Fuser {
    public:
     Fuser(aqs_schedule_s* sch, int n, int kmax) : sch_(sch), n_(n), kmax_(kmax) {}
 
-    void run(const aqs_op* ops, size_t count) {
+    void run(const aqs_op* ops_in, size_t count) {
+        // Uncontrolled swaps become relabellings: L[q] is the physical qubit
+        // holding logical qubit q; every other op is rewritten through L.
+        std::vector<int> L(n_);
+        for (int q = 0; q < n_; ++q) L[q] = q;
+        std::vector<aqs_op> ops;
+        ops.reserve(count);
+        for (size_t i = 0; i < count; ++i) {
+            aqs_op op = ops_in[i];
+            if (op.kind == AQS_OP_SWAP && op.ncontrols == 0) {
+                std::swap(L[op.targets[0]], L[op.targets[1]]);
+                continue;
+            }
+            for (int j = 0; j < op.ntargets; ++j) op.targets[j] = L[op.targets[j]];
+            for (int j = 0; j < op.ncontrols; ++j) op.controls[j] = L[op.controls[j]];
+            ops.push_back(op);
+        }
+        count = ops.size();
         // non-diagonal uses per qubit, for the look-ahead of diagonal runs
         std::vector<std::vector<size_t>> uses(n_);
         for (size_t i = 0; i < count; ++i)
@@ -277,6 +296,7 @@ class Fuser {
         }
         flush_terms();
         while (!open_.empty()) close_block(0);
+        restore_order(L);
     }
 
    private:
@@ -508,6 +528,50 @@ class Fuser {
 
     int pos(int q) const { return n_ - 1 - q; }
 
+    // Logical qubit q sits at physical qubit L[q]: move index bit pos(L[q]) to
+    // pos(q).  beta = tau2 o tau1 with both involutions (per cycle
+    // x_0 -> x_1 -> ..: tau1(x_i) = x_{-i}, tau2(x_i) = x_{1-i}).
+    void restore_order(const std::vector<int>& L) {
+        std::vector<int> beta(n_);
+        bool identity = true;
+        for (int q = 0; q < n_; ++q) {
+            beta[pos(L[q])] = pos(q);
+            if (L[q] != q) identity = false;
+        }
+        if (identity) return;
+        std::vector<int> t1(n_), t2(n_);
+        std::vector<char> seen(n_, 0);
+        for (int s = 0; s < n_; ++s) {
+            if (seen[s]) continue;
+            std::vector<int> cyc;
+            for (int x = s; !seen[x]; x = beta[x]) {
+                seen[x] = 1;
+                cyc.push_back(x);
+            }
+            const int k = (int)cyc.size();
+            for (int j = 0; j < k; ++j) {
+                t1[cyc[j]] = cyc[((-j) % k + k) % k];
+                t2[cyc[j]] = cyc[((1 - j) % k + k) % k];
+            }
+        }
+        for (const auto* t : {&t1, &t2}) {
+            bool id = true;
+            for (int b = 0; b < n_; ++b)
+                if ((*t)[b] != b) id = false;
+            if (id) continue;
+            aqs_op op{};
+            op.kind = AQS_OP_PERMB;
+            op.ntargets = 0;
+            for (int b = 0; b < n_ && op.ntargets < 5; ++b)
+                if ((*t)[b] != b) op.targets[op.ntargets++] = n_ - 1 - b;
+            sch_->passes.push_back(op);
+            sch_->mats.emplace_back();
+            sch_->diag_index.push_back(-1);
+            sch_->perm_index.push_back({(int)sch_->passes.size() - 1, (int)sch_->perms.size()});
+            sch_->perms.push_back(*t);
+        }
+    }
+
     void emit_table(std::vector<int> G, std::vector<int> outside, const std::vector<Term>& terms, cd global) {
         // drop anchor qubits no covered term uses
         std::vector<int> used;
@@ -706,6 +770,11 @@ aqs_status aqs_apply_schedule(aqs_state s, aqs_schedule sch) {
             if (sch->diag_index[p] >= 0) {
                 const auto& dp = sch->diags[sch->diag_index[p]];
                 apply_diag_table(s, dp.required, dp.anchor, dp.chunks, d);
+                continue;
+            }
+            if (op.kind == AQS_OP_PERMB) {
+                for (auto& pi : sch->perm_index)
+                    if (pi.first == (int)p) apply_bit_involution(s, sch->perms[pi.second]);
                 continue;
             }
             if (op.kind != AQS_OP_MATK) {

@@ -135,6 +135,9 @@ std::vector<float> pack_mfma_operand(int k, const double* u);
 void apply_diag_table(aqs_state s, const std::vector<int>& required, const std::vector<int>& anchor,
                       const std::vector<std::vector<int>>& chunks, const void* dev_tab);
 
+// In-place index-bit permutation (perm.hip); beta: involution of bit positions.
+void apply_bit_involution(aqs_state s, const std::vector<int>& beta);
+
 // Validation shared by apply and fuse.
 void validate_op(const aqs_op& op, int nqubits);
 

@@ -31,6 +31,7 @@ OP_SWAP = 4
 OP_MATK = 5
 OP_PHASE = 6
 OP_DIAGT = 7
+OP_PERMB = 8
 CTRL_ANY = 1
 
 PROFILE_SLOTS = ["other", "MAT1", "DIAG1", "X", "SWAP", "MATK", "PHASE", "MATK_MFMA"]

@@ -69,7 +69,11 @@ typedef enum aqs_op_kind {
     /* only in fused schedules (aqs_schedule_get): a diagonal table pass that
      * applies many deferred phase terms in one sweep; targets lists its
      * anchor qubits.  Not accepted by aqs_apply.                           */
-    AQS_OP_DIAGT = 7
+    AQS_OP_DIAGT = 7,
+    /* only in fused schedules: an in-place permutation of index bits that
+     * restores the qubit order after swaps were turned into relabellings;
+     * targets lists (up to 5 of) the qubits it moves.                      */
+    AQS_OP_PERMB = 8
 } aqs_op_kind;
 
 /* Control semantics: default = all control bits must be 1.  AQS_CTRL_ANY =

@@ -214,6 +214,44 @@ def test_fused_diagonal_deferral(kmax, precision):
     np.testing.assert_allclose(got, want, atol=tol)
 
 
+@pytest.mark.parametrize("n", [3, 7, 12, 17])
+def test_fused_swaps_are_relabelled_bit_exact(n):
+    """Swap-only circuits compile to at most two in-place bit-permutation
+    passes; the result is the exact permutation of the input."""
+    rng = np.random.default_rng(n)
+    gates = []
+    for _ in range(40):
+        a, b = (int(x) for x in rng.choice(n, size=2, replace=False))
+        gates.append(oracle.gate("Swap", a, b))
+    sch = hip.Schedule(ops.lower_all(gates), n, 5)
+    kinds = [k for k, _ in sch.passes()]
+    assert set(kinds) <= {hip.OP_PERMB} and len(kinds) <= 2
+    sv = random_state(n, 3).astype(np.complex64)
+    want = oracle.run(sv.astype(np.complex128), n, gates).astype(np.complex64)
+    for prec in (hip.C64, hip.C128):
+        st = hip.State(n, prec)
+        st.upload(sv.astype(st.dtype))
+        st.apply_schedule(sch)
+        got = st.download().astype(np.complex64)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+@pytest.mark.parametrize("kmax", [2, 5])
+def test_fused_swaps_mixed_with_gates(kmax):
+    n = 12
+    rng = np.random.default_rng(40 + kmax)
+    gates = []
+    for _ in range(150):
+        kind = str(rng.choice(["Swap", "U3", "CX", "CZ", "CSwap", "CPhase", "H"]))
+        q = [int(x) for x in rng.choice(n, size=3, replace=False)]
+        gates.append(circuits.gate(kind, q[0], q[1], q[2], angle=float(rng.uniform(-3, 3)),
+                                   phi=float(rng.uniform(-3, 3)), lam=float(rng.uniform(-3, 3))))
+    sv = random_state(n, 44)
+    want = oracle.run(sv, n, gates)
+    np.testing.assert_allclose(gpu_run(n, sv, gates, fused=kmax), want, atol=5e-6)
+    np.testing.assert_allclose(gpu_run(n, sv, gates, hip.C128, fused=kmax), want, atol=1e-12)
+
+
 def _haar(k, rng):
     z = (rng.normal(size=(1 << k, 1 << k)) + 1j * rng.normal(size=(1 << k, 1 << k))) / math.sqrt(2)
     q, r = np.linalg.qr(z)

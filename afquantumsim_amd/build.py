@@ -35,7 +35,7 @@ HIP_FLAGS = [
 
 HIP_SOURCES = ["state.hip", "gates.hip", "mfma.hip", "diag.hip", "perm.hip", "measure.hip", "batch.hip",
                "fusion.cpp", "serve.cpp"]
-CPP_SOURCES = ["quantum.cpp", "quantum_algo.cpp"]
+CPP_SOURCES = ["quantum.cpp", "quantum_algo.cpp", "quantum_gates.cpp"]
 
 
 def _run(cmd):

@@ -93,6 +93,65 @@ __global__ __launch_bounds__(kBlock) void k_diagt(DiagTArgs a) {
     }
 }
 
+// Complex64 variant: each lane takes U float4s (two amplitudes each, indices
+// 2f and 2f+1) with all loads issued before the table work.  Used when no
+// required bit is bit 0, so both halves of a float4 are enumerated together.
+constexpr int kDiagU = 4;
+
+__global__ __launch_bounds__(kBlock) void k_diagt_v4(DiagTArgs a) {
+    extern __shared__ unsigned char smem[];
+    float2* tab = reinterpret_cast<float2*>(smem);
+    const int entries = (1 << a.ng) * a.nc * 32;
+    const float2* gt = reinterpret_cast<const float2*>(a.tab);
+    for (int e = threadIdx.x; e < entries; e += kBlock) tab[e] = gt[e];
+    __syncthreads();
+    float4* sv = reinterpret_cast<float4*>(a.sv);
+    const uint64_t units = a.units >> 1;  // float4s enumerated
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock * kDiagU;
+    for (uint64_t base = (uint64_t)blockIdx.x * kBlock * kDiagU + threadIdx.x; base < units; base += stride) {
+        float4 v[kDiagU];
+        uint64_t f[kDiagU];
+#pragma unroll
+        for (int j = 0; j < kDiagU; ++j) {
+            uint64_t u = base + (uint64_t)j * kBlock;
+            // required bits (>= 1) in float4-index space, then back to amplitudes
+            for (int r = 0; r < a.nreq; ++r) {
+                const int p = a.req[r] - 1;
+                u = ((u >> p) << (p + 1)) | (u & ((1ull << p) - 1ull));
+            }
+            f[j] = u | (a.reqmask >> 1);
+            if (base + (uint64_t)j * kBlock < units) v[j] = ld_nt(&sv[f[j]]);
+        }
+#pragma unroll
+        for (int j = 0; j < kDiagU; ++j) {
+            if (base + (uint64_t)j * kBlock >= units) continue;
+            float4 x = v[j], y;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const uint64_t i = 2 * f[j] + h;
+                const uint32_t g = group_bits(i, a.gstart, a.gpos, a.ng);
+                float pr = 1.f, pi = 0.f;
+                for (int c = 0; c < a.nc; ++c) {
+                    const float2 t = tab[(g * a.nc + c) * 32 + group_bits(i, a.cstart[c], a.cpos[c], a.clen[c])];
+                    const float nr = pr * t.x - pi * t.y;
+                    pi = pr * t.y + pi * t.x;
+                    pr = nr;
+                }
+                const float xr = h ? x.z : x.x, xi = h ? x.w : x.y;
+                const float yr = pr * xr - pi * xi, yi = pr * xi + pi * xr;
+                if (h) {
+                    y.z = yr;
+                    y.w = yi;
+                } else {
+                    y.x = yr;
+                    y.y = yi;
+                }
+            }
+            st_nt(y, &sv[f[j]]);
+        }
+    }
+}
+
 }  // namespace
 
 namespace aqs_hip {
@@ -133,7 +192,13 @@ void apply_diag_table(aqs_state s, const std::vector<int>& required, const std::
     uint64_t blocks = (a.units + kBlock - 1) / kBlock;
     blocks = std::min<uint64_t>(std::max<uint64_t>(blocks, 1), 8192);
     ProfScope ps(s, AQS_OP_PHASE);
-    if (s->precision == AQS_C64) {
+    const bool bit0_required = !req.empty() && req[0] == 0;
+    if (s->precision == AQS_C64 && !bit0_required && s->n >= 2) {
+        const size_t lds = entries * sizeof(float2);
+        uint64_t b4 = (a.units / 2 + kBlock * kDiagU - 1) / (kBlock * kDiagU);
+        b4 = std::min<uint64_t>(std::max<uint64_t>(b4, 1), 16384);
+        hipLaunchKernelGGL(k_diagt_v4, dim3((unsigned)b4), dim3(kBlock), lds, s->stream, a);
+    } else if (s->precision == AQS_C64) {
         const size_t lds = entries * sizeof(float2);
         hipLaunchKernelGGL((k_diagt<float2, float>), dim3((unsigned)blocks), dim3(kBlock), lds, s->stream, a);
     } else {

@@ -16,6 +16,7 @@
 
 #include "aqs/quantum.h"
 #include "aqs/quantum_algo.h"
+#include "aqs/quantum_gates.h"
 
 using namespace aqs;
 
@@ -595,6 +596,89 @@ static void test_qsim_probability() {
     CHECK(fequal(qs.qubit_probability_true(qcount - 1), 0.5f, diff));
 }
 
+// :1011-1135 special gates (quantum_gates.h builders)
+static void test_special_gates() {
+    {
+        QCircuit qc(6);
+        QSimulator qs(6);
+        uint32_t target = 3;
+        qc << Gate(NControl_Gate(6, {0, 2, 4, 5}, target, X::gate()), 0);
+        for (uint32_t i = 0; i < qs.qubit_count(); ++i) qs.qubit(i) = QState::one();
+        qc.compile();
+        qs.generate_statevector();
+        qs.simulate(qc);
+        CHECK(qs.peek_measure_all() == 0b111011);
+        qs.qubit(target) = QState::zero();
+        qs.generate_statevector();
+        qs.simulate(qc);
+        CHECK(qs.peek_measure_all() == 0b111111);
+        qs.qubit(1) = QState::zero();
+        qs.qubit(target) = QState::zero();
+        qs.generate_statevector();
+        qs.simulate(qc);
+        CHECK(qs.peek_measure_all() == 0b101111);
+        qs.qubit(1) = QState::zero();
+        qs.qubit(target) = QState::one();
+        qs.generate_statevector();
+        qs.simulate(qc);
+        CHECK(qs.peek_measure_all() == 0b101011);
+    }
+    {
+        QCircuit qc(4);
+        qc << Gate(NControl_Gate(4, {3}, 0, X::gate()), 0);
+        QCircuit ref(4);
+        ref << CX(3, 0);
+        qc.compile();
+        ref.compile();
+        CHECK(near(qc.circuit(), ref.circuit(), 1e-6f));
+        qc.clear();
+        ref.clear();
+        qc << Gate(NControl_Gate(4, {0, 3}, 2, X::gate()), 0);
+        ref << CCNot(0, 3, 2);
+        qc.compile();
+        ref.compile();
+        CHECK(near(qc.circuit(), ref.circuit(), 1e-6f));
+    }
+    {
+        QCircuit qc(4);
+        qc << Gate(Control_Group_Gate(4, 1, {0, 2, 3}, X::gate()), 0);
+        QCircuit ref(4);
+        ref << CX{1, 0} << CX{1, 2} << CX{1, 3};
+        qc.compile();
+        ref.compile();
+        CHECK(near(qc.circuit(), ref.circuit(), 1e-6f));
+    }
+    {
+        QCircuit qc = Group_Gate(4, {0, 2, 3}, X::gate(), true);
+        QCircuit ref{4};
+        ref << X{0} << X{2} << X{3};
+        ref.compile();
+        CHECK(near(qc.circuit(), ref.circuit(), 1e-6f));
+    }
+    {
+        QCircuit circuit{4};
+        circuit << H{0} << CX{0, 2} << H{1} << Z{3};
+        circuit.compile();
+        QCircuit qc = Rewire_Gate(4, {2, 1, 3, 0}, circuit, true);
+        QCircuit ref{4};
+        ref << H{2} << CX{2, 3} << H{1} << Z{0};
+        ref.compile();
+        CHECK(near(qc.circuit(), ref.circuit(), 1e-6f));
+    }
+    {
+        QCircuit ref{4};
+        ref << CRotY{3, 0, pi / 2.f} << X{0} << CRotX{1, 3, -pi / 4.f} << CPhase{2, 1, pi / 3.f} << Swap{1, 2};
+        ref.compile();
+        QCircuit circuit{4};
+        circuit << Swap{1, 2} << CPhase{2, 1, -pi / 3.f} << CRotX{1, 3, pi / 4.f} << X{0} << CRotY{3, 0, -pi / 2.f};
+        circuit.compile();
+        QCircuit qc = Adjoint_Gate(circuit);
+        CHECK(near(qc.circuit(), ref.circuit(), 1e-5f));
+        // the input circuit is unchanged
+        CHECK(static_cast<const CPhase&>(*circuit.gate_list()[1]).angle == -pi / 3.f);
+    }
+}
+
 // Errors carry the reference's exception types (e.g. src/quantum.cpp:535-536, 967-975).
 static void test_errors() {
     QCircuit qc(3);
@@ -667,6 +751,7 @@ int main(int argc, char** argv) {
     test_qsim_profile_measure();
     test_qsim_gates();
     test_qsim_probability();
+    test_special_gates();
     test_errors();
     test_compile_equivalence();
     std::cout << "ALL PASSED" << std::endl;

@@ -33,8 +33,8 @@ HIP_FLAGS = [
     "-I", INCLUDE, "-I", CSRC,
 ]
 
-HIP_SOURCES = ["state.hip", "gates.hip", "mfma.hip", "diag.hip", "perm.hip", "measure.hip", "batch.hip",
-               "fusion.cpp", "serve.cpp"]
+HIP_SOURCES = ["state.hip", "gates.hip", "mfma.hip", "diag.hip", "perm.hip", "tile.hip", "measure.hip", "batch.hip",
+               "fusion.cpp", "tiling.cpp", "serve.cpp"]
 CPP_SOURCES = ["quantum.cpp", "quantum_algo.cpp", "quantum_gates.cpp"]
 
 
@@ -92,12 +92,28 @@ def build_cpp_tests() -> str:
     return CPP_TEST_BIN
 
 
+TILE_EMU_SRC = os.path.join(ROOT, "tests", "cpp", "tile_emu.cpp")
+TILE_EMU_LIB = os.path.join(ROOT, "tests", "cpp", "libtile_emu.so")
+
+
+def build_test_helpers() -> str:
+    """Host emulation of compiled schedules for the CPU tests of the tile scheduler."""
+    if not os.path.exists(TILE_EMU_SRC):
+        return ""
+    tmp = TILE_EMU_LIB + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-x", "hip", "-shared", "-o", tmp, TILE_EMU_SRC, "-L", PKG, "-laqs_hip",
+          "-Wl,-rpath,$ORIGIN/../../afquantumsim_amd"])
+    os.replace(tmp, TILE_EMU_LIB)
+    return TILE_EMU_LIB
+
+
 def build_all() -> None:
     if not os.path.exists(HIPCC) and not shutil.which("hipcc"):
         raise RuntimeError("hipcc not found")
     build_hip()
     build_cpp()
     build_cpp_tests()
+    build_test_helpers()
 
 
 if __name__ == "__main__":

@@ -191,7 +191,7 @@ void apply_diag_table(aqs_state s, const std::vector<int>& required, const std::
     const size_t entries = ((size_t)1 << a.ng) * std::max(1, a.nc) * 32;
     uint64_t blocks = (a.units + kBlock - 1) / kBlock;
     blocks = std::min<uint64_t>(std::max<uint64_t>(blocks, 1), 8192);
-    ProfScope ps(s, AQS_OP_PHASE);
+    ProfScope ps(s, kSlotDiagTable);
     const bool bit0_required = !req.empty() && req[0] == 0;
     if (s->precision == AQS_C64 && !bit0_required && s->n >= 2) {
         const size_t lds = entries * sizeof(float2);

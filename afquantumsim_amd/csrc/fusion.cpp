@@ -34,36 +34,9 @@
 #include <vector>
 
 #include "internal.h"
+#include "schedule.h"
 
 using namespace aqs_hip;
-using cd = std::complex<double>;
-
-namespace {
-
-struct DiagPass {
-    std::vector<int> required;            // amplitude bit positions
-    std::vector<int> anchor;              // amplitude bit positions (<= 5)
-    std::vector<std::vector<int>> chunks; // amplitude bit positions (<= 6 x 5)
-    std::vector<cd> table;                // [2^|anchor|][chunks][32]
-};
-
-}  // namespace
-
-struct aqs_schedule_s {
-    int n = 0;
-    size_t source_ops = 0;
-    std::vector<aqs_op> passes;
-    std::vector<std::vector<double>> mats;  // owned FP64 matrices (MATK passes)
-    std::vector<int> diag_index;            // -1 or index into diags
-    std::vector<DiagPass> diags;
-    std::vector<std::pair<int, int>> perm_index;  // (pass, index into perms)
-    std::vector<std::vector<int>> perms;          // bit involutions (AQS_OP_PERMB passes)
-    // device copies, built at first use for a given precision
-    void* dev = nullptr;
-    int dev_precision = -1;
-    std::vector<size_t> dev_off;
-    std::vector<char> dev_mfma;
-};
 
 namespace {
 
@@ -367,9 +340,7 @@ class Fuser {
     }
 
     void push(const aqs_op& op, std::vector<double> mat) {
-        sch_->passes.push_back(op);
-        sch_->mats.push_back(std::move(mat));
-        sch_->diag_index.push_back(-1);
+        sch_->push_plain(op, std::move(mat));
     }
 
     void close_touching(const std::vector<int>& S) {
@@ -528,49 +499,7 @@ class Fuser {
 
     int pos(int q) const { return n_ - 1 - q; }
 
-    // Logical qubit q sits at physical qubit L[q]: move index bit pos(L[q]) to
-    // pos(q).  beta = tau2 o tau1 with both involutions (per cycle
-    // x_0 -> x_1 -> ..: tau1(x_i) = x_{-i}, tau2(x_i) = x_{1-i}).
-    void restore_order(const std::vector<int>& L) {
-        std::vector<int> beta(n_);
-        bool identity = true;
-        for (int q = 0; q < n_; ++q) {
-            beta[pos(L[q])] = pos(q);
-            if (L[q] != q) identity = false;
-        }
-        if (identity) return;
-        std::vector<int> t1(n_), t2(n_);
-        std::vector<char> seen(n_, 0);
-        for (int s = 0; s < n_; ++s) {
-            if (seen[s]) continue;
-            std::vector<int> cyc;
-            for (int x = s; !seen[x]; x = beta[x]) {
-                seen[x] = 1;
-                cyc.push_back(x);
-            }
-            const int k = (int)cyc.size();
-            for (int j = 0; j < k; ++j) {
-                t1[cyc[j]] = cyc[((-j) % k + k) % k];
-                t2[cyc[j]] = cyc[((1 - j) % k + k) % k];
-            }
-        }
-        for (const auto* t : {&t1, &t2}) {
-            bool id = true;
-            for (int b = 0; b < n_; ++b)
-                if ((*t)[b] != b) id = false;
-            if (id) continue;
-            aqs_op op{};
-            op.kind = AQS_OP_PERMB;
-            op.ntargets = 0;
-            for (int b = 0; b < n_ && op.ntargets < 5; ++b)
-                if ((*t)[b] != b) op.targets[op.ntargets++] = n_ - 1 - b;
-            sch_->passes.push_back(op);
-            sch_->mats.emplace_back();
-            sch_->diag_index.push_back(-1);
-            sch_->perm_index.push_back({(int)sch_->passes.size() - 1, (int)sch_->perms.size()});
-            sch_->perms.push_back(*t);
-        }
-    }
+    void restore_order(const std::vector<int>& L) { emit_restore_order(sch_, L); }
 
     void emit_table(std::vector<int> G, std::vector<int> outside, const std::vector<Term>& terms, cd global) {
         // drop anchor qubits no covered term uses
@@ -644,14 +573,61 @@ class Fuser {
         op.ntargets = std::max(1, (int)G.size());
         for (size_t j = 0; j < G.size(); ++j) op.targets[j] = G[j];
         if (G.empty()) op.targets[0] = outside.empty() ? 0 : outside[0];
-        sch_->passes.push_back(op);
-        sch_->mats.emplace_back();
-        sch_->diag_index.push_back((int)sch_->diags.size());
+        sch_->push_plain(op);
+        sch_->diag_index.back() = (int)sch_->diags.size();
         sch_->diags.push_back(std::move(d));
     }
 };
 
 }  // namespace
+
+namespace aqs_hip {
+
+// Logical qubit q sits at physical qubit L[q]: move index bit pos(L[q]) to
+// pos(q).  beta = tau2 o tau1 with both involutions (per cycle
+// x_0 -> x_1 -> ..: tau1(x_i) = x_{-i}, tau2(x_i) = x_{1-i}).
+void emit_restore_order(aqs_schedule_s* sch, const std::vector<int>& L) {
+    const int n = sch->n;
+    auto pos = [n](int q) { return n - 1 - q; };
+    std::vector<int> beta(n);
+    bool identity = true;
+    for (int q = 0; q < n; ++q) {
+        beta[pos(L[q])] = pos(q);
+        if (L[q] != q) identity = false;
+    }
+    if (identity) return;
+    std::vector<int> t1(n), t2(n);
+    std::vector<char> seen(n, 0);
+    for (int s = 0; s < n; ++s) {
+        if (seen[s]) continue;
+        std::vector<int> cyc;
+        for (int x = s; !seen[x]; x = beta[x]) {
+            seen[x] = 1;
+            cyc.push_back(x);
+        }
+        const int k = (int)cyc.size();
+        for (int j = 0; j < k; ++j) {
+            t1[cyc[j]] = cyc[((-j) % k + k) % k];
+            t2[cyc[j]] = cyc[((1 - j) % k + k) % k];
+        }
+    }
+    for (const auto* t : {&t1, &t2}) {
+        bool id = true;
+        for (int b = 0; b < n; ++b)
+            if ((*t)[b] != b) id = false;
+        if (id) continue;
+        aqs_op op{};
+        op.kind = AQS_OP_PERMB;
+        op.ntargets = 0;
+        for (int b = 0; b < n && op.ntargets < 5; ++b)
+            if ((*t)[b] != b) op.targets[op.ntargets++] = n - 1 - b;
+        sch->push_plain(op);
+        sch->perm_index.push_back({(int)sch->passes.size() - 1, (int)sch->perms.size()});
+        sch->perms.push_back(*t);
+    }
+}
+
+}  // namespace aqs_hip
 
 extern "C" {
 
@@ -719,15 +695,28 @@ aqs_status aqs_apply_schedule(aqs_state s, aqs_schedule sch) {
             }
             std::vector<char> host;
             sch->dev_off.assign(sch->passes.size(), 0);
+            sch->dev_tile.assign(sch->passes.size(), {});
             sch->dev_mfma.assign(sch->passes.size(), 0);
             auto append = [&](const void* p, size_t bytes) {
                 size_t at = (host.size() + 255) & ~(size_t)255;
                 host.resize(at + bytes);
-                std::memcpy(host.data() + at, p, bytes);
+                if (bytes) std::memcpy(host.data() + at, p, bytes);
                 return at;
             };
             for (size_t p = 0; p < sch->passes.size(); ++p) {
                 const aqs_op& op = sch->passes[p];
+                if (sch->tile_index[p] >= 0) {
+                    const auto& t = sch->tiles[sch->tile_index[p]];
+                    auto& o = sch->dev_tile[p];
+                    o[0] = append(t.segs.data(), t.segs.size() * sizeof(TileSegDev));
+                    o[1] = append(t.ops.data(), t.ops.size() * sizeof(TileOpDev));
+                    o[2] = append(t.terms.data(), t.terms.size() * sizeof(TileTermDev));
+                    if (s->precision == AQS_C128) {
+                        o[3] = append(t.m128.data(), t.m128.size() * sizeof(double));
+                        o[4] = append(t.z128.data(), t.z128.size() * sizeof(double));
+                    }
+                    continue;
+                }
                 if (sch->diag_index[p] >= 0) {
                     const auto& d = sch->diags[sch->diag_index[p]];
                     if (s->precision == AQS_C64) {
@@ -767,6 +756,13 @@ aqs_status aqs_apply_schedule(aqs_state s, aqs_schedule sch) {
         for (size_t p = 0; p < sch->passes.size(); ++p) {
             const aqs_op& op = sch->passes[p];
             const void* d = (const char*)sch->dev + sch->dev_off[p];
+            if (sch->tile_index[p] >= 0) {
+                const auto& o = sch->dev_tile[p];
+                const char* base = (const char*)sch->dev;
+                apply_tile_pass(s, sch->tiles[sch->tile_index[p]],
+                                TileDevProgram{base + o[0], base + o[1], base + o[2], base + o[3], base + o[4]});
+                continue;
+            }
             if (sch->diag_index[p] >= 0) {
                 const auto& dp = sch->diags[sch->diag_index[p]];
                 apply_diag_table(s, dp.required, dp.anchor, dp.chunks, d);

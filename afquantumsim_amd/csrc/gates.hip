@@ -295,19 +295,25 @@ __global__ __launch_bounds__(kBlock) void k_phase(PhaseArgs<T2, R> a) {
     }
 }
 
+// U float4s per lane; IT = index type (32-bit when the float4 index space
+// fits, the shape that measured fastest for streaming passes).
+template <int U, typename IT>
 __global__ __launch_bounds__(kBlock) void k_phase_v4(PhaseArgs<float4, float> a) {
-    const uint64_t base = (uint64_t)blockIdx.x * (kBlock * kUnroll) + threadIdx.x;
-    float4 v[kUnroll];
-    uint64_t idx[kUnroll];
+    const IT units = (IT)a.units, setmask = (IT)a.setmask;
+    const IT base = (IT)blockIdx.x * (IT)(kBlock * U) + (IT)threadIdx.x;
+    float4 v[U];
+    IT idx[U];
+    bool ok[U];
 #pragma unroll
-    for (int j = 0; j < kUnroll; ++j) {
-        const uint64_t u = base + (uint64_t)j * kBlock;
-        idx[j] = (u < a.units) ? (insert_zeros(u, a.fixed) | a.setmask) : ~0ull;
-        if (idx[j] != ~0ull) v[j] = ld_nt(&a.sv[idx[j]]);
+    for (int j = 0; j < U; ++j) {
+        const IT u = base + (IT)j * kBlock;
+        ok[j] = u < units;
+        idx[j] = insert_zeros_t<IT>(u, a.fixed) | setmask;
+        if (ok[j]) v[j] = ld_nt(&a.sv[idx[j]]);
     }
 #pragma unroll
-    for (int j = 0; j < kUnroll; ++j) {
-        if (idx[j] == ~0ull) continue;
+    for (int j = 0; j < U; ++j) {
+        if (!ok[j]) continue;
         float4 x = v[j], y;
         y.x = a.pr * x.x - a.pi * x.y;
         y.y = a.pr * x.y + a.pi * x.x;
@@ -628,7 +634,12 @@ void apply_c64_fast_or_generic(aqs_state s, const aqs_op& op) {
                 a.units = s->N >> (1 + fp.size());
                 a.pr = mf[0];
                 a.pi = mf[1];
-                hipLaunchKernelGGL(k_phase_v4, dim3(blocks_for(a.units, kUnroll)), dim3(kBlock), 0, s->stream, a);
+                if (s->N / 2 <= (1ull << 32))
+                    hipLaunchKernelGGL((k_phase_v4<1, uint32_t>), dim3(blocks_for(a.units, 1)), dim3(kBlock), 0,
+                                       s->stream, a);
+                else
+                    hipLaunchKernelGGL((k_phase_v4<2, uint64_t>), dim3(blocks_for(a.units, 2)), dim3(kBlock), 0,
+                                       s->stream, a);
             } else {
                 PhaseArgs<float2, float> a{};
                 a.sv = (float2*)s->amps;

@@ -56,10 +56,14 @@ aqs_status guard(F&& f) {
     }
 }
 
-// Number of profile slots: index = aqs_op_kind (1..6), 7 = MATK on MFMA,
-// 0 = everything else (state preparation, measurement).
-constexpr int kProfileSlots = 8;
+// Profile slots: index = aqs_op_kind (1..6) of a plain pass, 7 = MATK on
+// MFMA, 8 = diagonal table, 9 = bit permutation, 10 = tile pass, 0 =
+// everything else (state preparation, measurement).
+constexpr int kProfileSlots = AQS_PROFILE_SLOTS;
 constexpr int kSlotMfma = 7;
+constexpr int kSlotDiagTable = 8;
+constexpr int kSlotPermute = 9;
+constexpr int kSlotTile = 10;
 
 struct ProfileRec {
     hipEvent_t start, stop;

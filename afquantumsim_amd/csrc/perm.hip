@@ -110,7 +110,7 @@ void apply_bit_involution(aqs_state s, const std::vector<int>& beta) {
     a.outer = 1ull << a.nr;
     const uint64_t blocks = std::min<uint64_t>(a.outer, 16384);
     const size_t lds = 2 * ((size_t)1 << a.nt) * s->amp_bytes();
-    ProfScope ps(s, AQS_OP_SWAP);
+    ProfScope ps(s, kSlotPermute);
     if (s->precision == AQS_C64)
         hipLaunchKernelGGL(k_bitperm<float2>, dim3((unsigned)blocks), dim3(kBlock), lds, s->stream, a);
     else

@@ -1,0 +1,146 @@
+// Compiled schedules: the passes aqs_fuse / aqs_fuse_tiled produce and
+// aqs_apply_schedule runs (shared by fusion.cpp, tiling.cpp, tile.hip).
+#ifndef AQS_SCHEDULE_H
+#define AQS_SCHEDULE_H
+
+#include <array>
+#include <complex>
+#include <cstdint>
+#include <vector>
+
+#include "internal.h"
+
+namespace aqs_hip {
+
+using cd = std::complex<double>;
+
+struct DiagPass {
+    std::vector<int> required;            // amplitude bit positions
+    std::vector<int> anchor;              // amplitude bit positions (<= 5)
+    std::vector<std::vector<int>> chunks; // amplitude bit positions (<= 6 x 5)
+    std::vector<cd> table;                // [2^|anchor|][chunks][32]
+};
+
+// Tile pass: the state is swept in tiles of 2^m amplitudes that share every
+// index bit outside `local` (m ascending bit positions, bits 0..4 always
+// among them).  A tile is held in LDS; each segment gives every thread 16 of
+// its amplitudes (4 register slots over local positions `reg`, the thread id
+// spread over the other local positions `thr`, ascending) and runs a list of
+// ops on them in registers.
+constexpr int kTileRegBits = 4;
+constexpr int kTileMaxThreadBits = 9;
+constexpr int kTileMaxLocal = kTileRegBits + kTileMaxThreadBits;
+
+enum TileOpKind : int32_t {
+    TILE_MAT = 0,     // 2x2 on register slot `slot`
+    TILE_XCH = 1,     // exchange across register slot `slot`
+    TILE_PHASES = 2,  // a run of diagonal gates as phase terms [term_begin, +term_count)
+};
+
+// Kernel dispatch code of an op (one switch per op, no divergent branch
+// around the register array):
+//   form 0  no control inside the tile (controls outside it: one test per tile)
+//   form 1  controls on register slots only (one uniform test per pair)
+//   form 2  controls on thread-held positions, or "any" controls (per-thread
+//           predicate, results selected)
+// MAT: 0 + 4*form + slot, XCH: 12 + 4*form + slot, PHASES: 24 direct, 25 zeta.
+constexpr int kTileCodePhases = 24;
+inline int tile_code(int kind, int form, int slot, bool zeta) {
+    if (kind == TILE_PHASES) return kTileCodePhases + (zeta ? 1 : 0);
+    return (kind == TILE_XCH ? 12 : 0) + 4 * form + slot;
+}
+
+// 64 bytes: the kernel keeps the next op's fields in flight while it runs
+// the current one.  Complex64 kernels read the FP32 matrix here; complex128
+// kernels read TilePass::m128[8 * op].
+struct TileOpDev {
+    int32_t code;     // tile_code()
+    uint32_t any;     // controls: at least one set instead of all set
+    uint32_t c_reg;   // control register slots (mask over the 4 slots)
+    uint32_t c_thr;   // control local positions held by the thread (mask over local positions)
+    uint64_t c_glb;   // control index bits outside the tile
+    uint32_t term_begin, term_count;
+    float mf[8];      // MAT: row-major 2x2 in FP32
+};
+static_assert(sizeof(TileOpDev) == 64, "compact tile op");
+
+// Phase term: multiply by z where every bit of the three masks is set
+// (register slots, thread-held local positions, index bits outside the
+// tile).  Terms of one PHASES op are sorted by register mask; the first term
+// of each equal-mask group carries the group length.  z in FP32 here, in
+// FP64 in TilePass::z128[2 * term].
+struct TileTermDev {
+    uint16_t m_reg;
+    uint16_t group;
+    uint32_t m_thr;
+    uint64_t m_glb;
+    float zf[2];
+};
+static_assert(sizeof(TileTermDev) == 24, "compact tile term");
+
+struct TileSegDev {
+    int32_t reg[kTileRegBits];
+    int32_t thr[kTileMaxThreadBits + 1];
+    uint32_t op_begin, op_count;
+};
+
+struct TilePass {
+    int m = 0;                     // local bits = 4 + log2(threads)
+    std::vector<int> local;        // ascending index-bit positions
+    std::vector<TileSegDev> segs;
+    std::vector<TileOpDev> ops;
+    std::vector<double> m128;  // [ops][8]
+    std::vector<TileTermDev> terms;
+    std::vector<double> z128;  // [terms][2]
+};
+
+// Device copies of one tile pass's program.
+struct TileDevProgram {
+    const void* segs;
+    const void* ops;
+    const void* terms;
+    const void* m128;  // complex128 states only
+    const void* z128;
+};
+
+// Runs one tile pass.
+void apply_tile_pass(aqs_state s, const TilePass& pass, const TileDevProgram& prog);
+
+}  // namespace aqs_hip
+
+struct aqs_schedule_s {
+    int n = 0;
+    size_t source_ops = 0;
+    std::vector<aqs_op> passes;
+    std::vector<std::vector<double>> mats;  // owned FP64 matrices (MATK passes)
+    std::vector<int> diag_index;            // -1 or index into diags
+    std::vector<aqs_hip::DiagPass> diags;
+    std::vector<std::pair<int, int>> perm_index;  // (pass, index into perms)
+    std::vector<std::vector<int>> perms;          // bit involutions (AQS_OP_PERMB passes)
+    std::vector<int> tile_index;                  // -1 or index into tiles
+    std::vector<aqs_hip::TilePass> tiles;
+    // device copies, built at first use for a given precision
+    void* dev = nullptr;
+    int dev_precision = -1;
+    std::vector<size_t> dev_off;
+    std::vector<std::array<size_t, 5>> dev_tile;  // tile passes: segs, ops, terms, m128, z128
+    std::vector<char> dev_mfma;
+
+    // appends a pass with no side data
+    void push_plain(const aqs_op& op, std::vector<double> mat = {}) {
+        passes.push_back(op);
+        mats.push_back(std::move(mat));
+        diag_index.push_back(-1);
+        tile_index.push_back(-1);
+    }
+};
+
+namespace aqs_hip {
+
+// Logical qubit q sits at physical qubit L[q]: appends the (at most two)
+// bit-involution passes that move every amplitude to its logical index.
+void emit_restore_order(aqs_schedule_s* sch, const std::vector<int>& L);
+
+}  // namespace aqs_hip
+
+#endif

@@ -1,0 +1,515 @@
+// Cache-blocked compile(): packs gates into tile passes (tile.hip).
+//
+// The reference's QCircuit::compile (src/quantum.cpp:199-210) multiplies
+// every gate into one 2^n x 2^n matrix and simulate() then makes one dense
+// product; the per-gate path makes one state sweep per gate.  A tile pass
+// makes one sweep for a whole run of gates: the state is cut into tiles of
+// 2^m amplitudes that share every index bit outside a chosen local set, and
+// every gate whose non-diagonal qubits are local is applied while the tile
+// sits in LDS.  Diagonal gates and controls may sit on any qubit (outside
+// the tile their bits are fixed per tile).
+//
+// Pass construction walks the remaining gates in program order.  A gate joins
+// the pass when (a) its non-diagonal qubits are local or can still be made
+// local (the local set grows up to m bits, bits 0..4 always in it), and
+// (b) it commutes with every gate passed over so far: per index bit the walk
+// records whether a skipped gate used it non-diagonally (then no later gate
+// may use it) or only diagonally -- as a control or a diagonal target --
+// (then later gates may use it diagonally too).  Two gates that share only
+// bits both use diagonally commute, so the pass applies its gates in program
+// order and every skipped gate still sees the same operator before it.
+//
+// Inside a pass, segments give each thread 4 register slots; a segment runs
+// the pass's gates, in order and under the same commutation rule, whose
+// non-diagonal qubit is in its slots, and the next segment's slots are the
+// first 4 distinct targets of the gates that are ready next.
+//
+// Uncontrolled swaps become relabellings undone by bit permutations at the
+// end (as in aqs_fuse); a controlled swap is CX(b->a) CCX(c, a->b) CX(b->a);
+// dense Gate / ControlGate blocks stay plain passes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "internal.h"
+#include "schedule.h"
+
+using namespace aqs_hip;
+
+namespace {
+
+// A phase term: multiply by z where every listed index bit is 1.
+struct Term {
+    std::vector<int> bits;
+    cd z;
+};
+
+struct GOp {
+    aqs_op op{};           // plain passes: the op itself
+    int kind = -1;         // TILE_MAT / TILE_XCH / TILE_PHASES (diagonal gate), -1 = plain pass
+    int tbit = -1;         // MAT / XCH target bit
+    std::vector<int> nd;   // bits used non-diagonally
+    std::vector<int> dg;   // bits used diagonally (controls, diagonal targets)
+    std::vector<int> ctrl; // control bits
+    bool any = false;
+    double m[8] = {};
+    std::vector<Term> terms;  // diagonal gates
+};
+
+class Tiler {
+   public:
+    Tiler(aqs_schedule_s* sch, int n, int m) : sch_(sch), n_(n), m_(m) {}
+
+    void run(const aqs_op* ops_in, size_t count) {
+        std::vector<int> L(n_);
+        for (int q = 0; q < n_; ++q) L[q] = q;
+        for (size_t i = 0; i < count; ++i) {
+            aqs_op op = ops_in[i];
+            if (op.kind == AQS_OP_SWAP && op.ncontrols == 0 && !(op.flags & AQS_CTRL_ANY)) {
+                std::swap(L[op.targets[0]], L[op.targets[1]]);
+                continue;
+            }
+            for (int j = 0; j < op.ntargets; ++j) op.targets[j] = L[op.targets[j]];
+            for (int j = 0; j < op.ncontrols; ++j) op.controls[j] = L[op.controls[j]];
+            lower(op);
+        }
+        std::vector<char> done(g_.size(), 0);
+        size_t first = 0;
+        while (true) {
+            while (first < g_.size() && done[first]) ++first;
+            if (first == g_.size()) break;
+            if (g_[first].kind < 0) {
+                plain(g_[first].op);
+                done[first] = 1;
+                continue;
+            }
+            build_pass(first, done);
+        }
+        emit_restore_order(sch_, L);
+    }
+
+   private:
+    aqs_schedule_s* sch_;
+    int n_, m_;
+    std::vector<GOp> g_;
+
+    int bit(int q) const { return n_ - 1 - q; }
+
+    void lower(const aqs_op& op) {
+        GOp o;
+        o.op = op;
+        o.any = (op.flags & AQS_CTRL_ANY) != 0;
+        for (int j = 0; j < op.ncontrols; ++j) o.ctrl.push_back(bit(op.controls[j]));
+        switch (op.kind) {
+            case AQS_OP_MAT1:
+            case AQS_OP_X:
+                o.kind = op.kind == AQS_OP_MAT1 ? TILE_MAT : TILE_XCH;
+                o.tbit = bit(op.targets[0]);
+                o.nd = {o.tbit};
+                o.dg = o.ctrl;
+                std::memcpy(o.m, op.m, sizeof(o.m));
+                break;
+            case AQS_OP_DIAG1: {
+                if (o.any) break;
+                // diag(d0, d1) on bit t under controls C = [C] d0 * [C, t] d1 / d0
+                o.kind = TILE_PHASES;
+                const int t = bit(op.targets[0]);
+                o.dg = o.ctrl;
+                o.dg.push_back(t);
+                const cd d0(op.m[0], op.m[1]), d1(op.m[6], op.m[7]);
+                if (d0 == cd(0)) break;  // not unitary: keep the plain pass
+                o.terms.push_back({o.ctrl, d0});
+                o.terms.push_back({o.dg, d1 / d0});
+                break;
+            }
+            case AQS_OP_PHASE:
+                if (o.any) break;
+                o.kind = TILE_PHASES;
+                o.dg = o.ctrl;
+                for (int j = 0; j < op.ntargets; ++j) o.dg.push_back(bit(op.targets[j]));
+                o.terms.push_back({o.dg, cd(op.m[0], op.m[1])});
+                break;
+            case AQS_OP_SWAP:
+                if (!o.any) {
+                    // controlled swap: CX(b->a) CCX(c.., a->b) CX(b->a)
+                    const int a = op.targets[0], b = op.targets[1];
+                    aqs_op x{};
+                    x.kind = AQS_OP_X;
+                    x.ntargets = 1;
+                    x.targets[0] = a;
+                    x.ncontrols = 1;
+                    x.controls[0] = b;
+                    lower(x);
+                    aqs_op t = x;
+                    t.targets[0] = b;
+                    t.ncontrols = 0;
+                    for (int j = 0; j < op.ncontrols; ++j) t.controls[t.ncontrols++] = op.controls[j];
+                    lower_wide_x(t, a);
+                    lower(x);
+                    return;
+                }
+                break;
+            default:
+                break;
+        }
+        if (o.kind == TILE_PHASES && o.terms.empty()) o.kind = -1;
+        if (o.kind < 0) {
+            o.nd.clear();
+            o.dg.clear();
+            for (int j = 0; j < op.ntargets; ++j) o.nd.push_back(bit(op.targets[j]));
+            for (int c : o.ctrl) o.nd.push_back(c);
+        }
+        g_.push_back(std::move(o));
+    }
+
+    // X on `t.targets[0]` controlled by t's controls plus qubit `extra` (the
+    // aqs_op form holds at most 3 controls; the tile form has no limit).
+    void lower_wide_x(const aqs_op& t, int extra) {
+        GOp o;
+        o.op = t;
+        o.kind = TILE_XCH;
+        o.tbit = bit(t.targets[0]);
+        o.nd = {o.tbit};
+        for (int j = 0; j < t.ncontrols; ++j) o.ctrl.push_back(bit(t.controls[j]));
+        o.ctrl.push_back(bit(extra));
+        o.dg = o.ctrl;
+        g_.push_back(std::move(o));
+    }
+
+    void plain(const aqs_op& op) {
+        std::vector<double> mat;
+        if (op.kind == AQS_OP_MATK) {
+            const size_t K = (size_t)1 << op.ntargets;
+            mat.assign(op.u, op.u + 2 * K * K);
+        }
+        sch_->push_plain(op, std::move(mat));
+    }
+
+    // Per-bit commutation state of the gates passed over: 0 untouched,
+    // 1 used only diagonally, 2 used non-diagonally.
+    static bool blocked(const GOp& o, const std::vector<char>& st) {
+        for (int b : o.nd)
+            if (st[b] != 0) return true;
+        for (int b : o.dg)
+            if (st[b] == 2) return true;
+        return false;
+    }
+    static void pass_over(const GOp& o, std::vector<char>& st) {
+        for (int b : o.nd) st[b] = 2;
+        for (int b : o.dg) st[b] = std::max<char>(st[b], 1);
+    }
+
+    void build_pass(size_t first, std::vector<char>& done) {
+        std::vector<char> local(n_, 0), st(n_, 0);
+        int nlocal = 0;
+        for (int b = 0; b < std::min(5, n_); ++b) local[b] = 1, ++nlocal;
+        std::vector<size_t> members;
+        int saturated = 0;  // bits at state 2
+        for (size_t i = first; i < g_.size() && saturated < n_; ++i) {
+            if (done[i]) continue;
+            const GOp& o = g_[i];
+            bool ok = o.kind >= 0 && !blocked(o, st);
+            int need = 0;
+            if (ok)
+                for (int b : o.nd)
+                    if (!local[b]) ++need;
+            if (ok && nlocal + need > m_) ok = false;
+            if (ok) {
+                for (int b : o.nd)
+                    if (!local[b]) local[b] = 1, ++nlocal;
+                members.push_back(i);
+            } else {
+                for (int b : o.nd)
+                    if (st[b] != 2) ++saturated;
+                pass_over(o, st);
+            }
+        }
+        // fill the local set with the lowest remaining bits (longer HBM runs)
+        for (int b = 0; b < n_ && nlocal < m_; ++b)
+            if (!local[b]) local[b] = 1, ++nlocal;
+        TilePass tp;
+        tp.m = m_;
+        std::vector<int> lp(n_, -1);
+        for (int b = 0; b < n_; ++b)
+            if (local[b]) {
+                lp[b] = (int)tp.local.size();
+                tp.local.push_back(b);
+            }
+        segments(members, lp, tp);
+        check_pass(tp);
+        for (size_t i : members) done[i] = 1;
+        if (tp.segs.empty()) return;  // the gates cancelled exactly
+        aqs_op op{};
+        op.kind = AQS_OP_TILE;
+        sch_->push_plain(op);
+        sch_->tile_index.back() = (int)sch_->tiles.size();
+        sch_->tiles.push_back(std::move(tp));
+    }
+
+    void segments(const std::vector<size_t>& members, const std::vector<int>& lp, TilePass& tp) {
+        std::vector<char> mdone(members.size(), 0);
+        size_t left = members.size();
+        while (left) {
+            // slots: the first 4 distinct targets among gates that are ready
+            std::vector<int> reg;
+            std::vector<char> st(n_, 0);
+            for (size_t k = 0; k < members.size() && reg.size() < (size_t)kTileRegBits; ++k) {
+                if (mdone[k]) continue;
+                const GOp& o = g_[members[k]];
+                if (blocked(o, st)) {
+                    pass_over(o, st);
+                    continue;
+                }
+                if (o.kind == TILE_PHASES) continue;
+                const int p = lp[o.tbit];
+                if (std::find(reg.begin(), reg.end(), p) == reg.end()) reg.push_back(p);
+            }
+            for (int p = 0; p < m_ && reg.size() < (size_t)kTileRegBits; ++p)
+                if (std::find(reg.begin(), reg.end(), p) == reg.end()) reg.push_back(p);
+            // run every gate that is ready and fits the slots
+            TileSegDev sd{};
+            for (int r = 0; r < kTileRegBits; ++r) sd.reg[r] = reg[r];
+            int t = 0;
+            for (int p = 0; p < m_; ++p)
+                if (std::find(reg.begin(), reg.end(), p) == reg.end()) sd.thr[t++] = p;
+            sd.op_begin = (uint32_t)tp.ops.size();
+            std::fill(st.begin(), st.end(), 0);
+            std::vector<const GOp*> run;  // consecutive diagonal gates, merged into one op
+            const size_t left_before = left;
+            for (size_t k = 0; k < members.size(); ++k) {
+                if (mdone[k]) continue;
+                const GOp& o = g_[members[k]];
+                const bool fits = o.kind == TILE_PHASES || std::find(reg.begin(), reg.end(), lp[o.tbit]) != reg.end();
+                if (!fits || blocked(o, st)) {
+                    pass_over(o, st);
+                    continue;
+                }
+                if (o.kind == TILE_PHASES) {
+                    run.push_back(&o);
+                } else {
+                    flush_phases(run, lp, reg, tp);
+                    push_op(o, lp, reg, tp);
+                }
+                mdone[k] = 1;
+                --left;
+            }
+            flush_phases(run, lp, reg, tp);
+            sd.op_count = (uint32_t)tp.ops.size() - sd.op_begin;
+            if (left == left_before) fail(AQS_ERR_RUNTIME, "tile schedule made no progress");
+            // (phase terms that multiply to exactly 1 leave nothing to run)
+            if (sd.op_count) tp.segs.push_back(sd);
+        }
+    }
+
+    // Every position the kernel turns into an LDS address or shift stays in
+    // range: slots < 4, local positions < m and distinct, index bits < n.
+    void check_pass(const TilePass& tp) const {
+        const int m = tp.m, w = m - kTileRegBits;
+        bool ok = m >= 10 && m <= kTileMaxLocal && (int)tp.local.size() == m;
+        for (const auto& s : tp.segs) {
+            uint32_t seen = 0;
+            for (int r = 0; r < kTileRegBits; ++r) {
+                ok = ok && s.reg[r] >= 0 && s.reg[r] < m && !(seen >> s.reg[r] & 1u);
+                if (ok) seen |= 1u << s.reg[r];
+            }
+            for (int b = 0; b < w; ++b) {
+                ok = ok && s.thr[b] >= 0 && s.thr[b] < m && !(seen >> s.thr[b] & 1u);
+                if (ok) seen |= 1u << s.thr[b];
+            }
+            ok = ok && (uint64_t)s.op_begin + s.op_count <= tp.ops.size();
+        }
+        for (const auto& o : tp.ops) {
+            ok = ok && o.code >= 0 && o.code <= kTileCodePhases + 1 && o.c_reg < (1u << kTileRegBits) &&
+                 o.c_thr < (1u << m) && (o.c_glb >> n_) == 0;
+            if (o.code < kTileCodePhases) {
+                const int form = (o.code % 12) / 4;
+                ok = ok && (form == 2 || (!o.any && o.c_thr == 0 && (form == 1) == (o.c_reg != 0)));
+            }
+            if (o.code >= kTileCodePhases) {
+                ok = ok && o.term_count > 0 && (uint64_t)o.term_begin + o.term_count <= tp.terms.size();
+                // groups tile the op's terms exactly
+                for (uint32_t k = 0; ok && k < o.term_count;) {
+                    const uint32_t g = tp.terms[o.term_begin + k].group;
+                    ok = g > 0 && k + g <= o.term_count;
+                    k += g ? g : 1;
+                }
+            }
+        }
+        ok = ok && tp.m128.size() == 8 * tp.ops.size() && tp.z128.size() == 2 * tp.terms.size();
+        for (const auto& t : tp.terms)
+            ok = ok && t.m_reg < (1u << kTileRegBits) && t.m_thr < (1u << m) && (t.m_glb >> n_) == 0;
+        if (!ok) fail(AQS_ERR_RUNTIME, "tile schedule malformed");
+    }
+
+    static int slot_of(const std::vector<int>& reg, int p) {
+        return (int)(std::find(reg.begin(), reg.end(), p) - reg.begin());
+    }
+
+    // Split index bits into register slots, thread-held positions and bits
+    // outside the tile for the segment layout `reg`.
+    void masks(const std::vector<int>& bits, const std::vector<int>& lp, const std::vector<int>& reg, uint32_t& m_reg,
+               uint32_t& m_thr, uint64_t& m_glb) const {
+        m_reg = m_thr = 0;
+        m_glb = 0;
+        for (int b : bits) {
+            if (lp[b] < 0)
+                m_glb |= 1ull << b;
+            else if (slot_of(reg, lp[b]) < kTileRegBits)
+                m_reg |= 1u << slot_of(reg, lp[b]);
+            else
+                m_thr |= 1u << lp[b];
+        }
+    }
+
+    void push_op(const GOp& o, const std::vector<int>& lp, const std::vector<int>& reg, TilePass& tp) const {
+        TileOpDev d{};
+        d.any = o.any ? 1 : 0;
+        masks(o.ctrl, lp, reg, d.c_reg, d.c_thr, d.c_glb);
+        const int form = (d.any || d.c_thr) ? 2 : d.c_reg ? 1 : 0;
+        d.code = tile_code(o.kind, form, slot_of(reg, lp[o.tbit]), false);
+        for (int i = 0; i < 8; ++i) d.mf[i] = (float)o.m[i];
+        tp.ops.push_back(d);
+        tp.m128.insert(tp.m128.end(), o.m, o.m + 8);
+    }
+
+    // One PHASES op for a run of diagonal gates: their terms, with equal
+    // masks multiplied together (FP64) and factors of exactly 1 dropped,
+    // sorted by register mask into groups.  Per thread the kernel forms each
+    // group's product; the direct form multiplies it into the 16 / 2^|mask|
+    // amplitudes of the group, the zeta form into one of 16 subset factors
+    // (then 32 + 16 products); the cheaper is chosen.
+    void flush_phases(std::vector<const GOp*>& run, const std::vector<int>& lp, const std::vector<int>& reg,
+                      TilePass& tp) const {
+        if (run.empty()) return;
+        struct M {
+            uint32_t reg, thr;
+            uint64_t glb;
+            cd z;
+        };
+        std::vector<M> merged;
+        for (const GOp* o : run)
+            for (const Term& t : o->terms) {
+                M d{0, 0, 0, t.z};
+                masks(t.bits, lp, reg, d.reg, d.thr, d.glb);
+                size_t i = 0;
+                while (i < merged.size() && !(merged[i].reg == d.reg && merged[i].thr == d.thr && merged[i].glb == d.glb))
+                    ++i;
+                if (i == merged.size())
+                    merged.push_back(d);
+                else
+                    merged[i].z *= t.z;
+            }
+        run.clear();
+        merged.erase(std::remove_if(merged.begin(), merged.end(), [](const M& m) { return m.z == cd(1); }),
+                     merged.end());
+        if (merged.empty()) return;
+        std::stable_sort(merged.begin(), merged.end(), [](const M& a, const M& b) { return a.reg < b.reg; });
+        TileOpDev op{};
+        op.term_begin = (uint32_t)tp.terms.size();
+        op.term_count = (uint32_t)merged.size();
+        double direct = 0, groups = 0;
+        size_t gend = 0;  // end of the current group
+        for (size_t i = 0; i < merged.size(); ++i) {
+            TileTermDev d{};
+            d.m_reg = (uint16_t)merged[i].reg;
+            d.m_thr = merged[i].thr;
+            d.m_glb = merged[i].glb;
+            d.zf[0] = (float)merged[i].z.real();
+            d.zf[1] = (float)merged[i].z.imag();
+            if (i == gend) {
+                size_t e = i;
+                while (e < merged.size() && merged[e].reg == merged[i].reg && e - i < 0xFFFF) ++e;
+                d.group = (uint16_t)(e - i);
+                gend = e;
+                direct += 16.0 / (double)(1u << __builtin_popcount(merged[i].reg));
+                groups += 1;
+            }
+            tp.terms.push_back(d);
+            tp.z128.push_back(merged[i].z.real());
+            tp.z128.push_back(merged[i].z.imag());
+        }
+        op.code = tile_code(TILE_PHASES, 0, 0, 48.0 + groups < direct);
+        tp.ops.push_back(op);
+        tp.m128.insert(tp.m128.end(), 8, 0.0);
+    }
+};
+
+// Relative cost of one pass in full-state sweeps (read + write of every
+// amplitude at the streaming rate).  In-tile work: a 2x2 product costs
+// about 1/13 of a sweep of the same state, an exchange or diagonal factor
+// about 1/30, a segment's LDS round trip about 1/40; a tile pass costs the
+// larger of its sweep and its work.
+double pass_cost(const aqs_schedule_s* sch, size_t p) {
+    const aqs_op& op = sch->passes[p];
+    if (sch->tile_index[p] >= 0) {
+        const TilePass& t = sch->tiles[sch->tile_index[p]];
+        double work = 0.025 * (double)t.segs.size();
+        for (const auto& o : t.ops) {
+            if (o.code < 12)
+                work += 0.075;
+            else if (o.code < kTileCodePhases)
+                work += 0.02;
+            else
+                work += o.code == kTileCodePhases + 1 ? 0.03 + 0.004 * o.term_count : 0.004 * o.term_count + 0.01;
+        }
+        return std::max(1.0, work) + 0.05;
+    }
+    switch (op.kind) {
+        case AQS_OP_MATK:
+            if (op.ncontrols == 0 && op.ntargets == 5) return 1.2;
+            if (op.ncontrols == 0 && op.ntargets == 4) return 1.15;
+            return 1.0 + 0.05 * (double)((size_t)1 << op.ntargets);
+        case AQS_OP_PHASE: {
+            const int bits = op.ntargets + op.ncontrols;
+            return bits >= 2 ? 0.4 : 0.6;
+        }
+        case AQS_OP_SWAP:
+            return 0.6;
+        default:
+            return op.ncontrols ? 0.6 : 1.0;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+aqs_status aqs_fuse_tiled(const aqs_op* ops, size_t count, int nqubits, int tile_qubits, aqs_schedule* out) {
+    if (nqubits < 10) return aqs_fuse(ops, count, nqubits, 5, out);
+    return guard([&] {
+        if (!out) fail(AQS_ERR_INVALID_ARGUMENT, "null output schedule");
+        *out = nullptr;
+        if (count && !ops) fail(AQS_ERR_INVALID_ARGUMENT, "null op array");
+        if (nqubits > AQS_MAX_QUBITS) fail(AQS_ERR_INVALID_ARGUMENT, "bad qubit count");
+        if (tile_qubits == 0) tile_qubits = kTileMaxLocal;
+        if (tile_qubits < 10 || tile_qubits > kTileMaxLocal)
+            fail(AQS_ERR_INVALID_ARGUMENT, "tile_qubits must be 10..13 (0 = 13)");
+        for (size_t i = 0; i < count; ++i) validate_op(ops[i], nqubits);
+        auto* sch = new aqs_schedule_s();
+        sch->n = nqubits;
+        sch->source_ops = count;
+        try {
+            Tiler t(sch, nqubits, std::min(tile_qubits, nqubits));
+            t.run(ops, count);
+            for (size_t p = 0; p < sch->passes.size(); ++p)
+                if (sch->passes[p].kind == AQS_OP_MATK) sch->passes[p].u = sch->mats[p].data();
+        } catch (...) {
+            delete sch;
+            throw;
+        }
+        *out = sch;
+    });
+}
+
+aqs_status aqs_schedule_cost(aqs_schedule sch, double* sweeps) {
+    return guard([&] {
+        if (!sch) fail(AQS_ERR_INVALID_ARGUMENT, "null schedule");
+        double c = 0;
+        for (size_t p = 0; p < sch->passes.size(); ++p) c += pass_cost(sch, p);
+        if (sweeps) *sweeps = c;
+    });
+}
+
+}  // extern "C"

@@ -32,9 +32,10 @@ OP_MATK = 5
 OP_PHASE = 6
 OP_DIAGT = 7
 OP_PERMB = 8
+OP_TILE = 9
 CTRL_ANY = 1
 
-PROFILE_SLOTS = ["other", "MAT1", "DIAG1", "X", "SWAP", "MATK", "PHASE", "MATK_MFMA"]
+PROFILE_SLOTS = ["other", "MAT1", "DIAG1", "X", "SWAP", "MATK", "PHASE", "MATK_MFMA", "DIAGT", "PERMB", "TILE"]
 
 
 class AqsError(RuntimeError):
@@ -77,7 +78,7 @@ EXPORTS = [
     "aqs_state_copy", "aqs_state_qubits", "aqs_state_precision", "aqs_state_device_ptr",
     "aqs_state_set_stream", "aqs_state_sync", "aqs_state_set_basis", "aqs_state_product",
     "aqs_state_gaussian", "aqs_state_upload", "aqs_state_download", "aqs_apply", "aqs_fuse",
-    "aqs_schedule_destroy", "aqs_schedule_info", "aqs_schedule_get", "aqs_apply_schedule",
+    "aqs_fuse_tiled", "aqs_schedule_cost", "aqs_schedule_destroy", "aqs_schedule_info", "aqs_schedule_get", "aqs_apply_schedule",
     "aqs_norm2", "aqs_normalize", "aqs_prob1", "aqs_prob1_all", "aqs_expect_sum_z",
     "aqs_probabilities", "aqs_amplitude", "aqs_sample", "aqs_measure_qubit", "aqs_collapse_to",
     "aqs_inner", "aqs_profile_enable", "aqs_profile_read", "aqs_timer_start", "aqs_timer_stop",
@@ -131,6 +132,8 @@ def load(path: str = LIB_PATH):
         "aqs_state_download": [vp, vp],
         "aqs_apply": [vp, ctypes.POINTER(Op), sz],
         "aqs_fuse": [ctypes.POINTER(Op), sz, i32, i32, ctypes.POINTER(vp)],
+        "aqs_fuse_tiled": [ctypes.POINTER(Op), sz, i32, i32, ctypes.POINTER(vp)],
+        "aqs_schedule_cost": [vp, dp],
         "aqs_schedule_destroy": [vp],
         "aqs_schedule_info": [vp, ctypes.POINTER(sz), ctypes.POINTER(sz)],
         "aqs_schedule_get": [vp, sz, ctypes.POINTER(Op)],
@@ -328,10 +331,11 @@ class State:
         check(_lib.aqs_profile_enable(self.h, int(enable)))
 
     def profile_read(self):
-        ms = (ctypes.c_double * 8)()
-        cnt = (ctypes.c_uint64 * 8)()
+        k = len(PROFILE_SLOTS)
+        ms = (ctypes.c_double * k)()
+        cnt = (ctypes.c_uint64 * k)()
         check(_lib.aqs_profile_read(self.h, ms, cnt))
-        return {PROFILE_SLOTS[i]: (ms[i], cnt[i]) for i in range(8) if cnt[i]}
+        return {PROFILE_SLOTS[i]: (ms[i], cnt[i]) for i in range(k) if cnt[i]}
 
     def timer_start(self):
         check(_lib.aqs_timer_start(self.h))
@@ -343,16 +347,47 @@ class State:
 
 
 class Schedule:
-    """A fused gate schedule (compile() output)."""
+    """A fused gate schedule (compile() output).
 
-    def __init__(self, ops, n: int, max_qubits: int = 5):
+    mode "blocks": dense blocks of <= max_qubits qubits (aqs_fuse);
+    mode "tiled":  cache-blocked tile passes of 2^tile_qubits amplitudes
+                   (aqs_fuse_tiled, tile_qubits 0 = 13);
+    mode "auto":   both, keeping the one of lower modelled cost.
+    """
+
+    def __init__(self, ops, n: int, max_qubits: int = 5, mode: str = "blocks", tile_qubits: int = 0):
         L = load()
         arr, keep = pack_ops(ops)
-        h = ctypes.c_void_p()
-        check(L.aqs_fuse(arr, len(ops), n, max_qubits, ctypes.byref(h)))
-        del keep
-        self.h = h
         self.n = n
+        self.h = None
+        if mode not in ("blocks", "tiled", "auto"):
+            raise ValueError(f"unknown schedule mode {mode!r}")
+        cands = []
+        if mode in ("blocks", "auto"):
+            h = ctypes.c_void_p()
+            check(L.aqs_fuse(arr, len(ops), n, max_qubits, ctypes.byref(h)))
+            cands.append(h)
+        if mode in ("tiled", "auto"):
+            h = ctypes.c_void_p()
+            check(L.aqs_fuse_tiled(arr, len(ops), n, tile_qubits, ctypes.byref(h)))
+            cands.append(h)
+        del keep
+        costs = []
+        for h in cands:
+            c = ctypes.c_double()
+            check(L.aqs_schedule_cost(h, ctypes.byref(c)))
+            costs.append(c.value)
+        best = min(range(len(cands)), key=lambda i: costs[i])
+        for i, h in enumerate(cands):
+            if i != best:
+                check(L.aqs_schedule_destroy(h))
+        self.h = cands[best]
+        self.mode = mode if mode != "auto" else ("blocks", "tiled")[best]
+        self._cost = costs[best]
+
+    def cost(self) -> float:
+        """Modelled cost in full-state sweeps."""
+        return self._cost
 
     def info(self):
         p, s = ctypes.c_size_t(), ctypes.c_size_t()

@@ -132,7 +132,8 @@ def random20(args, n=20, depth=40, shots=1000):
     st = hip.State(n, hip.C64)
     out = {"qubits": n, "depth": depth, "gates": len(lowered), "shots": shots}
     reps = max(3, args.steps)
-    for label, sch in (("plain", None), ("fused_k5", hip.Schedule(lowered, n, 5))):
+    for label, sch in (("plain", None), ("fused_k5", hip.Schedule(lowered, n, 5)),
+                       ("tiled", hip.Schedule(lowered, n, mode="tiled"))):
         def run_once():
             st.set_basis(0)
             if sch is None:
@@ -190,9 +191,9 @@ def qft30(args, n=30, seed=30):
     dt = time.perf_counter() - t0
     out["unfused"] = {"ms": round(1e3 * dt, 2), "gate_apps_per_s": round(len(lowered) / dt, 1),
                       "hbm_gbs": round(alg / dt / 1e9, 1), "infidelity": infid()}
-    for kmax in (5,):
+    for kmax in (5, "tiled"):
         t0 = time.perf_counter()
-        sch = hip.Schedule(lowered, n, kmax)
+        sch = hip.Schedule(lowered, n, kmax) if kmax != "tiled" else hip.Schedule(lowered, n, mode="tiled")
         fuse_s = time.perf_counter() - t0
         passes, _ = sch.info()
         st.gaussian(seed)
@@ -207,13 +208,14 @@ def qft30(args, n=30, seed=30):
         prof = st.profile_read()
         st.profile(False)
         mf_ms, mf_n = prof.get("MATK_MFMA", (0.0, 0))
-        out[f"fused_k{kmax}"] = {"passes": passes, "ms": round(1e3 * dt, 2),
+        out["tiled" if kmax == "tiled" else f"fused_k{kmax}"] = {"passes": passes, "ms": round(1e3 * dt, 2),
                                  "gate_apps_per_s": round(len(lowered) / dt, 1), "infidelity": infid(),
                                  "fuse_host_s": round(fuse_s, 3), "mfma_launches": mf_n,
                                  "mfma_avg_ms": round(mf_ms / max(1, mf_n), 4),
                                  "mfma_tflops": round(8 * 32 * (1 << n) / (mf_ms / max(1, mf_n) * 1e-3) / 1e12, 1)
                                  if mf_n else None,
-                                 "speedup_vs_unfused": round(out["unfused"]["ms"] / (1e3 * dt), 2)}
+                                 "speedup_vs_unfused": round(out["unfused"]["ms"] / (1e3 * dt), 2),
+                                 "profile_ms": {k: [round(v[0], 3), v[1]] for k, v in prof.items()}}
         sch.close()
     st.close()
     ref.close()
@@ -407,8 +409,11 @@ def run(args):
 
     # compile(): fused 5-qubit (and smaller) blocks on the same circuit
     fused = {}
-    for kmax in (4, 5):
-        sch = hip.Schedule(lowered, n, kmax)
+    for kmax in (4, 5, "tiled"):
+        if kmax == "tiled":
+            sch = hip.Schedule(lowered, n, mode="tiled")
+        else:
+            sch = hip.Schedule(lowered, n, kmax)
         passes, _ = sch.info()
         st.set_basis(0)
         st.apply_schedule(sch)
@@ -422,10 +427,11 @@ def run(args):
         fdt = (time.perf_counter() - t0) / reps
         ovf = st.inner(ref)
         inf_f = max(0.0, 1.0 - abs(ovf) ** 2 / st.norm2())
-        fused[f"k{kmax}"] = {"passes": passes, "ms_per_step": round(1e3 * fdt, 3),
-                             "gate_apps_per_s": round(n_gates / fdt, 1), "infidelity": inf_f,
-                             "speedup_vs_plain": round((dt / args.steps) / fdt, 3)}
-        log(f"fused k={kmax}: {fused[f'k{kmax}']}")
+        key = "tiled" if kmax == "tiled" else f"k{kmax}"
+        fused[key] = {"passes": passes, "ms_per_step": round(1e3 * fdt, 3),
+                      "gate_apps_per_s": round(n_gates / fdt, 1), "infidelity": inf_f,
+                      "speedup_vs_plain": round((dt / args.steps) / fdt, 3), "model_cost": round(sch.cost(), 2)}
+        log(f"fused {key}: {fused[key]}")
         sch.close()
     ref.close()
     st.close()

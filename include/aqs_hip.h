@@ -73,7 +73,10 @@ typedef enum aqs_op_kind {
     /* only in fused schedules: an in-place permutation of index bits that
      * restores the qubit order after swaps were turned into relabellings;
      * targets lists (up to 5 of) the qubits it moves.                      */
-    AQS_OP_PERMB = 8
+    AQS_OP_PERMB = 8,
+    /* only in tiled schedules (aqs_fuse_tiled): one sweep of the state in
+     * LDS-resident tiles that applies a run of gates; ntargets = 0.        */
+    AQS_OP_TILE = 9
 } aqs_op_kind;
 
 /* Control semantics: default = all control bits must be 1.  AQS_CTRL_ANY =
@@ -155,6 +158,20 @@ aqs_status aqs_autotune(aqs_state s, int reps, int* variant_by_qubit);
  * a dense unitary built in FP64; runs of diagonal ops become phase passes. */
 aqs_status aqs_fuse(const aqs_op* ops, size_t count, int nqubits, int max_qubits,
                     aqs_schedule* out);
+/* Cache-blocked compile: runs of one-qubit, controlled (X, CX, CH, CRX..,
+ * Toffoli, Or, CSwap) and diagonal ops become tile passes, each of which
+ * sweeps the state once in LDS-resident tiles of 2^tile_qubits amplitudes
+ * (tile_qubits 10..13, 0 = 13) and applies every gate of the run whose
+ * non-diagonal qubits fall in the tile; gates only move past gates they
+ * commute with.  Swaps become relabellings undone at the end; other ops
+ * (dense Gate / ControlGate blocks) stay plain passes.  States of fewer than
+ * 10 qubits get the aqs_fuse(.., 5, ..) schedule. */
+aqs_status aqs_fuse_tiled(const aqs_op* ops, size_t count, int nqubits, int tile_qubits, aqs_schedule* out);
+/* Modelled cost of running the schedule, in full-state sweeps: every pass
+ * counts 1 (a plain pass of a sparse gate its touched fraction), plus the
+ * in-tile gate work of tile passes and the MFMA work of dense blocks in
+ * sweep-time units.  Used to choose between aqs_fuse and aqs_fuse_tiled. */
+aqs_status aqs_schedule_cost(aqs_schedule sch, double* sweeps);
 aqs_status aqs_schedule_destroy(aqs_schedule sch);
 /* Number of fused passes and the ops each came from. */
 aqs_status aqs_schedule_info(aqs_schedule sch, size_t* passes, size_t* source_ops);
@@ -264,7 +281,11 @@ aqs_status aqs_server_destroy(aqs_server s);
 /* Per-kind kernel timing with HIP events on the state's stream.  While
  * enabled every gate kernel launch is bracketed by an event pair. */
 aqs_status aqs_profile_enable(aqs_state s, int enable);
-/* kinds: AQS_OP_* index 0..7 -> total ms and launch count since enable. */
+/* Slots (arrays of AQS_PROFILE_SLOTS): 0 state preparation and measurement,
+ * 1..6 the AQS_OP_* kind of a plain pass (MATK on the scalar kernel), 7 MATK
+ * on MFMA, 8 diagonal table passes, 9 bit-permutation passes, 10 tile
+ * passes.  Total ms and launch count since enable. */
+#define AQS_PROFILE_SLOTS 11
 aqs_status aqs_profile_read(aqs_state s, double* ms_by_kind, uint64_t* launches_by_kind);
 /* Event pair on the state's stream around arbitrary work. */
 aqs_status aqs_timer_start(aqs_state s);

@@ -737,6 +737,64 @@ static void test_compile_equivalence() {
     CHECK(cg.to_string() == "H,1,1:0,1;X,2,1:0,1,2;");
 }
 
+// Grover generators (src/quantum_algo.cpp:15-101; examples/grover_search.cpp:22-37).
+// After k iterations from |0..0> the marked index has probability
+// sin^2((2k+1) theta), sin theta = 2^(-n/2).
+static void test_grover() {
+    const uint32_t n = 4, marked = 5;
+    // qubit i carries bit i of marked_state: qubits 0 and 2 -> index 0b1010
+    const uint32_t marked_index = 0b1010;
+    QCircuit oracle = grover_oracle(n, marked);
+    for (uint32_t i = 0; i < (1u << n); ++i) {
+        QSimulator qs(n);
+        for (uint32_t q = 0; q < n; ++q) qs.qubit(q) = ((i >> (n - 1 - q)) & 1u) ? QState::one() : QState::zero();
+        qs.generate_statevector();
+        qs.simulate(oracle);
+        CHECK(fequal(qs.state(i).real(), i == marked_index ? -1.f : 1.f, 1e-6f));
+    }
+    const double theta = std::asin(std::pow(2.0, -0.5 * n));
+    for (uint32_t k = 1; k <= 4; ++k) {
+        QCircuit search = grover_search(n, oracle, k, "Oracle");
+        QCircuit fused = grover_search(n, oracle, k, "Oracle", true);
+        CHECK(search.representation() == fused.representation());
+        QSimulator a(n), b(n);
+        a.simulate(search);
+        b.simulate(fused);
+        const double expect = std::pow(std::sin((2 * k + 1) * theta), 2);
+        CHECK(std::fabs(a.state_probability(marked_index) - expect) < 1e-5);
+        CHECK(std::fabs(b.state_probability(marked_index) - expect) < 1e-5);
+        // grover_iteration is the same circuit without the H layer and barriers
+        QCircuit it(n);
+        for (uint32_t j = 0; j < n; ++j) it << H(j);
+        it << Gate(grover_iteration(n, oracle, k), 0);
+        QSimulator c(n);
+        c.simulate(it);
+        CHECK(near(a.statevector().host(), c.statevector().host(), 1e-6f));
+    }
+    // the reference benchmark's iteration count: floor(pi * sqrt(16) / 4) = 3
+    QSimulator qs(n);
+    qs.simulate(grover_search(n, oracle, 3));
+    auto profile = qs.profile_measure_all(1000);
+    uint32_t best = 0;
+    for (uint32_t i = 1; i < profile.size(); ++i)
+        if (profile[i] > profile[best]) best = i;
+    CHECK(best == marked_index);
+    bool threw = false;
+    try {
+        grover_oracle(n, 1u << n);
+    } catch (const std::invalid_argument&) {
+        threw = true;
+    }
+    CHECK(threw);
+    threw = false;
+    try {
+        grover_search(n + 1, oracle, 1);
+    } catch (const std::invalid_argument&) {
+        threw = true;
+    }
+    CHECK(threw);
+}
+
 int main(int argc, char** argv) {
     initialize(argc, argv);
     seed(20240229u);
@@ -754,6 +812,7 @@ int main(int argc, char** argv) {
     test_special_gates();
     test_errors();
     test_compile_equivalence();
+    test_grover();
     std::cout << "ALL PASSED" << std::endl;
     return 0;
 }

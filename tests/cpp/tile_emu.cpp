@@ -1,0 +1,226 @@
+// Test-only host emulation of compiled schedules, used by
+// tests/test_tiling.py to check the tile scheduler on the CPU.
+//
+// tile_emulate() builds a schedule with aqs_fuse_tiled (host code only) and
+// runs it on an FP64 host state, reading the tile passes exactly as the
+// device kernel (afquantumsim_amd/csrc/tile.hip) does: per tile, per
+// segment, per thread the 16 register amplitudes, with the op encodings
+// (register slots, thread-fixed positions, global bits) interpreted as the
+// kernel interprets them.  Plain passes and bit permutations are applied
+// directly.  The result is compared with the oracle by the Python test.
+#include <complex>
+#include <cstring>
+#include <vector>
+
+#include "schedule.h"
+
+using namespace aqs_hip;
+using cplx = std::complex<double>;
+
+namespace {
+
+void plain_apply(std::vector<cplx>& v, int n, const aqs_op& op) {
+    const uint64_t N = 1ull << n;
+    auto bit = [&](int q) { return 1ull << (n - 1 - q); };
+    uint64_t cmask = 0;
+    for (int j = 0; j < op.ncontrols; ++j) cmask |= bit(op.controls[j]);
+    const bool any = (op.flags & AQS_CTRL_ANY) != 0;
+    auto ok = [&](uint64_t i) { return !cmask || (any ? (i & cmask) != 0 : (i & cmask) == cmask); };
+    const cplx m00(op.m[0], op.m[1]), m01(op.m[2], op.m[3]), m10(op.m[4], op.m[5]), m11(op.m[6], op.m[7]);
+    switch (op.kind) {
+        case AQS_OP_MAT1:
+        case AQS_OP_DIAG1:
+        case AQS_OP_X: {
+            const uint64_t t = bit(op.targets[0]);
+            for (uint64_t i = 0; i < N; ++i) {
+                if ((i & t) || !ok(i)) continue;
+                const cplx a = v[i], b = v[i | t];
+                if (op.kind == AQS_OP_MAT1) {
+                    v[i] = m00 * a + m01 * b;
+                    v[i | t] = m10 * a + m11 * b;
+                } else if (op.kind == AQS_OP_DIAG1) {
+                    v[i] = m00 * a;
+                    v[i | t] = m11 * b;
+                } else {
+                    v[i] = b;
+                    v[i | t] = a;
+                }
+            }
+            break;
+        }
+        case AQS_OP_SWAP: {
+            const uint64_t a = bit(op.targets[0]), b = bit(op.targets[1]);
+            for (uint64_t i = 0; i < N; ++i)
+                if ((i & a) && !(i & b) && ok(i)) std::swap(v[i], v[i ^ a ^ b]);
+            break;
+        }
+        case AQS_OP_PHASE: {
+            uint64_t mask = cmask;
+            for (int j = 0; j < op.ntargets; ++j) mask |= bit(op.targets[j]);
+            for (uint64_t i = 0; i < N; ++i)
+                if ((i & mask) == mask) v[i] *= m00;
+            break;
+        }
+        case AQS_OP_MATK: {
+            const int k = op.ntargets;
+            const uint64_t K = 1ull << k;
+            std::vector<uint64_t> off(K);
+            uint64_t tmask = 0;
+            for (uint64_t r = 0; r < K; ++r)
+                for (int j = 0; j < k; ++j)
+                    if ((r >> (k - 1 - j)) & 1) off[r] |= bit(op.targets[j]);
+            for (int j = 0; j < k; ++j) tmask |= bit(op.targets[j]);
+            std::vector<cplx> x(K);
+            for (uint64_t i = 0; i < N; ++i) {
+                if ((i & tmask) || !ok(i)) continue;
+                for (uint64_t c = 0; c < K; ++c) x[c] = v[i | off[c]];
+                for (uint64_t r = 0; r < K; ++r) {
+                    cplx acc = 0;
+                    for (uint64_t c = 0; c < K; ++c) acc += cplx(op.u[2 * (r * K + c)], op.u[2 * (r * K + c) + 1]) * x[c];
+                    v[i | off[r]] = acc;
+                }
+            }
+            break;
+        }
+        default:
+            break;
+    }
+}
+
+void permute_bits(std::vector<cplx>& v, int n, const std::vector<int>& beta) {
+    std::vector<cplx> out(v.size());
+    for (uint64_t i = 0; i < v.size(); ++i) {
+        uint64_t j = 0;
+        for (int b = 0; b < n; ++b)
+            if ((i >> b) & 1) j |= 1ull << beta[b];
+        out[j] = v[i];
+    }
+    v.swap(out);
+}
+
+bool ctrl_ok(int j, uint32_t creg, bool any, bool g_any, bool t_all, bool t_any) {
+    if (any) return g_any || t_any || ((uint32_t)j & creg) != 0u;
+    return t_all && (((uint32_t)j & creg) == creg);
+}
+
+void tile_apply(std::vector<cplx>& v, int n, const TilePass& tp) {
+    const int m = tp.m, W = m - kTileRegBits;
+    const uint64_t tiles = (1ull << n) >> m;
+    std::vector<cplx> tile((size_t)1 << m);
+    for (uint64_t t = 0; t < tiles; ++t) {
+        uint64_t base = t;
+        for (int p = 0; p < m; ++p) {
+            const int q = tp.local[p];
+            base = ((base >> q) << (q + 1)) | (base & ((1ull << q) - 1ull));
+        }
+        auto gidx = [&](uint32_t l) {
+            uint64_t g = base;
+            for (int p = 0; p < m; ++p)
+                if ((l >> p) & 1) g |= 1ull << tp.local[p];
+            return g;
+        };
+        for (uint32_t l = 0; l < tile.size(); ++l) tile[l] = v[gidx(l)];
+        for (const auto& S : tp.segs) {
+            for (uint32_t tid = 0; tid < (1u << W); ++tid) {
+                uint32_t lthr = 0;
+                for (int b = 0; b < W; ++b) lthr |= ((tid >> b) & 1u) << S.thr[b];
+                auto addr = [&](int j) {
+                    uint32_t l = lthr;
+                    for (int r = 0; r < kTileRegBits; ++r)
+                        if ((j >> r) & 1) l |= 1u << S.reg[r];
+                    return l;
+                };
+                cplx x[16];
+                for (int j = 0; j < 16; ++j) x[j] = tile[addr(j)];
+                for (uint32_t o = 0; o < S.op_count; ++o) {
+                    const TileOpDev& op = tp.ops[S.op_begin + o];
+                    if (op.code >= kTileCodePhases) {
+                        // every term multiplies the amplitudes whose register
+                        // index holds its register mask (both device forms
+                        // compute this product)
+                        for (uint32_t k = 0; k < op.term_count; ++k) {
+                            const size_t e = op.term_begin + k;
+                            const TileTermDev& t = tp.terms[e];
+                            if ((base & t.m_glb) != t.m_glb || (lthr & t.m_thr) != t.m_thr) continue;
+                            for (int j = 0; j < 16; ++j)
+                                if (((uint32_t)j & t.m_reg) == t.m_reg) x[j] *= cplx(tp.z128[2 * e], tp.z128[2 * e + 1]);
+                        }
+                        continue;
+                    }
+                    const bool any = op.any != 0;
+                    const bool g_all = (base & op.c_glb) == op.c_glb, g_any = (base & op.c_glb) != 0;
+                    if (!any && !g_all) continue;
+                    const bool t_all = (lthr & op.c_thr) == op.c_thr, t_any = (lthr & op.c_thr) != 0;
+                    // code = (XCH ? 12 : 0) + 4 * form + slot; the form only
+                    // selects how the kernel evaluates the same controls
+                    const int slot = op.code % 4;
+                    const bool mat = op.code < 12;
+                    const int form = (op.code % 12) / 4;
+                    if (form == 0 && (op.c_reg || op.c_thr || any)) throw 1;
+                    if (form == 1 && (op.c_thr || any || !op.c_reg)) throw 1;
+                    for (int j = 0; j < 16; ++j) {
+                        if ((j >> slot) & 1) continue;
+                        if (!ctrl_ok(j, op.c_reg, any, g_any, t_all, t_any)) continue;
+                        const int k = j | (1 << slot);
+                        if (mat) {
+                            const double* m = &tp.m128[8 * (size_t)(S.op_begin + o)];
+                            const cplx p = x[j], q = x[k];
+                            x[j] = cplx(m[0], m[1]) * p + cplx(m[2], m[3]) * q;
+                            x[k] = cplx(m[4], m[5]) * p + cplx(m[6], m[7]) * q;
+                        } else {
+                            std::swap(x[j], x[k]);
+                        }
+                    }
+                }
+                for (int j = 0; j < 16; ++j) tile[addr(j)] = x[j];
+            }
+        }
+        for (uint32_t l = 0; l < tile.size(); ++l) v[gidx(l)] = tile[l];
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// Builds the tiled schedule of ops and applies it to `state` (2^n complex
+// FP64, interleaved) on the host.  Returns the aqs_status of the build;
+// counts[0..2] = passes, tile passes, segments.
+int tile_emulate(const aqs_op* ops, size_t count, int n, int tile_qubits, double* state, size_t* counts) {
+    aqs_schedule sch = nullptr;
+    const int st = aqs_fuse_tiled(ops, count, n, tile_qubits, &sch);
+    if (st != AQS_OK) return st;
+    std::vector<cplx> v((size_t)1 << n);
+    std::memcpy(v.data(), state, v.size() * sizeof(cplx));
+    size_t tiles = 0, segs = 0;
+    for (size_t p = 0; p < sch->passes.size(); ++p) {
+        const aqs_op& op = sch->passes[p];
+        if (sch->tile_index[p] >= 0) {
+            const TilePass& tp = sch->tiles[sch->tile_index[p]];
+            try {
+                tile_apply(v, n, tp);
+            } catch (int) {
+                aqs_schedule_destroy(sch);
+                return -2;  // dispatch form inconsistent with the controls
+            }
+            ++tiles;
+            segs += tp.segs.size();
+        } else if (sch->diag_index[p] >= 0) {
+            aqs_schedule_destroy(sch);
+            return -1;  // diagonal tables are not emulated here
+        } else if (op.kind == AQS_OP_PERMB) {
+            for (auto& pi : sch->perm_index)
+                if (pi.first == (int)p) permute_bits(v, n, sch->perms[pi.second]);
+        } else {
+            plain_apply(v, n, op);
+        }
+    }
+    std::memcpy(state, v.data(), v.size() * sizeof(cplx));
+    counts[0] = sch->passes.size();
+    counts[1] = tiles;
+    counts[2] = segs;
+    aqs_schedule_destroy(sch);
+    return AQS_OK;
+}
+
+}  // extern "C"

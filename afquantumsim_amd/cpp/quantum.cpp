@@ -356,7 +356,12 @@ void QCircuit::compile() {
     ops.reserve(rec.size());
     for (auto& r : rec) ops.push_back(r.op);
     auto seg = std::make_shared<detail::Segment>();
-    if (!ops.empty()) check(aqs_fuse(ops.data(), ops.size(), (int)qubits_, fusion_width_, &seg->sch));
+    if (!ops.empty()) {
+        if (fusion_width_ == 0)
+            check(aqs_fuse_auto(ops.data(), ops.size(), (int)qubits_, &seg->sch));
+        else
+            check(aqs_fuse(ops.data(), ops.size(), (int)qubits_, fusion_width_, &seg->sch));
+    }
     if (seg->sch) compiled_.push_back(seg);
     cached_index_ = gate_list_.size();
     matrix_.reset();

@@ -71,7 +71,7 @@ size_t state_footprint(int n) {
 
 struct aqs_server_s {
     size_t budget = 0;
-    int fuse_qubits = 5;
+    int fuse_qubits = 0;  // 0: aqs_fuse_auto
     double starvation_s = 0.25;
     int device = 0;
 
@@ -180,7 +180,10 @@ void run_request(aqs_server s, Request& r, hipStream_t stream) {
         if (rc != AQS_OK) fail(rc, aqs_last_error());
     };
     try {
-        ck(aqs_fuse(r.ops.data(), r.ops.size(), r.n, s->fuse_qubits, &sch));
+        if (s->fuse_qubits == 0)
+            ck(aqs_fuse_auto(r.ops.data(), r.ops.size(), r.n, &sch));
+        else
+            ck(aqs_fuse(r.ops.data(), r.ops.size(), r.n, s->fuse_qubits, &sch));
         ck(aqs_schedule_info(sch, &r.passes, nullptr));
         st = acquire_state(s, r.n);
         ck(aqs_state_set_stream(st, stream));
@@ -243,7 +246,7 @@ aqs_status aqs_server_create(size_t hbm_budget_bytes, int workers, int fuse_qubi
         if (!out) fail(AQS_ERR_INVALID_ARGUMENT, "null output handle");
         *out = nullptr;
         if (workers < 1 || workers > 16) fail(AQS_ERR_INVALID_ARGUMENT, "workers must be 1..16");
-        if (fuse_qubits < 1 || fuse_qubits > 5) fail(AQS_ERR_INVALID_ARGUMENT, "fuse_qubits must be 1..5");
+        if (fuse_qubits < 0 || fuse_qubits > 5) fail(AQS_ERR_INVALID_ARGUMENT, "fuse_qubits must be 0..5");
         auto* s = new aqs_server_s();
         s->budget = hbm_budget_bytes;
         s->fuse_qubits = fuse_qubits;

@@ -40,6 +40,8 @@ using namespace aqs_hip;
 
 namespace {
 
+constexpr int kTileAutoMinQubits = 22;
+
 // A phase term: multiply by z where every listed index bit is 1.
 struct Term {
     std::vector<int> bits;
@@ -436,30 +438,35 @@ class Tiler {
     }
 };
 
-// Relative cost of one pass in full-state sweeps (read + write of every
-// amplitude at the streaming rate).  In-tile work: a 2x2 product costs
-// about 1/13 of a sweep of the same state, an exchange or diagonal factor
-// about 1/30, a segment's LDS round trip about 1/40; a tile pass costs the
-// larger of its sweep and its work.
+// Modelled cost of one pass in full-state sweeps (read + write of every
+// amplitude at the streaming rate), fitted to MI355X timings of brickwork,
+// random-layer and QFT circuits at 22..28 qubits: a tile pass costs its
+// sweep (1.1) plus its in-register work -- a 2x2 product 0.107 of a sweep,
+// an exchange 0.046, a phase run 0.11 plus 0.011 per term; an MFMA block
+// 1.17-1.2, a diagonal table 1.25, a bit permutation 3.6, a plain
+// one-qubit pass 1, a sparse pass its touched fraction.
 double pass_cost(const aqs_schedule_s* sch, size_t p) {
     const aqs_op& op = sch->passes[p];
     if (sch->tile_index[p] >= 0) {
         const TilePass& t = sch->tiles[sch->tile_index[p]];
-        double work = 0.025 * (double)t.segs.size();
+        double c = 1.1;
         for (const auto& o : t.ops) {
             if (o.code < 12)
-                work += 0.075;
+                c += 0.107;
             else if (o.code < kTileCodePhases)
-                work += 0.02;
+                c += 0.046;
             else
-                work += o.code == kTileCodePhases + 1 ? 0.03 + 0.004 * o.term_count : 0.004 * o.term_count + 0.01;
+                c += 0.11 + 0.011 * o.term_count;
         }
-        return std::max(1.0, work) + 0.05;
+        return c;
     }
+    if (sch->diag_index[p] >= 0) return 1.25;
     switch (op.kind) {
+        case AQS_OP_PERMB:
+            return 3.6;
         case AQS_OP_MATK:
             if (op.ncontrols == 0 && op.ntargets == 5) return 1.2;
-            if (op.ncontrols == 0 && op.ntargets == 4) return 1.15;
+            if (op.ncontrols == 0 && op.ntargets == 4) return 1.17;
             return 1.0 + 0.05 * (double)((size_t)1 << op.ntargets);
         case AQS_OP_PHASE: {
             const int bits = op.ntargets + op.ncontrols;
@@ -470,6 +477,12 @@ double pass_cost(const aqs_schedule_s* sch, size_t p) {
         default:
             return op.ncontrols ? 0.6 : 1.0;
     }
+}
+
+double schedule_cost(const aqs_schedule_s* sch) {
+    double c = 0;
+    for (size_t p = 0; p < sch->passes.size(); ++p) c += pass_cost(sch, p);
+    return c;
 }
 
 }  // namespace
@@ -506,10 +519,29 @@ aqs_status aqs_fuse_tiled(const aqs_op* ops, size_t count, int nqubits, int tile
 aqs_status aqs_schedule_cost(aqs_schedule sch, double* sweeps) {
     return guard([&] {
         if (!sch) fail(AQS_ERR_INVALID_ARGUMENT, "null schedule");
-        double c = 0;
-        for (size_t p = 0; p < sch->passes.size(); ++p) c += pass_cost(sch, p);
-        if (sweeps) *sweeps = c;
+        if (sweeps) *sweeps = schedule_cost(sch);
     });
+}
+
+aqs_status aqs_fuse_auto(const aqs_op* ops, size_t count, int nqubits, aqs_schedule* out) {
+    aqs_schedule blocks = nullptr, tiled = nullptr;
+    aqs_status st = aqs_fuse(ops, count, nqubits, 5, &blocks);
+    if (st != AQS_OK) return st;
+    // below 2^22 amplitudes the state stays in the caches and launches of
+    // the block schedule are cheaper than in-tile work
+    if (nqubits < kTileAutoMinQubits) {
+        *out = blocks;
+        return AQS_OK;
+    }
+    st = aqs_fuse_tiled(ops, count, nqubits, 0, &tiled);
+    if (st != AQS_OK) {
+        aqs_schedule_destroy(blocks);
+        return st;
+    }
+    const bool keep_tiled = schedule_cost(tiled) < schedule_cost(blocks);
+    aqs_schedule_destroy(keep_tiled ? blocks : tiled);
+    *out = keep_tiled ? tiled : blocks;
+    return AQS_OK;
 }
 
 }  // extern "C"

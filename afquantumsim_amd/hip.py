@@ -78,7 +78,7 @@ EXPORTS = [
     "aqs_state_copy", "aqs_state_qubits", "aqs_state_precision", "aqs_state_device_ptr",
     "aqs_state_set_stream", "aqs_state_sync", "aqs_state_set_basis", "aqs_state_product",
     "aqs_state_gaussian", "aqs_state_upload", "aqs_state_download", "aqs_apply", "aqs_fuse",
-    "aqs_fuse_tiled", "aqs_schedule_cost", "aqs_schedule_destroy", "aqs_schedule_info", "aqs_schedule_get", "aqs_apply_schedule",
+    "aqs_fuse_tiled", "aqs_fuse_auto", "aqs_schedule_cost", "aqs_schedule_destroy", "aqs_schedule_info", "aqs_schedule_get", "aqs_apply_schedule",
     "aqs_norm2", "aqs_normalize", "aqs_prob1", "aqs_prob1_all", "aqs_expect_sum_z",
     "aqs_probabilities", "aqs_amplitude", "aqs_sample", "aqs_measure_qubit", "aqs_collapse_to",
     "aqs_inner", "aqs_profile_enable", "aqs_profile_read", "aqs_timer_start", "aqs_timer_stop",
@@ -134,6 +134,7 @@ def load(path: str = LIB_PATH):
         "aqs_fuse": [ctypes.POINTER(Op), sz, i32, i32, ctypes.POINTER(vp)],
         "aqs_fuse_tiled": [ctypes.POINTER(Op), sz, i32, i32, ctypes.POINTER(vp)],
         "aqs_schedule_cost": [vp, dp],
+        "aqs_fuse_auto": [ctypes.POINTER(Op), sz, i32, ctypes.POINTER(vp)],
         "aqs_schedule_destroy": [vp],
         "aqs_schedule_info": [vp, ctypes.POINTER(sz), ctypes.POINTER(sz)],
         "aqs_schedule_get": [vp, sz, ctypes.POINTER(Op)],
@@ -352,7 +353,8 @@ class Schedule:
     mode "blocks": dense blocks of <= max_qubits qubits (aqs_fuse);
     mode "tiled":  cache-blocked tile passes of 2^tile_qubits amplitudes
                    (aqs_fuse_tiled, tile_qubits 0 = 13);
-    mode "auto":   both, keeping the one of lower modelled cost.
+    mode "auto":   aqs_fuse_auto (blocks below 22 qubits, else the lower
+                   modelled cost of the two).
     """
 
     def __init__(self, ops, n: int, max_qubits: int = 5, mode: str = "blocks", tile_qubits: int = 0):
@@ -362,28 +364,21 @@ class Schedule:
         self.h = None
         if mode not in ("blocks", "tiled", "auto"):
             raise ValueError(f"unknown schedule mode {mode!r}")
-        cands = []
-        if mode in ("blocks", "auto"):
-            h = ctypes.c_void_p()
+        h = ctypes.c_void_p()
+        if mode == "blocks":
             check(L.aqs_fuse(arr, len(ops), n, max_qubits, ctypes.byref(h)))
-            cands.append(h)
-        if mode in ("tiled", "auto"):
-            h = ctypes.c_void_p()
+        elif mode == "tiled":
             check(L.aqs_fuse_tiled(arr, len(ops), n, tile_qubits, ctypes.byref(h)))
-            cands.append(h)
+        else:
+            check(L.aqs_fuse_auto(arr, len(ops), n, ctypes.byref(h)))
         del keep
-        costs = []
-        for h in cands:
-            c = ctypes.c_double()
-            check(L.aqs_schedule_cost(h, ctypes.byref(c)))
-            costs.append(c.value)
-        best = min(range(len(cands)), key=lambda i: costs[i])
-        for i, h in enumerate(cands):
-            if i != best:
-                check(L.aqs_schedule_destroy(h))
-        self.h = cands[best]
-        self.mode = mode if mode != "auto" else ("blocks", "tiled")[best]
-        self._cost = costs[best]
+        self.h = h
+        c = ctypes.c_double()
+        check(L.aqs_schedule_cost(h, ctypes.byref(c)))
+        self._cost = c.value
+        self.mode = mode
+        if mode == "auto":
+            self.mode = "tiled" if any(k == OP_TILE for k, _ in self.passes()) else "blocks"
 
     def cost(self) -> float:
         """Modelled cost in full-state sweeps."""
@@ -473,7 +468,7 @@ class Batch:
 class Server:
     """Continuous-batching request server on one device."""
 
-    def __init__(self, budget_bytes: int, workers: int = 4, fuse_qubits: int = 5):
+    def __init__(self, budget_bytes: int, workers: int = 4, fuse_qubits: int = 0):
         L = load()
         h = ctypes.c_void_p()
         check(L.aqs_server_create(int(budget_bytes), workers, fuse_qubits, ctypes.byref(h)))

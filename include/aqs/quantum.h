@@ -174,7 +174,9 @@ class QCircuit {
     std::string& representation() noexcept { return representation_; }
     const std::string& representation() const noexcept { return representation_; }
 
-    /* Fuses the not-yet-compiled gates into <= 5-qubit blocks. */
+    /* Compiles the not-yet-compiled gates into device passes: <= 5-qubit
+     * dense blocks, or from 22 qubits up LDS-resident tile passes when the
+     * library's cost model rates them cheaper (aqs_fuse_auto). */
     void compile();
     void clear();
     void clear_cache();
@@ -184,7 +186,8 @@ class QCircuit {
 
     /* Number of fused device passes the compiled part runs. */
     size_t compiled_passes() const;
-    /* Maximum qubits per fused block used by compile() (1..5, default 5). */
+    /* compile() with dense blocks of at most k (1..5) qubits only; 0 (the
+     * default) lets the library choose between blocks and tile passes. */
     void set_fusion_width(int k) { fusion_width_ = k; }
 
     friend bool operator==(const QCircuit& lhs, const QCircuit& rhs);
@@ -194,7 +197,7 @@ class QCircuit {
     std::string representation_;
     uint32_t qubits_ = 0;
     std::size_t cached_index_ = 0;
-    int fusion_width_ = 5;
+    int fusion_width_ = 0;
     std::vector<std::shared_ptr<detail::Segment>> compiled_;
     detail::Sink* sink_ = nullptr;
     mutable std::shared_ptr<CircuitMatrix> matrix_;

@@ -172,6 +172,9 @@ aqs_status aqs_fuse_tiled(const aqs_op* ops, size_t count, int nqubits, int tile
  * in-tile gate work of tile passes and the MFMA work of dense blocks in
  * sweep-time units.  Used to choose between aqs_fuse and aqs_fuse_tiled. */
 aqs_status aqs_schedule_cost(aqs_schedule sch, double* sweeps);
+/* compile()'s default: the aqs_fuse(.., 5, ..) schedule, or from 22 qubits
+ * up the aqs_fuse_tiled(.., 0, ..) one when its modelled cost is lower. */
+aqs_status aqs_fuse_auto(const aqs_op* ops, size_t count, int nqubits, aqs_schedule* out);
 aqs_status aqs_schedule_destroy(aqs_schedule sch);
 /* Number of fused passes and the ops each came from. */
 aqs_status aqs_schedule_info(aqs_schedule sch, size_t* passes, size_t* source_ops);
@@ -259,6 +262,7 @@ typedef struct aqs_server_stats_t {
     uint64_t resident_bytes, peak_resident_bytes, budget_bytes;
 } aqs_server_stats_t;
 
+/* fuse_qubits: 1..5 = aqs_fuse blocks of that width, 0 = aqs_fuse_auto. */
 aqs_status aqs_server_create(size_t hbm_budget_bytes, int workers, int fuse_qubits, aqs_server* out);
 /* A request waiting longer than this blocks later requests from jumping
  * ahead of it (default 0.25 s). */

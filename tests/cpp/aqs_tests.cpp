@@ -726,6 +726,22 @@ static void test_compile_equivalence() {
         sb.simulate(b);
         CHECK(near(sa.statevector().host(), sb.statevector().host(), 2e-6f));
     }
+    {
+        // from 22 qubits compile() may choose LDS-resident tile passes
+        const uint32_t n = 22;
+        QCircuit a(n), b(n);
+        for (QCircuit* qc : {&a, &b})
+            for (uint32_t layer = 0; layer < 6; ++layer) {
+                for (uint32_t q = 0; q < n; ++q) *qc << U3(q, 0.3f + 0.1f * q, 0.2f * layer, -0.7f + 0.05f * q);
+                for (uint32_t q = layer % 2; q + 1 < n; q += 2) *qc << CZ(q, q + 1);
+            }
+        b.compile();
+        CHECK(b.compiled_passes() * 4 < b.gate_list().size());
+        QSimulator sa(n), sb(n);
+        sa.simulate(a);
+        sb.simulate(b);
+        CHECK(near(sa.statevector().host(), sb.statevector().host(), 1e-5f));
+    }
     QCircuit r(3);
     r << H(0) << CX(0, 1) << Phase(2, pi / 4) << CPhase(1, 2, -pi / 2) << CSwap(0, 1, 2);
     CHECK(r.representation() == "H,0,1:0;X,1,1:0,1;T,0,1:2;S†,1,1:1,2;Swap,1,2:0,12;");

@@ -120,11 +120,3 @@ def test_tiled_matches_plain_at_24_qubits():
     g = got.astype(np.complex128)
     assert abs(np.vdot(g, g).real - 1.0) < 1e-5
 
-
-def test_auto_mode_picks_lower_cost():
-    n = 20
-    lowered = ops.lower_all(circuits.brickwork(n, 6, seed=1))
-    a = hip.Schedule(lowered, n, mode="blocks")
-    b = hip.Schedule(lowered, n, mode="tiled")
-    c = hip.Schedule(lowered, n, mode="auto")
-    assert c.cost() == min(a.cost(), b.cost())

@@ -160,3 +160,16 @@ def test_tiled_cancelling_phases(emu):
     got, counts = emulate(emu, n, sv, g)
     assert counts[1] == 1
     np.testing.assert_allclose(got, oracle.run(sv, n, g), atol=1e-12)
+
+
+def test_auto_mode():
+    """aqs_fuse_auto: blocks below 22 qubits, else the lower modelled cost."""
+    small = ops.lower_all(circuits.brickwork(20, 6, seed=1))
+    assert hip.Schedule(small, 20, mode="auto").mode == "blocks"
+    for n, gates in ((24, circuits.brickwork(24, 8, seed=1)), (24, circuits.fourier_transform(24, True))):
+        lowered = ops.lower_all(gates)
+        a = hip.Schedule(lowered, n, mode="blocks")
+        b = hip.Schedule(lowered, n, mode="tiled")
+        c = hip.Schedule(lowered, n, mode="auto")
+        assert c.cost() == min(a.cost(), b.cost())
+        assert c.mode == ("tiled" if b.cost() < a.cost() else "blocks")

@@ -47,31 +47,71 @@ __device__ __forceinline__ uint64_t permute_bits(uint64_t v, const int* map, int
     return r;
 }
 
+// Tile-local tables, built once per workgroup: index offsets of tile bits
+// 5.. and the permuted position of every tile element.
+constexpr int kMaxTile = 1 << 10;
+
 template <typename T2>
 __global__ __launch_bounds__(kBlock) void k_bitperm(PermArgs a) {
-    extern __shared__ unsigned char smem[];
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     const int tile = 1 << a.nt;
     T2* A = reinterpret_cast<T2*>(smem);
     T2* B = A + tile;
+    __shared__ uint64_t hi_off[32];
+    __shared__ uint16_t pmap[kMaxTile];
+    // tile bits 0..min(nt, 5)-1 are index bits 0..4 (all of them when nt < 5)
+    const int hi = a.nt > 5 ? a.nt - 5 : 0;
+    for (int v = threadIdx.x; v < (1 << hi); v += kBlock) {
+        uint64_t o = 0;
+        for (int j = 0; j < hi; ++j)
+            if ((v >> j) & 1) o |= 1ull << a.tpos[5 + j];
+        hi_off[v] = o;
+    }
+    for (int t = threadIdx.x; t < tile; t += kBlock) pmap[t] = (uint16_t)permute_bits((uint64_t)t, a.tmap, a.nt);
+    __syncthreads();
     T2* sv = reinterpret_cast<T2*>(a.sv);
     for (uint64_t o = blockIdx.x; o < a.outer; o += gridDim.x) {
         const uint64_t o2 = permute_bits(o, a.rmap, a.nr);
         if (o2 < o) continue;  // the partner tile's workgroup does this pair
         const uint64_t b1 = deposit(o, a.rpos, a.nr), b2 = deposit(o2, a.rpos, a.nr);
-        for (int t = threadIdx.x; t < tile; t += kBlock) {
-            const uint64_t off = deposit((uint64_t)t, a.tpos, a.nt);
-            const int pt = (int)permute_bits((uint64_t)t, a.tmap, a.nt);
-            A[pt] = sv[b1 | off];
-            if (o2 != o) B[pt] = sv[b2 | off];
-        }
-        __syncthreads();
-        for (int t = threadIdx.x; t < tile; t += kBlock) {
-            const uint64_t off = deposit((uint64_t)t, a.tpos, a.nt);
-            if (o2 != o) {
-                sv[b2 | off] = A[t];
-                sv[b1 | off] = B[t];
-            } else {
-                sv[b1 | off] = A[t];
+        const bool pair = o2 != o;
+        if constexpr (sizeof(T2) == 8) {
+            // complex64: amplitude pairs (16 bytes) along the contiguous bits
+            float4* sv4 = reinterpret_cast<float4*>(a.sv);
+            for (int f = threadIdx.x; f < tile / 2; f += kBlock) {
+                const int t = 2 * f;
+                const uint64_t off = (uint64_t)(t & 31) | hi_off[t >> 5];
+                const float4 x = ld_nt(&sv4[(b1 | off) >> 1]);
+                A[pmap[t]] = make_float2(x.x, x.y);
+                A[pmap[t + 1]] = make_float2(x.z, x.w);
+                if (pair) {
+                    const float4 y = ld_nt(&sv4[(b2 | off) >> 1]);
+                    B[pmap[t]] = make_float2(y.x, y.y);
+                    B[pmap[t + 1]] = make_float2(y.z, y.w);
+                }
+            }
+            __syncthreads();
+            for (int f = threadIdx.x; f < tile / 2; f += kBlock) {
+                const int t = 2 * f;
+                const uint64_t off = (uint64_t)(t & 31) | hi_off[t >> 5];
+                const float2 p = A[t], q = A[t + 1];
+                st_nt(make_float4(p.x, p.y, q.x, q.y), &sv4[((pair ? b2 : b1) | off) >> 1]);
+                if (pair) {
+                    const float2 u = B[t], v = B[t + 1];
+                    st_nt(make_float4(u.x, u.y, v.x, v.y), &sv4[(b1 | off) >> 1]);
+                }
+            }
+        } else {
+            for (int t = threadIdx.x; t < tile; t += kBlock) {
+                const uint64_t off = (uint64_t)(t & 31) | hi_off[t >> 5];
+                A[pmap[t]] = sv[b1 | off];
+                if (pair) B[pmap[t]] = sv[b2 | off];
+            }
+            __syncthreads();
+            for (int t = threadIdx.x; t < tile; t += kBlock) {
+                const uint64_t off = (uint64_t)(t & 31) | hi_off[t >> 5];
+                sv[(pair ? b2 : b1) | off] = A[t];
+                if (pair) sv[b1 | off] = B[t];
             }
         }
         __syncthreads();
@@ -108,6 +148,7 @@ void apply_bit_involution(aqs_state s, const std::vector<int>& beta) {
         a.rmap[j] = (int)(std::find(R.begin(), R.end(), beta[R[j]]) - R.begin());
     }
     a.outer = 1ull << a.nr;
+    if (a.nt < 1 || a.nt > 10) fail(AQS_ERR_INVALID_ARGUMENT, "bit permutation tile out of range");
     const uint64_t blocks = std::min<uint64_t>(a.outer, 16384);
     const size_t lds = 2 * ((size_t)1 << a.nt) * s->amp_bytes();
     ProfScope ps(s, kSlotPermute);

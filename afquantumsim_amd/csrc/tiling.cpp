@@ -443,7 +443,7 @@ class Tiler {
 // random-layer and QFT circuits at 22..28 qubits: a tile pass costs its
 // sweep (1.1) plus its in-register work -- a 2x2 product 0.107 of a sweep,
 // an exchange 0.046, a phase run 0.11 plus 0.011 per term; an MFMA block
-// 1.17-1.2, a diagonal table 1.25, a bit permutation 3.6, a plain
+// 1.17-1.2, a diagonal table 1.25, a bit permutation 1.8, a plain
 // one-qubit pass 1, a sparse pass its touched fraction.
 double pass_cost(const aqs_schedule_s* sch, size_t p) {
     const aqs_op& op = sch->passes[p];
@@ -463,7 +463,7 @@ double pass_cost(const aqs_schedule_s* sch, size_t p) {
     if (sch->diag_index[p] >= 0) return 1.25;
     switch (op.kind) {
         case AQS_OP_PERMB:
-            return 3.6;
+            return 1.8;
         case AQS_OP_MATK:
             if (op.ncontrols == 0 && op.ntargets == 5) return 1.2;
             if (op.ncontrols == 0 && op.ntargets == 4) return 1.17;

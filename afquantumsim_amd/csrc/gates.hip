@@ -323,6 +323,51 @@ __global__ __launch_bounds__(kBlock) void k_phase_v4(PhaseArgs<float4, float> a)
     }
 }
 
+// Phase whose required bits include some of bits 0..3: the amplitudes it
+// changes do not fill whole 128-byte lines, so it sweeps every line that
+// holds one (bits >= 4 fixed by insertion), writes whole lines back and
+// tests the low required bits per amplitude.
+struct PhaseLineArgs {
+    float4* sv;
+    uint64_t units;
+    Fixed fixed;       // required bits >= 4, as float4-index bits
+    uint64_t setmask;  // the same bits set (float4 index)
+    uint32_t lowmask;  // required bits 0..3 (amplitude index)
+    float pr, pi;
+};
+
+template <int U, typename IT>
+__global__ __launch_bounds__(kBlock) void k_phase_lines(PhaseLineArgs a) {
+    const IT units = (IT)a.units, setmask = (IT)a.setmask;
+    const IT base = (IT)blockIdx.x * (IT)(kBlock * U) + (IT)threadIdx.x;
+    float4 v[U];
+    IT idx[U];
+    bool ok[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+        const IT u = base + (IT)j * kBlock;
+        ok[j] = u < units;
+        idx[j] = insert_zeros_t<IT>(u, a.fixed) | setmask;
+        if (ok[j]) v[j] = ld_nt(&a.sv[idx[j]]);
+    }
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+        if (!ok[j]) continue;
+        const uint32_t i0 = 2u * (uint32_t)(idx[j] & 7u);
+        const bool c0 = (i0 & a.lowmask) == a.lowmask, c1 = ((i0 | 1u) & a.lowmask) == a.lowmask;
+        float4 x = v[j], y = x;
+        if (c0) {
+            y.x = a.pr * x.x - a.pi * x.y;
+            y.y = a.pr * x.y + a.pi * x.x;
+        }
+        if (c1) {
+            y.z = a.pr * x.z - a.pi * x.w;
+            y.w = a.pr * x.w + a.pi * x.z;
+        }
+        st_nt(y, &a.sv[idx[j]]);
+    }
+}
+
 // DIAG1 complex64 float4 path (controls >= 1; any target bit).
 struct Diag1V4Args {
     float4* sv;
@@ -624,7 +669,30 @@ void apply_c64_fast_or_generic(aqs_state s, const aqs_op& op) {
                 mask |= m;
                 fp.push_back(qpos(n, op.targets[j]));
             }
-            if (all_ge1(fp) && n >= 2) {
+            uint32_t lowmask = 0;
+            std::vector<int> high;
+            for (int p : fp) {
+                if (p < 4)
+                    lowmask |= 1u << p;
+                else
+                    high.push_back(p - 1);
+            }
+            if (lowmask && n >= 4) {
+                PhaseLineArgs a{};
+                a.sv = (float4*)s->amps;
+                a.fixed = make_fixed(high);
+                a.setmask = (mask >> 1) & ~(uint64_t)7;
+                a.lowmask = lowmask;
+                a.units = s->N >> (1 + high.size());
+                a.pr = mf[0];
+                a.pi = mf[1];
+                if (s->N / 2 <= (1ull << 32))
+                    hipLaunchKernelGGL((k_phase_lines<1, uint32_t>), dim3(blocks_for(a.units, 1)), dim3(kBlock), 0,
+                                       s->stream, a);
+                else
+                    hipLaunchKernelGGL((k_phase_lines<2, uint64_t>), dim3(blocks_for(a.units, 2)), dim3(kBlock), 0,
+                                       s->stream, a);
+            } else if (all_ge1(fp) && n >= 2) {
                 PhaseArgs<float4, float> a{};
                 a.sv = (float4*)s->amps;
                 std::vector<int> f4;

@@ -67,7 +67,32 @@ def cpu_baseline(gates, n, budget_s=12.0):
         pass
     return {"value": done / dt, "unit": "gate-apps/s", "cores": threads, "kind": "port",
             "sample": f"first {done} of {len(gates)} gates of the {n}-qubit brickwork, FP64 oracle "
-                      f"(oracle/aqs_oracle.c, OpenMP), host CPU: {cpu}"}
+                      f"(oracle/aqs_oracle.c, OpenMP), host CPU: {cpu}",
+            "reference_method": reference_method_baseline(n, threads)}
+
+
+def reference_method_baseline(n_target, threads, sizes=(10, 11, 12)):
+    """The reference's own algorithm restated on the CPU (oracle.refmethod_run:
+    an N x N identity product, then a dense N x N product per dense gate and an
+    O(N) pass per CSR gate) on depth-2 brickwork circuits at small n; the time
+    per gate application grows as 4^n, so the fit t(n) = c 4^n extrapolates
+    it to the n_target-qubit workload."""
+    import oracle
+    rates, per_gate = {}, {}
+    for n in sizes:
+        gates = circuits.brickwork(n, 2, seed=SEED)
+        sv = oracle.basis_state(n, 0)
+        t0 = time.perf_counter()
+        oracle.refmethod_run(sv, n, gates, threads)
+        dt = time.perf_counter() - t0
+        rates[n] = round(len(gates) / dt, 2)
+        per_gate[n] = dt / len(gates)
+    c = sum(per_gate[n] / 4.0 ** n for n in sizes) / len(sizes)
+    return {"unit": "gate-apps/s", "cores": threads, "kind": "reference-method restatement",
+            "measured": rates, "fit": "seconds per gate = c * 4^n", "c": c,
+            "extrapolated_value": 1.0 / (c * 4.0 ** n_target),
+            "sample": f"depth-2 brickwork at n = {list(sizes)}; the reference itself needs ArrayFire, "
+                      "absent from this image"}
 
 
 def batched(args):

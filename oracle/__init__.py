@@ -66,6 +66,8 @@ def lib():
         L.orc_apply.argtypes = [dp, ctypes.c_int, gp]
         L.orc_run.argtypes = [dp, ctypes.c_int, gp, ctypes.c_int]
         L.orc_run_threads.argtypes = [dp, ctypes.c_int, gp, ctypes.c_int, ctypes.c_int]
+        L.orc_refmethod_run.argtypes = [dp, ctypes.c_int, gp, ctypes.c_int, ctypes.c_int]
+        L.orc_refmethod_run.restype = ctypes.c_int
         L.orc_matrix_1q.argtypes = [gp, dp]
         L.orc_dense_operator.argtypes = [ctypes.c_int, gp, dp]
         L.orc_gemv.argtypes = [ctypes.c_int, dp, dp, dp]
@@ -122,6 +124,15 @@ def run(sv: np.ndarray, n: int, gates, threads: int = 1) -> np.ndarray:
     out = np.ascontiguousarray(sv, dtype=np.complex128).copy()
     arr, _keep = _pack(gates)
     lib().orc_run_threads(_dp(out.view(np.float64)), n, arr, len(gates), threads)
+    return out
+
+
+def refmethod_run(sv: np.ndarray, n: int, gates, threads: int = 1) -> np.ndarray:
+    """The reference's method (dense N x N products for dense gates), for timing."""
+    out = np.ascontiguousarray(sv, dtype=np.complex128).copy()
+    arr, _keep = _pack(gates)
+    if lib().orc_refmethod_run(_dp(out.view(np.float64)), n, arr, len(gates), threads) != 0:
+        raise MemoryError("N x N operator allocation failed")
     return out
 
 

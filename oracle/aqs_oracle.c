@@ -529,3 +529,88 @@ void orc_sample_fp64_cdf(const double* sv, int n, const double* u, int shots, ui
     }
     free(cdf);
 }
+
+/* ---------------------------------------------------------------------------
+ * Timing restatement of the reference's method (CPU baseline only).
+ *
+ * simulate() on an uncompiled circuit (src/quantum.cpp:277-291) first
+ * multiplies the state by the circuit's N x N operator, the identity while
+ * nothing is compiled (:283), then applies every gate's operator.  Gates the
+ * reference holds as dense N x N operators -- H, RotX, RotY, RotZ built as
+ * I (x) U (x) I (src/utils.cpp:169-195, src/quantum.cpp:676-831), the
+ * controlled rotations and Gate / ControlGate (src/quantum.cpp:1424-1510,
+ * 1705-1960) -- cost a dense build plus an N x N product; the CSR gates (X, Y,
+ * Z, Phase, Swap, CX, CY, CZ, CH, CPhase, CSwap, CCNot, Or) an O(N) row pass.
+ * U3, which the reference lacks, is its decomposition Phase(phi) RotY(theta)
+ * Phase(lambda).  FP64, rows split over OpenMP threads.
+ * ------------------------------------------------------------------------- */
+
+static int reference_dense_kind(int kind) {
+    switch (kind) {
+        case ORC_H: case ORC_ROTX: case ORC_ROTY: case ORC_ROTZ:
+        case ORC_CROTX: case ORC_CROTY: case ORC_CROTZ:
+        case ORC_CIRCUIT: case ORC_CONTROL_CIRCUIT:
+            return 1;
+        default:
+            return 0;
+    }
+}
+
+static void gemv_threads(int n, const double* op, double* sv, double* tmp, int threads) {
+    const int64_t N = (int64_t)1 << n;
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (int64_t i = 0; i < N; ++i) {
+        double re = 0.0, im = 0.0;
+        const double* row = op + 2 * (size_t)i * (size_t)N;
+        for (int64_t j = 0; j < N; ++j) {
+            re += row[2 * j] * sv[2 * j] - row[2 * j + 1] * sv[2 * j + 1];
+            im += row[2 * j] * sv[2 * j + 1] + row[2 * j + 1] * sv[2 * j];
+        }
+        tmp[2 * i] = re;
+        tmp[2 * i + 1] = im;
+    }
+    memcpy(sv, tmp, sizeof(double) * 2 * (size_t)N);
+}
+
+static void reference_gate(double* sv, int n, const orc_gate* g, double* op, double* tmp, int threads) {
+    if (reference_dense_kind(g->kind)) {
+        orc_dense_operator(n, g, op);
+        gemv_threads(n, op, sv, tmp, threads);
+    } else {
+        orc_apply(sv, n, g);
+    }
+}
+
+int orc_refmethod_run(double* sv, int n, const orc_gate* gates, int count, int threads) {
+    const size_t N = (size_t)1 << n;
+    double* op = (double*)malloc(sizeof(double) * 2 * N * N);
+    double* tmp = (double*)malloc(sizeof(double) * 2 * N);
+    if (!op || !tmp) {
+        free(op);
+        free(tmp);
+        return -1;
+    }
+    /* the uncompiled circuit's operator: the identity (src/quantum.cpp:159-186) */
+    memset(op, 0, sizeof(double) * 2 * N * N);
+    for (size_t i = 0; i < N; ++i) op[2 * (i * N + i)] = 1.0;
+    gemv_threads(n, op, sv, tmp, threads);
+    for (int k = 0; k < count; ++k) {
+        const orc_gate* g = &gates[k];
+        if (g->kind == ORC_U3) {
+            orc_gate p = *g, r = *g;
+            p.kind = ORC_PHASE;
+            p.angle = g->lam;
+            reference_gate(sv, n, &p, op, tmp, threads);
+            r.kind = ORC_ROTY;
+            r.angle = g->angle;
+            reference_gate(sv, n, &r, op, tmp, threads);
+            p.angle = g->phi;
+            reference_gate(sv, n, &p, op, tmp, threads);
+        } else {
+            reference_gate(sv, n, g, op, tmp, threads);
+        }
+    }
+    free(op);
+    free(tmp);
+    return 0;
+}

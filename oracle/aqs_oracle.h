@@ -123,6 +123,12 @@ void orc_run_threads(double* sv, int n, const orc_gate* gates, int count,
                      int threads);
 int orc_max_threads(void);
 
+/* Timing restatement of the reference's simulate() on an uncompiled circuit
+ * (identity N x N product, then a dense N x N product per dense gate and an
+ * O(N) pass per CSR gate); same result as orc_run up to rounding.  Allocates
+ * one N x N operator; returns -1 when that fails. */
+int orc_refmethod_run(double* sv, int n, const orc_gate* gates, int count, int threads);
+
 #ifdef __cplusplus
 }
 #endif

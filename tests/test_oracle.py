@@ -250,3 +250,20 @@ def test_gaussian_state_normalised():
     assert abs(np.linalg.norm(sv) - 1.0) < 1e-14
     raw = np.array([oracle.lib().orc_gaussian_component(1234, i, 0) for i in range(1 << n)])
     assert abs(raw.mean()) < 0.2 and 0.8 < raw.std() < 1.2
+
+
+def test_refmethod_matches_oracle():
+    """The reference-method timing restatement (dense N x N products) computes
+    the same state as the index-level oracle for every gate kind."""
+    n = 6
+    rng = np.random.default_rng(11)
+    gates = []
+    for spec in ALL_KINDS_4Q:
+        kind, q, *ang = spec
+        qs = [int(x) for x in rng.choice(n, size=len(q), replace=False)]
+        gates.append(kind_gate((kind, tuple(qs), ang[0]) if ang else (kind, tuple(qs))))
+    U = np.linalg.qr(rng.normal(size=(4, 4)) + 1j * rng.normal(size=(4, 4)))[0]
+    gates.append(oracle.gate("Circuit", 2, k=2, u=U))
+    gates.append(oracle.gate("ControlCircuit", 0, 3, k=2, u=U))
+    sv = random_state(n, 5)
+    np.testing.assert_allclose(oracle.refmethod_run(sv, n, gates, 2), oracle.run(sv, n, gates), atol=1e-12)

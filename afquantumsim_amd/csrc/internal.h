@@ -105,6 +105,8 @@ namespace aqs_hip {
 // Library stream for states without an external stream.
 hipStream_t default_stream();
 int current_device();
+// Compute units of the current device (read once at initialisation).
+int device_cu_count();
 
 // Make sure the state's staging buffer holds at least `bytes`.
 void ensure_staging(aqs_state s, size_t bytes);

@@ -231,12 +231,7 @@ bool apply_matk_mfma(aqs_state s, const aqs_op& op, const float* dev_apack) {
         a.tc[c] = v;
     }
     a.apack = dev_apack;
-    hipDeviceProp_t prop;
-    static int cus = 0;
-    if (!cus) {
-        AQS_HIP(hipGetDeviceProperties(&prop, current_device()));
-        cus = prop.multiProcessorCount;
-    }
+    const int cus = device_cu_count();
     const uint64_t blocks = std::min<uint64_t>(a.tiles, (uint64_t)cus * 16);
     ProfScope ps(s, kSlotMfma);
     if (k == 5)

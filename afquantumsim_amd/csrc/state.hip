@@ -5,6 +5,7 @@
 // interleaved complex values per aqs_state.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -21,8 +22,9 @@ void set_error(const std::string& msg) { g_error = msg; }
 void clear_error() { g_error.clear(); }
 
 static std::mutex g_init_mutex;
-static int g_device = -1;
-static hipStream_t g_stream = nullptr;
+static std::atomic<int> g_device{-1};
+static std::atomic<int> g_cus{0};
+static std::atomic<hipStream_t> g_stream{nullptr};
 
 static void init_device(int device) {
     std::lock_guard<std::mutex> lock(g_init_mutex);
@@ -33,11 +35,14 @@ static void init_device(int device) {
     if (device < 0 || device >= count)
         fail(AQS_ERR_OUT_OF_RANGE, "device index " + std::to_string(device) + " out of range");
     AQS_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    AQS_HIP(hipGetDeviceProperties(&prop, device));
     if (!g_stream || g_device != device) {
         hipStream_t st;
         AQS_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         g_stream = st;
     }
+    g_cus = prop.multiProcessorCount;
     g_device = device;
 }
 
@@ -49,6 +54,11 @@ hipStream_t default_stream() {
 int current_device() {
     if (g_device < 0) init_device(0);
     return g_device;
+}
+
+int device_cu_count() {
+    if (g_device < 0) init_device(0);
+    return g_cus;
 }
 
 void ensure_staging(aqs_state s, size_t bytes) {

@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "internal.h"
 #include "schedule.h"
@@ -352,15 +353,16 @@ template <typename T2, int W>
 void launch(aqs_state s, const TileArgs& a, const TileDevProgram& p, int per_cu) {
     constexpr int M = W + kTileRegBits;
     const size_t lds = (size_t)(1 << (M - 5)) * sizeof(uint64_t) + lds_len(1u << M) * sizeof(T2);
-    static int cus = 0;
-    if (!cus) {
-        hipDeviceProp_t prop;
-        AQS_HIP(hipGetDeviceProperties(&prop, current_device()));
-        cus = prop.multiProcessorCount;
+    if (lds > 65536) {
+        // once per kernel instantiation: raising the limit is not free per launch
+        static std::once_flag once;
+        static hipError_t rc = hipSuccess;
+        std::call_once(once, [&] {
+            rc = hipFuncSetAttribute((const void*)k_tile<T2, W>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        });
+        AQS_HIP(rc);
     }
-    if (lds > 65536)
-        AQS_HIP(hipFuncSetAttribute((const void*)k_tile<T2, W>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    const uint64_t blocks = std::min<uint64_t>(a.tiles, (uint64_t)cus * per_cu);
+    const uint64_t blocks = std::min<uint64_t>(a.tiles, (uint64_t)device_cu_count() * per_cu);
     hipLaunchKernelGGL((k_tile<T2, W>), dim3((unsigned)blocks), dim3(1 << W), lds, s->stream,
                        reinterpret_cast<T2*>(s->amps), reinterpret_cast<const TileSegDev*>(p.segs),
                        reinterpret_cast<const TileOpDev*>(p.ops), reinterpret_cast<const TileTermDev*>(p.terms),

@@ -711,9 +711,12 @@ aqs_status aqs_apply_schedule(aqs_state s, aqs_schedule sch) {
                     o[0] = append(t.segs.data(), t.segs.size() * sizeof(TileSegDev));
                     o[1] = append(t.ops.data(), t.ops.size() * sizeof(TileOpDev));
                     o[2] = append(t.terms.data(), t.terms.size() * sizeof(TileTermDev));
+                    o[5] = append(t.folds.data(), t.folds.size() * sizeof(TileFoldDev));
+                    o[6] = append(t.fold_terms.data(), t.fold_terms.size() * sizeof(TileFoldTermDev));
                     if (s->precision == AQS_C128) {
                         o[3] = append(t.m128.data(), t.m128.size() * sizeof(double));
                         o[4] = append(t.z128.data(), t.z128.size() * sizeof(double));
+                        o[7] = append(t.fold_z128.data(), t.fold_z128.size() * sizeof(double));
                     }
                     continue;
                 }
@@ -760,7 +763,8 @@ aqs_status aqs_apply_schedule(aqs_state s, aqs_schedule sch) {
                 const auto& o = sch->dev_tile[p];
                 const char* base = (const char*)sch->dev;
                 apply_tile_pass(s, sch->tiles[sch->tile_index[p]],
-                                TileDevProgram{base + o[0], base + o[1], base + o[2], base + o[3], base + o[4]});
+                                TileDevProgram{base + o[0], base + o[1], base + o[2], base + o[3], base + o[4],
+                                               base + o[5], base + o[6], base + o[7]});
                 continue;
             }
             if (sch->diag_index[p] >= 0) {

@@ -50,9 +50,8 @@ inline int tile_code(int kind, int form, int slot, bool zeta) {
     return (kind == TILE_XCH ? 12 : 0) + 4 * form + slot;
 }
 
-// 64 bytes: the kernel keeps the next op's fields in flight while it runs
-// the current one.  Complex64 kernels read the FP32 matrix here; complex128
-// kernels read TilePass::m128[8 * op].
+// 64 bytes, read with scalar loads.  Complex64 kernels read the FP32 matrix
+// here; complex128 kernels read TilePass::m128[8 * op].
 struct TileOpDev {
     int32_t code;     // tile_code()
     uint32_t any;     // controls: at least one set instead of all set
@@ -68,15 +67,32 @@ static_assert(sizeof(TileOpDev) == 64, "compact tile op");
 // (register slots, thread-held local positions, index bits outside the
 // tile).  Terms of one PHASES op are sorted by register mask; the first term
 // of each equal-mask group carries the group length.  z in FP32 here, in
-// FP64 in TilePass::z128[2 * term].
+// FP64 in TilePass::z128[2 * term].  A folded term (fold >= 0) takes its z
+// from the per-tile value of fold `fold` instead.
 struct TileTermDev {
     uint16_t m_reg;
     uint16_t group;
     uint32_t m_thr;
     uint64_t m_glb;
     float zf[2];
+    int32_t fold;
+    uint32_t pad;
 };
-static_assert(sizeof(TileTermDev) == 24, "compact tile term");
+static_assert(sizeof(TileTermDev) == 32, "compact tile term");
+
+// Fold: the product of the phase terms that pair one local bit with bits
+// outside the tile, evaluated once per tile (it only depends on the tile's
+// outer bits): value = product over [begin, begin + count) of fold terms of
+// (z where the tile has every bit of m_glb).  The run's folded term then
+// applies the value to the amplitudes with that local bit set.
+struct TileFoldDev {
+    uint32_t begin, count;
+};
+struct TileFoldTermDev {
+    uint64_t m_glb;
+    float zf[2];
+};
+constexpr int kTileMaxFolds = 256;
 
 struct TileSegDev {
     int32_t reg[kTileRegBits];
@@ -92,6 +108,9 @@ struct TilePass {
     std::vector<double> m128;  // [ops][8]
     std::vector<TileTermDev> terms;
     std::vector<double> z128;  // [terms][2]
+    std::vector<TileFoldDev> folds;
+    std::vector<TileFoldTermDev> fold_terms;
+    std::vector<double> fold_z128;  // [fold_terms][2]
 };
 
 // Device copies of one tile pass's program.
@@ -101,6 +120,9 @@ struct TileDevProgram {
     const void* terms;
     const void* m128;  // complex128 states only
     const void* z128;
+    const void* folds;
+    const void* fold_terms;
+    const void* fold_z128;
 };
 
 // Runs one tile pass.
@@ -123,7 +145,7 @@ struct aqs_schedule_s {
     void* dev = nullptr;
     int dev_precision = -1;
     std::vector<size_t> dev_off;
-    std::vector<std::array<size_t, 5>> dev_tile;  // tile passes: segs, ops, terms, m128, z128
+    std::vector<std::array<size_t, 8>> dev_tile;  // tile passes: TileDevProgram fields
     std::vector<char> dev_mfma;
 
     // appends a pass with no side data

@@ -38,6 +38,7 @@ namespace {
 struct TileArgs {
     uint64_t tiles;
     int nseg;
+    int nfold;
     int local[kTileMaxLocal];
 };
 
@@ -118,16 +119,22 @@ __device__ __forceinline__ void cmul_acc(R& ar, R& ai, R zr, R zi) {
 // form multiplies P into the amplitudes whose register index holds the
 // mask, the zeta form into the subset factor f[mask], then F[j] = product
 // of f[s] over the subsets s of j and one multiplication per amplitude.
-// Product over one group of terms (same register mask) for this thread.
-template <typename R>
+// Product over one group of terms (same register mask) for this thread;
+// folded terms read their per-tile value from LDS.
+template <typename T2, typename R>
 __device__ __forceinline__ void group_product(const TileTermDev* __restrict__ tt, const double* __restrict__ z128,
-                                              uint32_t k, uint32_t g, uint32_t lthr, uint64_t base, R& pr, R& pi) {
+                                              const T2* fold_val, uint32_t k, uint32_t g, uint32_t lthr,
+                                              uint64_t base, R& pr, R& pi) {
     pr = R(1);
     pi = R(0);
     for (uint32_t e = k; e < k + g; ++e) {
         const bool c = ((lthr & tt[e].m_thr) == tt[e].m_thr) && ((base & tt[e].m_glb) == tt[e].m_glb);
         R zr, zi;
-        if constexpr (sizeof(R) == 4) {
+        if (tt[e].fold >= 0) {
+            const T2 v = fold_val[tt[e].fold];
+            zr = v.x;
+            zi = v.y;
+        } else if constexpr (sizeof(R) == 4) {
             zr = tt[e].zf[0];
             zi = tt[e].zf[1];
         } else {
@@ -140,12 +147,12 @@ __device__ __forceinline__ void group_product(const TileTermDev* __restrict__ tt
 
 template <typename T2, typename R>
 __device__ __forceinline__ void phases_direct(T2 (&x)[16], const TileTermDev* __restrict__ tt,
-                                           const double* __restrict__ z128, uint32_t count, uint32_t lthr,
-                                           uint64_t base) {
+                                           const double* __restrict__ z128, const T2* fold_val, uint32_t count,
+                                           uint32_t lthr, uint64_t base) {
     for (uint32_t k = 0; k < count;) {
         const uint32_t g = tt[k].group, mr = tt[k].m_reg;
         R pr, pi;
-        group_product(tt, z128, k, g, lthr, base, pr, pi);
+        group_product<T2, R>(tt, z128, fold_val, k, g, lthr, base, pr, pi);
         k += g;
 #pragma unroll
         for (int j = 0; j < 16; ++j)
@@ -155,8 +162,8 @@ __device__ __forceinline__ void phases_direct(T2 (&x)[16], const TileTermDev* __
 
 template <typename T2, typename R>
 __device__ __forceinline__ void phases_zeta(T2 (&x)[16], const TileTermDev* __restrict__ tt,
-                                         const double* __restrict__ z128, uint32_t count, uint32_t lthr,
-                                         uint64_t base) {
+                                         const double* __restrict__ z128, const T2* fold_val, uint32_t count,
+                                         uint32_t lthr, uint64_t base) {
     R fr[16], fi[16];
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
@@ -166,7 +173,7 @@ __device__ __forceinline__ void phases_zeta(T2 (&x)[16], const TileTermDev* __re
     for (uint32_t k = 0; k < count;) {
         const uint32_t g = tt[k].group, mr = tt[k].m_reg;
         R pr, pi;
-        group_product(tt, z128, k, g, lthr, base, pr, pi);
+        group_product<T2, R>(tt, z128, fold_val, k, g, lthr, base, pr, pi);
         k += g;
         switch (mr) {
 #define AQS_TERM_CASE(S) \
@@ -244,7 +251,8 @@ template <typename T2, int W>
 __global__ __launch_bounds__(1 << W, (sizeof(T2) == 8 ? 4 : 2)) void k_tile(
     T2* __restrict__ sv, const TileSegDev* __restrict__ segs, const TileOpDev* __restrict__ ops,
     const TileTermDev* __restrict__ terms, const double* __restrict__ m128, const double* __restrict__ z128,
-    TileArgs a) {
+    const TileFoldDev* __restrict__ folds, const TileFoldTermDev* __restrict__ fold_terms,
+    const double* __restrict__ fold_z128, TileArgs a) {
     using R = typename Cplx<T2>::R;
     constexpr int T = 1 << W;
     constexpr int M = W + kTileRegBits;
@@ -252,6 +260,7 @@ __global__ __launch_bounds__(1 << W, (sizeof(T2) == 8 ? 4 : 2)) void k_tile(
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     uint64_t* hi_tab = reinterpret_cast<uint64_t*>(smem);
     T2* lds = reinterpret_cast<T2*>(smem + HI * sizeof(uint64_t));
+    T2* fold_val = lds + lds_len(1u << M);
     const int tid = threadIdx.x;
     // index offsets of the local positions 5..M-1
     for (int v = tid; v < HI; v += T) {
@@ -269,6 +278,22 @@ __global__ __launch_bounds__(1 << W, (sizeof(T2) == 8 ? 4 : 2)) void k_tile(
             base = ((base >> q) << (q + 1)) | (base & ((1ull << q) - 1ull));
         }
         load_tile<T2, M, T>(sv, base, hi_tab, lds, tid);
+        // per-tile values of the folded phase terms (outer bits only)
+        for (int f = tid; f < a.nfold; f += T) {
+            R pr = R(1), pi = R(0);
+            const TileFoldDev fd = folds[f];
+            for (uint32_t e = fd.begin; e < fd.begin + fd.count; ++e) {
+                if ((base & fold_terms[e].m_glb) != fold_terms[e].m_glb) continue;
+                if constexpr (sizeof(R) == 4)
+                    cmul_acc(pr, pi, fold_terms[e].zf[0], fold_terms[e].zf[1]);
+                else
+                    cmul_acc(pr, pi, fold_z128[2 * e], fold_z128[2 * e + 1]);
+            }
+            T2 v;
+            v.x = pr;
+            v.y = pi;
+            fold_val[f] = v;
+        }
         __syncthreads();
         for (int sg = 0; sg < a.nseg; ++sg) {
             const TileSegDev& S = segs[sg];
@@ -286,26 +311,22 @@ __global__ __launch_bounds__(1 << W, (sizeof(T2) == 8 ? 4 : 2)) void k_tile(
             T2 x[16];
 #pragma unroll
             for (int j = 0; j < 16; ++j) x[j] = lds[addr(j)];
-            // the next op's fields load while the current one runs
             const uint32_t ob = S.op_begin, oc = S.op_count;
-            TileOpDev op = ops[ob];
             for (uint32_t o = 0; o < oc; ++o) {
-                const TileOpDev next = ops[ob + (o + 1 < oc ? o + 1 : o)];
+                const TileOpDev& op = ops[ob + o];
                 const int code = op.code;
                 if (code >= kTileCodePhases) {
                     if (code == kTileCodePhases)
-                        phases_direct<T2, R>(x, terms + op.term_begin, z128 + 2 * (size_t)op.term_begin,
+                        phases_direct<T2, R>(x, terms + op.term_begin, z128 + 2 * (size_t)op.term_begin, fold_val,
                                              op.term_count, lthr, base);
                     else
-                        phases_zeta<T2, R>(x, terms + op.term_begin, z128 + 2 * (size_t)op.term_begin,
+                        phases_zeta<T2, R>(x, terms + op.term_begin, z128 + 2 * (size_t)op.term_begin, fold_val,
                                            op.term_count, lthr, base);
-                    op = next;
                     continue;
                 }
                 const bool any = op.any != 0;
                 // all-set controls outside the tile: one test per tile
                 if (!any && (base & op.c_glb) != op.c_glb) {
-                    op = next;
                     continue;
                 }
                 const uint32_t creg = op.c_reg, cthr = op.c_thr;
@@ -338,7 +359,6 @@ __global__ __launch_bounds__(1 << W, (sizeof(T2) == 8 ? 4 : 2)) void k_tile(
 #undef AQS_PAIRS
                     default: break;
                 }
-                op = next;
             }
 #pragma unroll
             for (int j = 0; j < 16; ++j) lds[addr(j)] = x[j];
@@ -352,13 +372,16 @@ __global__ __launch_bounds__(1 << W, (sizeof(T2) == 8 ? 4 : 2)) void k_tile(
 template <typename T2, int W>
 void launch(aqs_state s, const TileArgs& a, const TileDevProgram& p, int per_cu) {
     constexpr int M = W + kTileRegBits;
-    const size_t lds = (size_t)(1 << (M - 5)) * sizeof(uint64_t) + lds_len(1u << M) * sizeof(T2);
-    if (lds > 65536) {
+    constexpr size_t lds_fixed = (size_t)(1 << (M - 5)) * sizeof(uint64_t) + lds_len(1u << M) * sizeof(T2);
+    const size_t lds = lds_fixed + (size_t)a.nfold * sizeof(T2);
+    constexpr size_t lds_max = lds_fixed + (size_t)kTileMaxFolds * sizeof(T2);
+    if (lds_max > 65536) {
         // once per kernel instantiation: raising the limit is not free per launch
         static std::once_flag once;
         static hipError_t rc = hipSuccess;
         std::call_once(once, [&] {
-            rc = hipFuncSetAttribute((const void*)k_tile<T2, W>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            rc = hipFuncSetAttribute((const void*)k_tile<T2, W>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)lds_max);
         });
         AQS_HIP(rc);
     }
@@ -366,7 +389,10 @@ void launch(aqs_state s, const TileArgs& a, const TileDevProgram& p, int per_cu)
     hipLaunchKernelGGL((k_tile<T2, W>), dim3((unsigned)blocks), dim3(1 << W), lds, s->stream,
                        reinterpret_cast<T2*>(s->amps), reinterpret_cast<const TileSegDev*>(p.segs),
                        reinterpret_cast<const TileOpDev*>(p.ops), reinterpret_cast<const TileTermDev*>(p.terms),
-                       reinterpret_cast<const double*>(p.m128), reinterpret_cast<const double*>(p.z128), a);
+                       reinterpret_cast<const double*>(p.m128), reinterpret_cast<const double*>(p.z128),
+                       reinterpret_cast<const TileFoldDev*>(p.folds),
+                       reinterpret_cast<const TileFoldTermDev*>(p.fold_terms),
+                       reinterpret_cast<const double*>(p.fold_z128), a);
     AQS_HIP(hipGetLastError());
 }
 
@@ -385,6 +411,8 @@ void apply_tile_pass(aqs_state s, const TilePass& pass, const TileDevProgram& pr
     TileArgs a{};
     a.tiles = s->N >> m;
     a.nseg = (int)pass.segs.size();
+    a.nfold = (int)pass.folds.size();
+    if (a.nfold > kTileMaxFolds) fail(AQS_ERR_INVALID_ARGUMENT, "tile pass has too many folded terms");
     for (int p = 0; p < m; ++p) a.local[p] = pass.local[p];
     ProfScope ps(s, kSlotTile);
     const bool c64 = s->precision == AQS_C64;

@@ -341,7 +341,11 @@ class Tiler {
         }
         ok = ok && tp.m128.size() == 8 * tp.ops.size() && tp.z128.size() == 2 * tp.terms.size();
         for (const auto& t : tp.terms)
-            ok = ok && t.m_reg < (1u << kTileRegBits) && t.m_thr < (1u << m) && (t.m_glb >> n_) == 0;
+            ok = ok && t.m_reg < (1u << kTileRegBits) && t.m_thr < (1u << m) && (t.m_glb >> n_) == 0 &&
+                 t.fold >= -1 && t.fold < (int32_t)tp.folds.size() && (t.fold < 0 || t.m_glb == 0);
+        ok = ok && tp.folds.size() <= (size_t)kTileMaxFolds && tp.fold_z128.size() == 2 * tp.fold_terms.size();
+        for (const auto& f : tp.folds) ok = ok && (uint64_t)f.begin + f.count <= tp.fold_terms.size();
+        for (const auto& f : tp.fold_terms) ok = ok && (f.m_glb >> n_) == 0;
         if (!ok) fail(AQS_ERR_RUNTIME, "tile schedule malformed");
     }
 
@@ -407,7 +411,56 @@ class Tiler {
         merged.erase(std::remove_if(merged.begin(), merged.end(), [](const M& m) { return m.z == cd(1); }),
                      merged.end());
         if (merged.empty()) return;
-        std::stable_sort(merged.begin(), merged.end(), [](const M& a, const M& b) { return a.reg < b.reg; });
+        // Terms pairing one local bit with bits outside the tile: per local
+        // bit, one folded term whose value the kernel forms once per tile.
+        std::vector<M> kept;
+        std::vector<std::pair<uint32_t, uint32_t>> fold_keys;  // (reg, thr) of each new fold
+        std::vector<std::vector<M>> fold_members;
+        for (const M& t : merged) {
+            const int local = __builtin_popcount(t.reg) + __builtin_popcount(t.thr);
+            if (local != 1 || t.glb == 0) {
+                kept.push_back(t);
+                continue;
+            }
+            size_t f = 0;
+            while (f < fold_keys.size() && fold_keys[f] != std::make_pair(t.reg, t.thr)) ++f;
+            if (f == fold_keys.size()) {
+                if (tp.folds.size() + fold_keys.size() >= (size_t)kTileMaxFolds) {
+                    kept.push_back(t);
+                    continue;
+                }
+                fold_keys.push_back({t.reg, t.thr});
+                fold_members.emplace_back();
+            }
+            fold_members[f].push_back(t);
+        }
+        std::vector<int32_t> fold_of(kept.size(), -1);
+        for (size_t f = 0; f < fold_keys.size(); ++f) {
+            if (fold_members[f].size() < 2) {  // a lone term gains nothing from folding
+                kept.push_back(fold_members[f][0]);
+                fold_of.push_back(-1);
+                continue;
+            }
+            TileFoldDev fd{(uint32_t)tp.fold_terms.size(), (uint32_t)fold_members[f].size()};
+            for (const M& t : fold_members[f]) {
+                tp.fold_terms.push_back(TileFoldTermDev{t.glb, {(float)t.z.real(), (float)t.z.imag()}});
+                tp.fold_z128.push_back(t.z.real());
+                tp.fold_z128.push_back(t.z.imag());
+            }
+            kept.push_back(M{fold_keys[f].first, fold_keys[f].second, 0, cd(1)});
+            fold_of.push_back((int32_t)tp.folds.size());
+            tp.folds.push_back(fd);
+        }
+        std::vector<size_t> order(kept.size());
+        for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return kept[a].reg < kept[b].reg; });
+        std::vector<M> sorted_terms;
+        std::vector<int32_t> sorted_fold;
+        for (size_t i : order) {
+            sorted_terms.push_back(kept[i]);
+            sorted_fold.push_back(fold_of[i]);
+        }
+        merged.swap(sorted_terms);
         TileOpDev op{};
         op.term_begin = (uint32_t)tp.terms.size();
         op.term_count = (uint32_t)merged.size();
@@ -420,6 +473,7 @@ class Tiler {
             d.m_glb = merged[i].glb;
             d.zf[0] = (float)merged[i].z.real();
             d.zf[1] = (float)merged[i].z.imag();
+            d.fold = sorted_fold[i];
             if (i == gend) {
                 size_t e = i;
                 while (e < merged.size() && merged[e].reg == merged[i].reg && e - i < 0xFFFF) ++e;
@@ -440,23 +494,31 @@ class Tiler {
 
 // Modelled cost of one pass in full-state sweeps (read + write of every
 // amplitude at the streaming rate), fitted to MI355X timings of brickwork,
-// random-layer and QFT circuits at 22..28 qubits: a tile pass costs its
-// sweep (1.1) plus its in-register work -- a 2x2 product 0.107 of a sweep,
-// an exchange 0.046, a phase run 0.11 plus 0.011 per term; an MFMA block
-// 1.17-1.2, a diagonal table 1.25, a bit permutation 1.8, a plain
-// one-qubit pass 1, a sparse pass its touched fraction.
+// random-layer and QFT circuits at 22..28 qubits (mean error 9%): a tile
+// pass costs its sweep (1.1) plus its in-register work -- a 2x2 product
+// 0.107 of a sweep, an exchange 0.034, a phase term 0.059 and each complex
+// product a phase run applies per thread 0.0016; an MFMA block 1.17-1.2, a
+// diagonal table 1.25, a bit permutation 1.8, a plain one-qubit pass 1, a
+// sparse pass its touched fraction.
 double pass_cost(const aqs_schedule_s* sch, size_t p) {
     const aqs_op& op = sch->passes[p];
     if (sch->tile_index[p] >= 0) {
         const TilePass& t = sch->tiles[sch->tile_index[p]];
         double c = 1.1;
         for (const auto& o : t.ops) {
-            if (o.code < 12)
+            if (o.code < 12) {
                 c += 0.107;
-            else if (o.code < kTileCodePhases)
-                c += 0.046;
-            else
-                c += 0.11 + 0.011 * o.term_count;
+            } else if (o.code < kTileCodePhases) {
+                c += 0.034;
+            } else {
+                const bool zeta = o.code == kTileCodePhases + 1;
+                double products = zeta ? 48.0 : 0.0;
+                for (uint32_t k = 0; k < o.term_count; ++k) {
+                    const TileTermDev& tt = t.terms[o.term_begin + k];
+                    if (tt.group) products += zeta ? 1.0 : 16.0 / (double)(1u << __builtin_popcount(tt.m_reg));
+                }
+                c += 0.059 * o.term_count + 0.0016 * products;
+            }
         }
         return c;
     }

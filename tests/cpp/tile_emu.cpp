@@ -120,6 +120,12 @@ void tile_apply(std::vector<cplx>& v, int n, const TilePass& tp) {
             return g;
         };
         for (uint32_t l = 0; l < tile.size(); ++l) tile[l] = v[gidx(l)];
+        // folded terms: one value per tile from the outer bits
+        std::vector<cplx> fold_val(tp.folds.size(), cplx(1));
+        for (size_t f = 0; f < tp.folds.size(); ++f)
+            for (uint32_t e = tp.folds[f].begin; e < tp.folds[f].begin + tp.folds[f].count; ++e)
+                if ((base & tp.fold_terms[e].m_glb) == tp.fold_terms[e].m_glb)
+                    fold_val[f] *= cplx(tp.fold_z128[2 * e], tp.fold_z128[2 * e + 1]);
         for (const auto& S : tp.segs) {
             for (uint32_t tid = 0; tid < (1u << W); ++tid) {
                 uint32_t lthr = 0;
@@ -142,8 +148,9 @@ void tile_apply(std::vector<cplx>& v, int n, const TilePass& tp) {
                             const size_t e = op.term_begin + k;
                             const TileTermDev& t = tp.terms[e];
                             if ((base & t.m_glb) != t.m_glb || (lthr & t.m_thr) != t.m_thr) continue;
+                            const cplx z = t.fold >= 0 ? fold_val[t.fold] : cplx(tp.z128[2 * e], tp.z128[2 * e + 1]);
                             for (int j = 0; j < 16; ++j)
-                                if (((uint32_t)j & t.m_reg) == t.m_reg) x[j] *= cplx(tp.z128[2 * e], tp.z128[2 * e + 1]);
+                                if (((uint32_t)j & t.m_reg) == t.m_reg) x[j] *= z;
                         }
                         continue;
                     }
@@ -185,14 +192,14 @@ extern "C" {
 
 // Builds the tiled schedule of ops and applies it to `state` (2^n complex
 // FP64, interleaved) on the host.  Returns the aqs_status of the build;
-// counts[0..2] = passes, tile passes, segments.
+// counts[0..3] = passes, tile passes, segments, folded term groups.
 int tile_emulate(const aqs_op* ops, size_t count, int n, int tile_qubits, double* state, size_t* counts) {
     aqs_schedule sch = nullptr;
     const int st = aqs_fuse_tiled(ops, count, n, tile_qubits, &sch);
     if (st != AQS_OK) return st;
     std::vector<cplx> v((size_t)1 << n);
     std::memcpy(v.data(), state, v.size() * sizeof(cplx));
-    size_t tiles = 0, segs = 0;
+    size_t tiles = 0, segs = 0, folds = 0;
     for (size_t p = 0; p < sch->passes.size(); ++p) {
         const aqs_op& op = sch->passes[p];
         if (sch->tile_index[p] >= 0) {
@@ -205,6 +212,7 @@ int tile_emulate(const aqs_op* ops, size_t count, int n, int tile_qubits, double
             }
             ++tiles;
             segs += tp.segs.size();
+            folds += tp.folds.size();
         } else if (sch->diag_index[p] >= 0) {
             aqs_schedule_destroy(sch);
             return -1;  // diagonal tables are not emulated here
@@ -219,6 +227,7 @@ int tile_emulate(const aqs_op* ops, size_t count, int n, int tile_qubits, double
     counts[0] = sch->passes.size();
     counts[1] = tiles;
     counts[2] = segs;
+    counts[3] = folds;
     aqs_schedule_destroy(sch);
     return AQS_OK;
 }

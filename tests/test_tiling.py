@@ -35,7 +35,7 @@ def emu():
 def emulate(lib, n, sv, gates, tile_qubits=0):
     arr, keep = hip.pack_ops(ops.lower_all(gates, exact=True))
     v = np.ascontiguousarray(sv, dtype=np.complex128).copy()
-    counts = (ctypes.c_size_t * 3)()
+    counts = (ctypes.c_size_t * 4)()
     rc = lib.tile_emulate(arr, len(gates) and len(arr), n, tile_qubits, v.ctypes.data, counts)
     del keep
     assert rc == 0, (rc, hip.load().aqs_last_error())
@@ -64,14 +64,14 @@ def random_all_kinds(n, count, seed):
 
 @pytest.mark.parametrize("n,tile", [(10, 0), (12, 10), (13, 11), (14, 13), (15, 12)])
 def test_tiled_random_layered(emu, n, tile):
-    passes, tiles, segs = check(emu, n, circuits.random_layered(n, 8, seed=n), seed=1, tile_qubits=tile)
+    passes, tiles, segs, _ = check(emu, n, circuits.random_layered(n, 8, seed=n), seed=1, tile_qubits=tile)
     assert tiles >= 1 and segs >= tiles
 
 
 @pytest.mark.parametrize("n", [11, 14, 16])
 def test_tiled_brickwork(emu, n):
     gates = circuits.brickwork(n, 6, seed=3)
-    passes, tiles, segs = check(emu, n, gates, seed=2)
+    passes, tiles, segs, _ = check(emu, n, gates, seed=2)
     # one sweep per tile pass, far fewer than gates
     assert passes < len(gates) / 4
 
@@ -173,3 +173,21 @@ def test_auto_mode():
         c = hip.Schedule(lowered, n, mode="auto")
         assert c.cost() == min(a.cost(), b.cost())
         assert c.mode == ("tiled" if b.cost() < a.cost() else "blocks")
+
+
+@pytest.mark.parametrize("tile", [10, 12])
+def test_tiled_folded_phase_terms(emu, tile):
+    """Phase ladders between tile qubits and outer qubits are folded into one
+    per-tile value per local bit (QFT, random CPhase / controlled-RZ mixes)."""
+    n = 16
+    counts = check(emu, n, circuits.fourier_transform(n, final_swaps=True), seed=3, tile_qubits=tile)
+    assert counts[3] > 0
+    rng = np.random.default_rng(tile)
+    g = [oracle.gate("H", q) for q in range(n)]
+    for _ in range(150):
+        a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+        kind = ("CPhase", "CRotZ", "CZ")[int(rng.integers(3))]
+        g.append(oracle.gate(kind, a, b, angle=float(rng.uniform(-3, 3))))
+    g += [oracle.gate("RotX", q, angle=0.2 * q) for q in range(n)]
+    counts = check(emu, n, g, seed=4, tile_qubits=tile)
+    assert counts[3] > 0

@@ -769,6 +769,21 @@ aqs_status aqs_apply_schedule(aqs_state s, aqs_schedule sch) {
             }
             if (sch->diag_index[p] >= 0) {
                 const auto& dp = sch->diags[sch->diag_index[p]];
+                // or in the prologue of the next MFMA block when its anchor
+                // lies in that block's tile
+                if (p + 1 < sch->passes.size() && sch->dev_mfma[p + 1] && sch->passes[p + 1].kind == AQS_OP_MATK &&
+                    sch->tile_index[p + 1] < 0 && sch->diag_index[p + 1] < 0) {
+                    DiagEpilogueDesc epi;
+                    epi.required = dp.required;
+                    epi.anchor = dp.anchor;
+                    epi.chunks = dp.chunks;
+                    epi.dev_tab = d;
+                    if (apply_matk_mfma(s, sch->passes[p + 1], (const float*)((const char*)sch->dev + sch->dev_off[p + 1]),
+                                        &epi, true)) {
+                        ++p;
+                        continue;
+                    }
+                }
                 apply_diag_table(s, dp.required, dp.anchor, dp.chunks, d);
                 continue;
             }
@@ -781,8 +796,22 @@ aqs_status aqs_apply_schedule(aqs_state s, aqs_schedule sch) {
                 apply_op(s, op);
                 continue;
             }
-            if (sch->dev_mfma[p] && apply_matk_mfma(s, op, (const float*)d)) continue;
             if (sch->dev_mfma[p]) {
+                // a diagonal table pass right after the block rides in its
+                // epilogue when its anchor lies in the block's tile
+                if (p + 1 < sch->passes.size() && sch->diag_index[p + 1] >= 0) {
+                    const auto& dp = sch->diags[sch->diag_index[p + 1]];
+                    DiagEpilogueDesc epi;
+                    epi.required = dp.required;
+                    epi.anchor = dp.anchor;
+                    epi.chunks = dp.chunks;
+                    epi.dev_tab = (const char*)sch->dev + sch->dev_off[p + 1];
+                    if (apply_matk_mfma(s, op, (const float*)d, &epi)) {
+                        ++p;
+                        continue;
+                    }
+                }
+                if (apply_matk_mfma(s, op, (const float*)d)) continue;
                 apply_op(s, op);  // MFMA shape not usable at this size: plain matrix path
                 continue;
             }

@@ -127,9 +127,21 @@ struct ProfScope {
 
 // Gate dispatch (gates.hip / mfma.hip).
 void apply_op(aqs_state s, const aqs_op& op);
+// A diagonal table pass (anchor, chunks, required bits as amplitude bit
+// positions; dev_tab = its float2 table on the device) to apply in the
+// epilogue of an MFMA block.
+struct DiagEpilogueDesc {
+    std::vector<int> required, anchor;
+    std::vector<std::vector<int>> chunks;
+    const void* dev_tab = nullptr;
+};
+
 // k-qubit dense block on MFMA (k = 4 or 5, complex64); returns false when
-// the op shape is not supported by the MFMA path.
-bool apply_matk_mfma(aqs_state s, const aqs_op& op, const float* dev_apack);
+// the op shape is not supported by the MFMA path, or when `epi` is given
+// and its anchor bits do not lie in the block's tile.  `epi` is applied
+// after the block, or before it when `before` is set.
+bool apply_matk_mfma(aqs_state s, const aqs_op& op, const float* dev_apack, const DiagEpilogueDesc* epi = nullptr,
+                     bool before = false);
 // MATK with the matrix already on the device (FP32 for complex64, FP64 for
 // complex128, plain row-major).
 void apply_matk_device(aqs_state s, const aqs_op& op, const void* dev_u);

@@ -21,6 +21,16 @@
 //   k = 4: the stacked [Ur; Ui] (32 x 16) times Xr and times Xi
 //          -> 16 MFMAs per 32 columns.
 //
+// Diagonal epilogue.  A diagonal table pass (diag.hip) that follows a block,
+// whose anchor bits lie in the block's tile and which has at most one chunk
+// touching the tile, is applied while the tile is written out: per tile the
+// chunks outside the tile give one factor per anchor value (formed by the
+// first lanes into LDS), and the chunk inside the tile, if any, is looked up
+// per amplitude in a copy of its table slice held in LDS for the whole
+// launch.  The table pass then costs no sweep of its own.  The same table
+// can instead precede the block (prologue): the factors then scale the tile
+// as it is read in.
+//
 // Operand layouts of v_mfma_f32_32x32x2_f32 (wave64):
 //   A (32 x 2):  lane l holds A[l % 32][l / 32]
 //   B (2 x 32):  lane l holds B[l / 32][l % 32]
@@ -38,6 +48,20 @@ namespace {
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
+// Diagonal table applied in the epilogue (amplitude bit positions).
+struct MfmaEpilogue {
+    const float2* tab;    // [2^ng][nc][32]
+    uint64_t reqmask;     // bits that must be 1 for a factor other than 1
+    int ng, gstart;       // anchor bits (gstart >= 0: consecutive from gstart)
+    int gpos[5];
+    int nc;
+    uint32_t outside;     // chunks whose bits all lie outside the tile
+    int inside;           // the chunk touching the tile, or -1
+    int cstart[6];
+    int clen[6];
+    int cpos[6][5];
+};
+
 struct MfmaArgs {
     float4* sv4;
     uint64_t tiles;
@@ -48,7 +72,37 @@ struct MfmaArgs {
     uint32_t tr[32];      // tile-index bits of target sub-index r
     uint32_t tc[32];      // tile-index bits of column c
     const float* apack;
+    MfmaEpilogue epi;
 };
+
+__device__ __forceinline__ uint32_t bits_of(uint64_t i, int start, const int* pos, int len) {
+    if (start >= 0) return (uint32_t)(i >> start) & ((1u << len) - 1u);
+    uint32_t v = 0;
+#pragma unroll
+    for (int b = 0; b < 5; ++b) {
+        if (b >= len) break;
+        v |= (uint32_t)((i >> pos[b]) & 1ull) << b;
+    }
+    return v;
+}
+
+// Factor of amplitude i: the per-tile part pg[g] times the inside chunk's
+// entry tin[g][v] (tin: that chunk's table slice in LDS).
+__device__ __forceinline__ float2 epi_factor(const MfmaEpilogue& e, const float2* pg, const float2* tin, uint64_t i) {
+    if ((i & e.reqmask) != e.reqmask) return make_float2(1.f, 0.f);
+    const uint32_t g = bits_of(i, e.gstart, e.gpos, e.ng);
+    float2 f = pg[g];
+    if (e.inside >= 0) {
+        const int c = e.inside;
+        const float2 t = tin[g * 32 + bits_of(i, e.cstart[c], e.cpos[c], e.clen[c])];
+        f = make_float2(f.x * t.x - f.y * t.y, f.x * t.y + f.y * t.x);
+    }
+    return f;
+}
+
+__device__ __forceinline__ float2 cmul2(float2 f, float xr, float xi) {
+    return make_float2(f.x * xr - f.y * xi, f.x * xi + f.y * xr);
+}
 
 __device__ __forceinline__ uint32_t padded(uint32_t t) { return t + ((t >> 5) << 1); }
 
@@ -62,12 +116,20 @@ __device__ __forceinline__ uint64_t tile_base(uint64_t t, const MfmaArgs& a) {
     return t;  // amplitude index of the tile's first element (even)
 }
 
-template <int K>
+// DIAG: 0 none, 1 diagonal table after the block (epilogue), 2 before it (prologue).
+template <int K, int DIAG>
 __global__ __launch_bounds__(64) void k_matk_mfma(MfmaArgs a) {
+    constexpr bool EPI = DIAG != 0;
     constexpr int TILE = 32 * K;            // amplitudes per tile
     constexpr int CHUNKS = TILE / 128;      // 64 lanes x float4 = 128 amplitudes
     constexpr int LDS_F2 = TILE + TILE / 16;
     __shared__ __attribute__((aligned(16))) float2 lds[LDS_F2];
+    __shared__ float2 pg[EPI ? 32 : 1];     // epilogue: per-tile factor per anchor value
+    __shared__ float2 tin[EPI ? 1024 : 1];  // epilogue: the inside chunk's table slice [g][v]
+    if (EPI && a.epi.inside >= 0) {
+        for (int e = threadIdx.x; e < (32 << a.epi.ng); e += 64)
+            tin[e] = a.epi.tab[((e >> 5) * a.epi.nc + a.epi.inside) * 32 + (e & 31)];
+    }
     const int lane = threadIdx.x;
     const int col = lane & 31;
     const int half = lane >> 5;
@@ -92,6 +154,28 @@ __global__ __launch_bounds__(64) void k_matk_mfma(MfmaArgs a) {
     }
     while (t < a.tiles) {
         const uint64_t b4 = tile_base(t, a) >> 1;
+        if (EPI && lane < (1 << a.epi.ng)) {
+            // chunks outside the tile: fixed by the tile's outer bits
+            const uint64_t base = b4 << 1;
+            float2 f = make_float2(1.f, 0.f);
+            for (int c = 0; c < a.epi.nc; ++c) {
+                if (!((a.epi.outside >> c) & 1u)) continue;
+                const float2 v = a.epi.tab[(lane * a.epi.nc + c) * 32 +
+                                           bits_of(base, a.epi.cstart[c], a.epi.cpos[c], a.epi.clen[c])];
+                f = make_float2(f.x * v.x - f.y * v.y, f.x * v.y + f.y * v.x);
+            }
+            pg[lane] = f;
+        }
+        if (DIAG == 2) {
+            __syncthreads();  // pg of this tile before the prologue reads it
+#pragma unroll
+            for (int j = 0; j < CHUNKS; ++j) {
+                const uint64_t i0 = (b4 + a.gj[j] + glane) << 1;
+                const float2 f0 = epi_factor(a.epi, pg, tin, i0), f1 = epi_factor(a.epi, pg, tin, i0 | 1ull);
+                const float2 y0 = cmul2(f0, pf[j].x, pf[j].y), y1 = cmul2(f1, pf[j].z, pf[j].w);
+                pf[j] = make_float4(y0.x, y0.y, y1.x, y1.y);
+            }
+        }
         // tile -> LDS (memory order)
 #pragma unroll
         for (int j = 0; j < CHUNKS; ++j) {
@@ -141,7 +225,14 @@ __global__ __launch_bounds__(64) void k_matk_mfma(MfmaArgs a) {
 #pragma unroll
         for (int j = 0; j < CHUNKS; ++j) {
             const uint32_t tau = 2u * (uint32_t)(j * 64 + lane);
-            st_nt(*reinterpret_cast<const float4*>(&lds[padded(tau)]), &a.sv4[b4 + a.gj[j] + glane]);
+            float4 v = *reinterpret_cast<const float4*>(&lds[padded(tau)]);
+            if (DIAG == 1) {
+                const uint64_t i0 = (b4 + a.gj[j] + glane) << 1;
+                const float2 f0 = epi_factor(a.epi, pg, tin, i0), f1 = epi_factor(a.epi, pg, tin, i0 | 1ull);
+                const float2 y0 = cmul2(f0, v.x, v.y), y1 = cmul2(f1, v.z, v.w);
+                v = make_float4(y0.x, y0.y, y1.x, y1.y);
+            }
+            st_nt(v, &a.sv4[b4 + a.gj[j] + glane]);
         }
         __syncthreads();
         t = tn;
@@ -188,7 +279,8 @@ std::vector<float> pack_mfma_operand(int k, const double* u) {
     return out;
 }
 
-bool apply_matk_mfma(aqs_state s, const aqs_op& op, const float* dev_apack) {
+bool apply_matk_mfma(aqs_state s, const aqs_op& op, const float* dev_apack, const DiagEpilogueDesc* epi,
+                     bool before) {
     const int k = op.ntargets;
     if (s->precision != AQS_C64 || op.ncontrols != 0 || (k != 4 && k != 5)) return false;
     if (s->n < k + 5 || s->n > 36) return false;
@@ -231,13 +323,56 @@ bool apply_matk_mfma(aqs_state s, const aqs_op& op, const float* dev_apack) {
         a.tc[c] = v;
     }
     a.apack = dev_apack;
+    if (epi) {
+        // the anchor must lie in the tile (it indexes the per-tile factors);
+        // a chunk is either wholly outside the tile or looked up per amplitude
+        auto in_tile = [&](int b) { return std::find(T.begin(), T.end(), b) != T.end(); };
+        if (epi->anchor.size() > 5 || epi->chunks.size() > 6 || !epi->dev_tab) return false;
+        for (int b : epi->anchor)
+            if (!in_tile(b)) return false;
+        MfmaEpilogue& e = a.epi;
+        e.tab = reinterpret_cast<const float2*>(epi->dev_tab);
+        for (int b : epi->required) e.reqmask |= 1ull << b;
+        auto consecutive = [](const std::vector<int>& p) {
+            for (size_t j = 1; j < p.size(); ++j)
+                if (p[j] != p[0] + (int)j) return -1;
+            return p.empty() ? 0 : p[0];
+        };
+        e.ng = (int)epi->anchor.size();
+        for (int j = 0; j < e.ng; ++j) e.gpos[j] = epi->anchor[j];
+        e.gstart = consecutive(epi->anchor);
+        e.nc = (int)epi->chunks.size();
+        for (int c = 0; c < e.nc; ++c) {
+            const auto& ch = epi->chunks[c];
+            if (ch.size() > 5) return false;
+            e.clen[c] = (int)ch.size();
+            for (size_t j = 0; j < ch.size(); ++j) e.cpos[c][j] = ch[j];
+            e.cstart[c] = consecutive(ch);
+            bool outside = true;
+            for (int b : ch) outside = outside && !in_tile(b);
+            if (outside) e.outside |= 1u << c;
+        }
+        e.inside = -1;
+        for (int c = 0; c < e.nc; ++c) {
+            if ((e.outside >> c) & 1u) continue;
+            if (e.inside >= 0) return false;  // two chunks touch the tile: separate pass
+            e.inside = c;
+        }
+    }
     const int cus = device_cu_count();
     const uint64_t blocks = std::min<uint64_t>(a.tiles, (uint64_t)cus * 16);
     ProfScope ps(s, kSlotMfma);
-    if (k == 5)
-        hipLaunchKernelGGL(k_matk_mfma<32>, dim3((unsigned)blocks), dim3(64), 0, s->stream, a);
-    else
-        hipLaunchKernelGGL(k_matk_mfma<16>, dim3((unsigned)blocks), dim3(64), 0, s->stream, a);
+    const int mode = !epi ? 0 : before ? 2 : 1;
+    const dim3 grid((unsigned)blocks), block(64);
+    if (k == 5) {
+        if (mode == 0) hipLaunchKernelGGL((k_matk_mfma<32, 0>), grid, block, 0, s->stream, a);
+        if (mode == 1) hipLaunchKernelGGL((k_matk_mfma<32, 1>), grid, block, 0, s->stream, a);
+        if (mode == 2) hipLaunchKernelGGL((k_matk_mfma<32, 2>), grid, block, 0, s->stream, a);
+    } else {
+        if (mode == 0) hipLaunchKernelGGL((k_matk_mfma<16, 0>), grid, block, 0, s->stream, a);
+        if (mode == 1) hipLaunchKernelGGL((k_matk_mfma<16, 1>), grid, block, 0, s->stream, a);
+        if (mode == 2) hipLaunchKernelGGL((k_matk_mfma<16, 2>), grid, block, 0, s->stream, a);
+    }
     AQS_HIP(hipGetLastError());
     return true;
 }

@@ -498,8 +498,9 @@ class Tiler {
 // pass costs its sweep (1.1) plus its in-register work -- a 2x2 product
 // 0.107 of a sweep, an exchange 0.034, a phase term 0.059 and each complex
 // product a phase run applies per thread 0.0016; an MFMA block 1.17-1.2, a
-// diagonal table 1.25, a bit permutation 1.8, a plain one-qubit pass 1, a
-// sparse pass its touched fraction.
+// diagonal table 1.25 (0.05 when it rides in a neighbouring MFMA block), a
+// bit permutation 1.8, a plain one-qubit pass 1, a sparse pass its touched
+// fraction.
 double pass_cost(const aqs_schedule_s* sch, size_t p) {
     const aqs_op& op = sch->passes[p];
     if (sch->tile_index[p] >= 0) {
@@ -541,9 +542,43 @@ double pass_cost(const aqs_schedule_s* sch, size_t p) {
     }
 }
 
+// Index bits of the tile an MFMA block sweeps (its targets and the five
+// lowest other bits), or empty when the op is not an MFMA block.
+std::vector<int> mfma_tile_bits(const aqs_schedule_s* sch, size_t p) {
+    if (p >= sch->passes.size()) return {};
+    const aqs_op& op = sch->passes[p];
+    const int n = sch->n;
+    if (op.kind != AQS_OP_MATK || op.ncontrols || (op.ntargets != 4 && op.ntargets != 5) || n < op.ntargets + 5)
+        return {};
+    std::vector<int> T;
+    for (int j = 0; j < op.ntargets; ++j) T.push_back(n - 1 - op.targets[j]);
+    for (int b = 0, added = 0; b < n && added < 5; ++b)
+        if (std::find(T.begin(), T.end(), b) == T.end()) {
+            T.push_back(b);
+            ++added;
+        }
+    return T;
+}
+
+// A diagonal table next to an MFMA block whose tile holds its anchor runs
+// inside that block's sweep (apply_matk_mfma epilogue / prologue).
+bool diag_rides_block(const aqs_schedule_s* sch, size_t p) {
+    const auto& anchor = sch->diags[sch->diag_index[p]].anchor;
+    for (size_t q : {p - 1, p + 1}) {
+        if (p == 0 && q == p - 1) continue;
+        const std::vector<int> T = mfma_tile_bits(sch, q);
+        if (T.empty()) continue;
+        bool inside = true;
+        for (int b : anchor) inside = inside && std::find(T.begin(), T.end(), b) != T.end();
+        if (inside) return true;
+    }
+    return false;
+}
+
 double schedule_cost(const aqs_schedule_s* sch) {
     double c = 0;
-    for (size_t p = 0; p < sch->passes.size(); ++p) c += pass_cost(sch, p);
+    for (size_t p = 0; p < sch->passes.size(); ++p)
+        c += (sch->diag_index[p] >= 0 && diag_rides_block(sch, p)) ? 0.05 : pass_cost(sch, p);
     return c;
 }
 

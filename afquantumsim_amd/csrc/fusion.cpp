@@ -23,6 +23,12 @@
 //     diagonal or acts on other qubits);
 //   - a Swap that fits no open block becomes a plain pass (it moves half the
 //     state; a dense block would move all of it).
+// aqs_fuse also fuses a second program order in which every diagonal op is
+// hoisted to just after the last non-diagonal op sharing one of its qubits
+// (diagonal ops commute with each other and with ops on other qubits), and
+// keeps the schedule of lower modelled cost.  Ladders written "phases, then
+// the Hadamard" (inverse_fourier_transform) become "Hadamard, then its
+// phases", whose deferred terms flush once per block instead of per qubit.
 // Each block's unitary is accumulated in FP64 and rounded to FP32 once, at
 // upload for a complex64 state.
 #include <hip/hip_runtime.h>
@@ -202,7 +208,7 @@ class Fuser {
    public:
     Fuser(aqs_schedule_s* sch, int n, int kmax) : sch_(sch), n_(n), kmax_(kmax) {}
 
-    void run(const aqs_op* ops_in, size_t count) {
+    void run(const aqs_op* ops_in, size_t count, bool hoist) {
         // Uncontrolled swaps become relabellings: L[q] is the physical qubit
         // holding logical qubit q; every other op is rewritten through L.
         std::vector<int> L(n_);
@@ -220,6 +226,7 @@ class Fuser {
             ops.push_back(op);
         }
         count = ops.size();
+        if (hoist) ops = hoist_diagonals(ops);
         // non-diagonal uses per qubit, for the look-ahead of diagonal runs
         std::vector<std::vector<size_t>> uses(n_);
         for (size_t i = 0; i < count; ++i)
@@ -277,6 +284,32 @@ class Fuser {
     int n_, kmax_;
     std::vector<Block> open_;
     std::vector<Term> terms_;
+
+    // Every diagonal op moves to just after the last non-diagonal op that
+    // shares a qubit with it (or to the front); non-diagonal ops keep their
+    // order, and so do diagonal ops that land on the same spot.
+    std::vector<aqs_op> hoist_diagonals(const std::vector<aqs_op>& ops) const {
+        std::vector<long> last(n_, -1);       // last non-diagonal op on each qubit
+        std::vector<std::vector<size_t>> after(ops.size() + 1);  // slot 0 = front
+        for (size_t i = 0; i < ops.size(); ++i) {
+            if (!is_diagonal(ops[i])) {
+                for (int q : op_qubits(ops[i])) last[q] = (long)i;
+                continue;
+            }
+            long a = -1;
+            for (int q : op_qubits(ops[i])) a = std::max(a, last[q]);
+            after[a + 1].push_back(i);
+        }
+        std::vector<aqs_op> out;
+        out.reserve(ops.size());
+        for (size_t j : after[0]) out.push_back(ops[j]);
+        for (size_t i = 0; i < ops.size(); ++i) {
+            if (is_diagonal(ops[i])) continue;
+            out.push_back(ops[i]);
+            for (size_t j : after[i + 1]) out.push_back(ops[j]);
+        }
+        return out;
+    }
 
     std::vector<size_t> touching(const std::vector<int>& S) {
         std::vector<size_t> hit;
@@ -639,19 +672,32 @@ aqs_status aqs_fuse(const aqs_op* ops, size_t count, int nqubits, int max_qubits
         if (nqubits < 1 || nqubits > AQS_MAX_QUBITS) fail(AQS_ERR_INVALID_ARGUMENT, "bad qubit count");
         if (max_qubits < 1 || max_qubits > 5) fail(AQS_ERR_INVALID_ARGUMENT, "max_qubits must be 1..5");
         for (size_t i = 0; i < count; ++i) validate_op(ops[i], nqubits);
-        auto* sch = new aqs_schedule_s();
-        sch->n = nqubits;
-        sch->source_ops = count;
-        try {
-            Fuser f(sch, nqubits, max_qubits);
-            f.run(ops, count);
-            for (size_t p = 0; p < sch->passes.size(); ++p)
-                if (sch->passes[p].kind == AQS_OP_MATK) sch->passes[p].u = sch->mats[p].data();
-        } catch (...) {
-            delete sch;
-            throw;
+        aqs_schedule_s* best = nullptr;
+        double best_cost = 0;
+        for (bool hoist : {false, true}) {
+            auto* sch = new aqs_schedule_s();
+            sch->n = nqubits;
+            sch->source_ops = count;
+            try {
+                Fuser f(sch, nqubits, max_qubits);
+                f.run(ops, count, hoist);
+                for (size_t p = 0; p < sch->passes.size(); ++p)
+                    if (sch->passes[p].kind == AQS_OP_MATK) sch->passes[p].u = sch->mats[p].data();
+            } catch (...) {
+                delete sch;
+                delete best;
+                throw;
+            }
+            const double c = schedule_cost(sch);
+            if (!best || c < best_cost) {
+                delete best;
+                best = sch;
+                best_cost = c;
+            } else {
+                delete sch;
+            }
         }
-        *out = sch;
+        *out = best;
     });
 }
 

@@ -163,6 +163,9 @@ namespace aqs_hip {
 // bit-involution passes that move every amplitude to its logical index.
 void emit_restore_order(aqs_schedule_s* sch, const std::vector<int>& L);
 
+// Modelled run time of a schedule in full-state sweeps (tiling.cpp).
+double schedule_cost(const aqs_schedule_s* sch);
+
 }  // namespace aqs_hip
 
 #endif

@@ -575,14 +575,14 @@ bool diag_rides_block(const aqs_schedule_s* sch, size_t p) {
     return false;
 }
 
-double schedule_cost(const aqs_schedule_s* sch) {
+}  // namespace
+
+double aqs_hip::schedule_cost(const aqs_schedule_s* sch) {
     double c = 0;
     for (size_t p = 0; p < sch->passes.size(); ++p)
         c += (sch->diag_index[p] >= 0 && diag_rides_block(sch, p)) ? 0.05 : pass_cost(sch, p);
     return c;
 }
-
-}  // namespace
 
 extern "C" {
 

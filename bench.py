@@ -191,11 +191,13 @@ def random20(args, n=20, depth=40, shots=1000):
     return out
 
 
-def qft30(args, n=30, seed=30):
-    """30-qubit QFT (fourier_transform + final swaps) on a seeded Gaussian
-    complex64 state (normalised in FP64): unfused per-gate path, compile()
-    fused into <= 5-qubit MFMA blocks, and a complex128 run as reference."""
-    gates = circuits.fourier_transform(n, final_swaps=True)
+def qft30(args, n=30, seed=30, inverse=False):
+    """30-qubit QFT (fourier_transform + final swaps, or
+    inverse_fourier_transform) on a seeded Gaussian complex64 state
+    (normalised in FP64): unfused per-gate path, compile() fused into <= 5-qubit
+    MFMA blocks, the tiled schedule, and a complex128 run as reference."""
+    gates = (circuits.inverse_fourier_transform(n) if inverse
+             else circuits.fourier_transform(n, final_swaps=True))
     lowered = ops.lower_all(gates)
     alg = algorithmic_bytes(lowered, n)
     st = hip.State(n, hip.C64)
@@ -244,7 +246,7 @@ def qft30(args, n=30, seed=30):
         sch.close()
     st.close()
     ref.close()
-    log(f"qft30: {out}")
+    log(f"{'iqft30' if inverse else 'qft30'}: {out}")
     return out
 
 
@@ -464,6 +466,7 @@ def run(args):
     batch = batched(args) if not args.no_batch else None
     r20 = random20(args)
     qft = qft30(args) if not args.no_qft else None
+    iqft = qft30(args, inverse=True) if not args.no_qft else None
     serve = served(args) if args.serve_requests > 0 else None
 
     cpu = cpu_baseline(gates, n) if not args.no_cpu else None
@@ -493,6 +496,7 @@ def run(args):
         "batched": batch,
         "random20": r20,
         "qft30": qft,
+        "iqft30": iqft,
         "served": serve,
         "cpu_baseline": cpu,
         "device": dev,

@@ -7,7 +7,11 @@
 // segment, per thread the 16 register amplitudes, with the op encodings
 // (register slots, thread-fixed positions, global bits) interpreted as the
 // kernel interprets them.  Plain passes and bit permutations are applied
-// directly.  The result is compared with the oracle by the Python test.
+// directly.  block_emulate() does the same for the dense-block schedule of
+// aqs_fuse, diagonal tables included (afquantumsim_amd/csrc/diag.hip:
+// factor = product over chunks of table[anchor value][chunk][chunk value]
+// on amplitudes whose required bits are all set).  The results are compared
+// with the oracle by the Python tests.
 #include <complex>
 #include <cstring>
 #include <vector>
@@ -84,6 +88,24 @@ void plain_apply(std::vector<cplx>& v, int n, const aqs_op& op) {
         }
         default:
             break;
+    }
+}
+
+void diag_table_apply(std::vector<cplx>& v, const DiagPass& d) {
+    const size_t nc = d.chunks.size();
+    for (uint64_t i = 0; i < v.size(); ++i) {
+        bool req = true;
+        for (int b : d.required) req = req && ((i >> b) & 1);
+        if (!req) continue;
+        uint32_t g = 0;
+        for (size_t j = 0; j < d.anchor.size(); ++j) g |= (uint32_t)((i >> d.anchor[j]) & 1) << j;
+        cplx f = 1.0;
+        for (size_t c = 0; c < nc; ++c) {
+            uint32_t x = 0;
+            for (size_t j = 0; j < d.chunks[c].size(); ++j) x |= (uint32_t)((i >> d.chunks[c][j]) & 1) << j;
+            f *= d.table[(g * nc + c) * 32 + x];
+        }
+        v[i] *= f;
     }
 }
 
@@ -228,6 +250,39 @@ int tile_emulate(const aqs_op* ops, size_t count, int n, int tile_qubits, double
     counts[1] = tiles;
     counts[2] = segs;
     counts[3] = folds;
+    aqs_schedule_destroy(sch);
+    return AQS_OK;
+}
+
+// Builds the dense-block schedule of ops (aqs_fuse, blocks of at most kmax
+// qubits) and applies it to `state` on the host.  counts[0..3] = passes,
+// dense blocks, diagonal tables, bit permutations.
+int block_emulate(const aqs_op* ops, size_t count, int n, int kmax, double* state, size_t* counts) {
+    aqs_schedule sch = nullptr;
+    const int st = aqs_fuse(ops, count, n, kmax, &sch);
+    if (st != AQS_OK) return st;
+    std::vector<cplx> v((size_t)1 << n);
+    std::memcpy(v.data(), state, v.size() * sizeof(cplx));
+    size_t blocks = 0, tables = 0, perms = 0;
+    for (size_t p = 0; p < sch->passes.size(); ++p) {
+        const aqs_op& op = sch->passes[p];
+        if (sch->diag_index[p] >= 0) {
+            diag_table_apply(v, sch->diags[sch->diag_index[p]]);
+            ++tables;
+        } else if (op.kind == AQS_OP_PERMB) {
+            for (auto& pi : sch->perm_index)
+                if (pi.first == (int)p) permute_bits(v, n, sch->perms[pi.second]);
+            ++perms;
+        } else {
+            blocks += op.kind == AQS_OP_MATK;
+            plain_apply(v, n, op);
+        }
+    }
+    std::memcpy(state, v.data(), v.size() * sizeof(cplx));
+    counts[0] = sch->passes.size();
+    counts[1] = blocks;
+    counts[2] = tables;
+    counts[3] = perms;
     aqs_schedule_destroy(sch);
     return AQS_OK;
 }

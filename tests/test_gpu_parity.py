@@ -214,6 +214,30 @@ def test_fused_diagonal_deferral(kmax, precision):
     np.testing.assert_allclose(got, want, atol=tol)
 
 
+@pytest.mark.parametrize("name,n", [("qft", 14), ("qft", 20), ("iqft", 16), ("iqft", 21)])
+def test_fused_tables_ride_mfma_blocks(name, n):
+    """A diagonal table next to a 4/5-qubit MFMA block whose tile holds its
+    anchor bits is applied in that block's store loop (table after the
+    block) or at its load (table before it): fewer table launches than
+    table passes, same state as the oracle."""
+    gates = (circuits.fourier_transform(n, final_swaps=True) if name == "qft"
+             else circuits.inverse_fourier_transform(n))
+    sv = random_state(n, n)
+    want = oracle.run(sv, n, gates, threads=8)
+    sch = hip.Schedule(ops.lower_all(gates), n, 5)
+    tables = sum(k == hip.OP_DIAGT for k, _ in sch.passes())
+    st = hip.State(n, hip.C64)
+    st.upload(sv.astype(st.dtype))
+    st.profile(True)
+    st.apply_schedule(sch)
+    launched = st.profile_read().get("DIAGT", (0.0, 0))[1]
+    got = st.download()
+    st.close()
+    assert tables >= 1 and launched < tables
+    assert oracle.infidelity(got, want) < 1e-9
+    np.testing.assert_allclose(got, want, atol=5e-6)
+
+
 @pytest.mark.parametrize("n", [3, 7, 12, 17])
 def test_fused_swaps_are_relabelled_bit_exact(n):
     """Swap-only circuits compile to at most two in-place bit-permutation

@@ -6,12 +6,18 @@
 // moves amplitude i to P(i), where P sends index bit b to bit beta(b) and
 // beta is an involution, so P pairs amplitudes and works in place.
 //
-// Tiling: the tile bits T are bits 0..4 and their partners (<= 10 bits), so
-// tiles are runs of >= 32 amplitudes and T is closed under beta; the other
-// bits R are permuted among themselves.  A workgroup takes an outer index o
-// over R and its partner o' = beta_R(o) (o < o'; o == o' is a tile permuted
-// within itself), stages both tiles in LDS and writes each one, permuted by
-// beta_T, to the other tile's place with coalesced stores.
+// Tiling: the tile bits T are bits 0..4, their partners and the pairs added
+// below (10-11 bits), so tiles are runs of >= 32 amplitudes and T is closed
+// under beta; the other bits R are permuted among themselves.  A workgroup
+// takes an outer index o over R and its partner o' = beta_R(o) (o < o';
+// o == o' is a tile permuted within itself), stages both tiles in LDS and
+// writes each one, permuted by beta_T, to the other tile's place with
+// coalesced stores.
+//
+// Small tiles starve the pass (a 5-bit tile, as when beta only moves middle
+// bits, ran at 0.1 TB/s at 30 qubits; 8 bits at 0.9 TB/s), so T is grown to
+// at least 10 bits with further outer bits and their partners, lowest first:
+// 10-11 bits run at 2.8-3.1 TB/s on the same permutations.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,8 +34,8 @@ constexpr int kBlock = 256;
 struct PermArgs {
     void* sv;
     int nt, nr;
-    int tpos[10];     // tile bit positions (ascending; tpos[j] = j for j < 5)
-    int tmap[10];     // tile bit j -> tile bit tmap[j]
+    int tpos[12];     // tile bit positions (ascending; tpos[j] = j for j < 5)
+    int tmap[12];     // tile bit j -> tile bit tmap[j]
     int rpos[34];     // outer bit positions (ascending)
     int rmap[34];     // outer bit j -> outer bit rmap[j]
     uint64_t outer;   // 2^nr
@@ -49,15 +55,22 @@ __device__ __forceinline__ uint64_t permute_bits(uint64_t v, const int* map, int
 
 // Tile-local tables, built once per workgroup: index offsets of tile bits
 // 5.. and the permuted position of every tile element.
-constexpr int kMaxTile = 1 << 10;
+constexpr int kMaxTileBits = 11;
+constexpr int kMaxTile = 1 << kMaxTileBits;
+
+// LDS slot of tile element p: one element of padding per 32.  Permuted
+// writes (a bit reversal sends consecutive elements 32 apart) then spread
+// over the banks instead of hitting one; the contiguous reads stay
+// contiguous within each run of 32.
+__device__ __forceinline__ int lds_slot(int p) { return p + (p >> 5); }
 
 template <typename T2>
 __global__ __launch_bounds__(kBlock) void k_bitperm(PermArgs a) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     const int tile = 1 << a.nt;
     T2* A = reinterpret_cast<T2*>(smem);
-    T2* B = A + tile;
-    __shared__ uint64_t hi_off[32];
+    T2* B = A + lds_slot(tile);
+    __shared__ uint64_t hi_off[kMaxTile >> 5];
     __shared__ uint16_t pmap[kMaxTile];
     // tile bits 0..min(nt, 5)-1 are index bits 0..4 (all of them when nt < 5)
     const int hi = a.nt > 5 ? a.nt - 5 : 0;
@@ -67,7 +80,8 @@ __global__ __launch_bounds__(kBlock) void k_bitperm(PermArgs a) {
             if ((v >> j) & 1) o |= 1ull << a.tpos[5 + j];
         hi_off[v] = o;
     }
-    for (int t = threadIdx.x; t < tile; t += kBlock) pmap[t] = (uint16_t)permute_bits((uint64_t)t, a.tmap, a.nt);
+    for (int t = threadIdx.x; t < tile; t += kBlock)
+        pmap[t] = (uint16_t)lds_slot((int)permute_bits((uint64_t)t, a.tmap, a.nt));
     __syncthreads();
     T2* sv = reinterpret_cast<T2*>(a.sv);
     for (uint64_t o = blockIdx.x; o < a.outer; o += gridDim.x) {
@@ -94,10 +108,11 @@ __global__ __launch_bounds__(kBlock) void k_bitperm(PermArgs a) {
             for (int f = threadIdx.x; f < tile / 2; f += kBlock) {
                 const int t = 2 * f;
                 const uint64_t off = (uint64_t)(t & 31) | hi_off[t >> 5];
-                const float2 p = A[t], q = A[t + 1];
+                const int l = lds_slot(t);  // t even: t + 1 shares its run
+                const float2 p = A[l], q = A[l + 1];
                 st_nt(make_float4(p.x, p.y, q.x, q.y), &sv4[((pair ? b2 : b1) | off) >> 1]);
                 if (pair) {
-                    const float2 u = B[t], v = B[t + 1];
+                    const float2 u = B[l], v = B[l + 1];
                     st_nt(make_float4(u.x, u.y, v.x, v.y), &sv4[(b1 | off) >> 1]);
                 }
             }
@@ -110,8 +125,8 @@ __global__ __launch_bounds__(kBlock) void k_bitperm(PermArgs a) {
             __syncthreads();
             for (int t = threadIdx.x; t < tile; t += kBlock) {
                 const uint64_t off = (uint64_t)(t & 31) | hi_off[t >> 5];
-                sv[(pair ? b2 : b1) | off] = A[t];
-                if (pair) sv[b1 | off] = B[t];
+                sv[(pair ? b2 : b1) | off] = A[lds_slot(t)];
+                if (pair) sv[b1 | off] = B[lds_slot(t)];
             }
         }
         __syncthreads();
@@ -129,9 +144,22 @@ void apply_bit_involution(aqs_state s, const std::vector<int>& beta) {
     for (int b = 0; b < n; ++b)
         if (beta[b] < 0 || beta[b] >= n || beta[beta[b]] != b) fail(AQS_ERR_INVALID_ARGUMENT, "not an involution");
     std::vector<char> in_t(n, 0);
+    int nt = 0;
     for (int b = 0; b < std::min(n, 5); ++b) {
+        nt += !in_t[b];
         in_t[b] = 1;
+        nt += !in_t[beta[b]];
         in_t[beta[b]] = 1;
+    }
+    // grow the tile to >= 10 bits (LDS: two tiles of 2^11 complex64 or
+    // 2^10 complex128 amplitudes at most)
+    const int cap = s->precision == AQS_C64 ? kMaxTileBits : 10;
+    for (int b = 5; b < n && nt < 10; ++b) {
+        if (in_t[b]) continue;
+        const int add = beta[b] == b ? 1 : 2;
+        if (nt + add > cap) continue;
+        in_t[b] = in_t[beta[b]] = 1;
+        nt += add;
     }
     std::vector<int> T, R;
     for (int b = 0; b < n; ++b) (in_t[b] ? T : R).push_back(b);
@@ -147,10 +175,12 @@ void apply_bit_involution(aqs_state s, const std::vector<int>& beta) {
         a.rpos[j] = R[j];
         a.rmap[j] = (int)(std::find(R.begin(), R.end(), beta[R[j]]) - R.begin());
     }
+
     a.outer = 1ull << a.nr;
-    if (a.nt < 1 || a.nt > 10) fail(AQS_ERR_INVALID_ARGUMENT, "bit permutation tile out of range");
+    if (a.nt < 1 || a.nt > cap) fail(AQS_ERR_INVALID_ARGUMENT, "bit permutation tile out of range");
     const uint64_t blocks = std::min<uint64_t>(a.outer, 16384);
-    const size_t lds = 2 * ((size_t)1 << a.nt) * s->amp_bytes();
+    const size_t slots = ((size_t)1 << a.nt) + (((size_t)1 << a.nt) >> 5);
+    const size_t lds = 2 * slots * s->amp_bytes();
     ProfScope ps(s, kSlotPermute);
     if (s->precision == AQS_C64)
         hipLaunchKernelGGL(k_bitperm<float2>, dim3((unsigned)blocks), dim3(kBlock), lds, s->stream, a);

@@ -260,6 +260,36 @@ def test_fused_swaps_are_relabelled_bit_exact(n):
         assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
 
 
+def structured_swaps(n, name):
+    """Swap lists whose restore permutations need the bit-permutation tile
+    grown beyond bits 0..4 and their partners."""
+    if name == "reverse":
+        return [oracle.gate("Swap", i, n - 1 - i) for i in range(n // 2)]
+    if name == "rotate":
+        return [oracle.gate("Swap", i, i + 1) for i in range(n - 1)]
+    if name == "middle":  # index bits 5..n-6 reversed, bits 0..4 fixed
+        return [oracle.gate("Swap", 5 + i, n - 6 - i) for i in range((n - 10) // 2)]
+    # low qubits exchanged among themselves only
+    return [oracle.gate("Swap", n - 1, n - 2), oracle.gate("Swap", n - 3, n - 5)]
+
+
+@pytest.mark.parametrize("n", [11, 16, 21])
+@pytest.mark.parametrize("name", ["reverse", "rotate", "middle", "low"])
+def test_fused_structured_permutations_bit_exact(n, name):
+    gates = structured_swaps(n, name)
+    sch = hip.Schedule(ops.lower_all(gates), n, 5)
+    assert {k for k, _ in sch.passes()} <= {hip.OP_PERMB}
+    sv = random_state(n, 5).astype(np.complex64)
+    want = oracle.run(sv.astype(np.complex128), n, gates).astype(np.complex64)
+    for prec in (hip.C64, hip.C128):
+        st = hip.State(n, prec)
+        st.upload(sv.astype(st.dtype))
+        st.apply_schedule(sch)
+        got = st.download().astype(np.complex64)
+        st.close()
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (n, name, prec)
+
+
 @pytest.mark.parametrize("kmax", [2, 5])
 def test_fused_swaps_mixed_with_gates(kmax):
     n = 12

@@ -48,9 +48,9 @@ __device__ __forceinline__ uint64_t insert_zeros(uint64_t t, const Fixed& f) {
     return t;
 }
 
-inline unsigned blocks_for(uint64_t units, int per_thread) {
+inline unsigned blocks_for(uint64_t units, int per_thread, int block = kBlock) {
     uint64_t threads = (units + per_thread - 1) / per_thread;
-    uint64_t b = (threads + kBlock - 1) / kBlock;
+    uint64_t b = (threads + block - 1) / block;
     return (unsigned)std::max<uint64_t>(b, 1);
 }
 
@@ -89,19 +89,22 @@ __device__ __forceinline__ IT insert_zeros_t(IT t, const Fixed& f) {
     return t;
 }
 
-// U independent units per thread; IT = index type (32-bit when the float4
-// index space fits).  The variant is chosen per (qubits, target bit) by
-// aqs_autotune.
-template <int U, typename IT>
-__global__ __launch_bounds__(kBlock) void k_mat1_v4(Mat1V4Args a) {
+// U independent units per thread, BS threads per block; IT = index type
+// (32-bit when the float4 index space fits).  The variant is chosen per
+// (qubits, target bit) by aqs_autotune.  One-wave blocks with one unit per
+// lane stream fastest for most partner distances (6.5-6.8 TB/s at 28 qubits
+// for partners 2^7..2^19 float4 apart, against 5.6-5.9 with 256-thread
+// blocks of 4 units); 256-thread blocks win for near and far partners.
+template <int U, typename IT, int BS>
+__global__ __launch_bounds__(BS) void k_mat1_v4(Mat1V4Args a) {
     const IT units = (IT)a.units, stride = (IT)a.stride, cmask = (IT)a.cmask;
-    const IT base = (IT)blockIdx.x * (IT)(kBlock * U) + (IT)threadIdx.x;
+    const IT base = (IT)blockIdx.x * (IT)(BS * U) + (IT)threadIdx.x;
     float4 lo[U], hi[U];
     IT idx[U];
     bool ok[U];
 #pragma unroll
     for (int j = 0; j < U; ++j) {
-        const IT u = base + (IT)j * kBlock;
+        const IT u = base + (IT)j * BS;
         ok[j] = u < units;
         idx[j] = insert_zeros_t<IT>(u, a.fixed) | cmask;
         if (ok[j]) {
@@ -130,24 +133,32 @@ __global__ __launch_bounds__(kBlock) void k_mat1_v4(Mat1V4Args a) {
     }
 }
 
-// Launch-shape variants of the float4 MAT1 pass.
-constexpr int kMat1Variants = 4;
+// Launch-shape variants of the float4 MAT1 pass: (units per thread, index
+// type, block size) = 0 (1, 32-bit, 256), 1 (2, 64-bit, 256), 2 (4, 64-bit,
+// 256), 3 (2, 32-bit, 256), 4 (1, 32-bit, 64), 5 (2, 32-bit, 64).
+constexpr int kMat1Variants = 6;
 
 void launch_mat1_v4(const Mat1V4Args& a, int variant, uint64_t n_float4, hipStream_t st) {
     const bool fits32 = n_float4 <= (1ull << 32);
-    if (!fits32 && (variant == 0 || variant == 3)) variant = 1;
+    if (!fits32 && variant != 1 && variant != 2) variant = 1;
     switch (variant) {
         case 0:
-            hipLaunchKernelGGL((k_mat1_v4<1, uint32_t>), dim3(blocks_for(a.units, 1)), dim3(kBlock), 0, st, a);
+            hipLaunchKernelGGL((k_mat1_v4<1, uint32_t, 256>), dim3(blocks_for(a.units, 1, 256)), dim3(256), 0, st, a);
             break;
         case 1:
-            hipLaunchKernelGGL((k_mat1_v4<2, uint64_t>), dim3(blocks_for(a.units, 2)), dim3(kBlock), 0, st, a);
+            hipLaunchKernelGGL((k_mat1_v4<2, uint64_t, 256>), dim3(blocks_for(a.units, 2, 256)), dim3(256), 0, st, a);
             break;
         case 2:
-            hipLaunchKernelGGL((k_mat1_v4<4, uint64_t>), dim3(blocks_for(a.units, 4)), dim3(kBlock), 0, st, a);
+            hipLaunchKernelGGL((k_mat1_v4<4, uint64_t, 256>), dim3(blocks_for(a.units, 4, 256)), dim3(256), 0, st, a);
+            break;
+        case 3:
+            hipLaunchKernelGGL((k_mat1_v4<2, uint32_t, 256>), dim3(blocks_for(a.units, 2, 256)), dim3(256), 0, st, a);
+            break;
+        case 4:
+            hipLaunchKernelGGL((k_mat1_v4<1, uint32_t, 64>), dim3(blocks_for(a.units, 1, 64)), dim3(64), 0, st, a);
             break;
         default:
-            hipLaunchKernelGGL((k_mat1_v4<2, uint32_t>), dim3(blocks_for(a.units, 2)), dim3(kBlock), 0, st, a);
+            hipLaunchKernelGGL((k_mat1_v4<2, uint32_t, 64>), dim3(blocks_for(a.units, 2, 64)), dim3(64), 0, st, a);
             break;
     }
 }
@@ -952,5 +963,23 @@ extern "C" aqs_status aqs_autotune(aqs_state s, int reps, int* variant_by_target
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
+    });
+}
+
+// Pins the one-qubit launch shape for this state's size and one target
+// qubit on the current device (the choice aqs_autotune would store).
+extern "C" aqs_status aqs_tune_set(aqs_state s, int qubit, int variant) {
+    return guard([&] {
+        if (!s || !s->amps) fail(AQS_ERR_INVALID_ARGUMENT, "null state handle");
+        if (qubit < 0 || qubit >= s->n) fail(AQS_ERR_OUT_OF_RANGE, "qubit index out of range");
+        if (variant < 0 || variant >= kMat1Variants) fail(AQS_ERR_INVALID_ARGUMENT, "no such launch-shape variant");
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        if (!g_tune_init) {
+            for (auto& d : g_tune)
+                for (auto& q : d)
+                    for (auto& t : q) t = -1;
+            g_tune_init = true;
+        }
+        g_tune[current_device() & 7][s->n][s->n - 1 - qubit] = variant;
     });
 }

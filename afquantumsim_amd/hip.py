@@ -83,7 +83,7 @@ EXPORTS = [
     "aqs_probabilities", "aqs_amplitude", "aqs_sample", "aqs_measure_qubit", "aqs_collapse_to",
     "aqs_inner", "aqs_profile_enable", "aqs_profile_read", "aqs_timer_start", "aqs_timer_stop",
     "aqs_batch_create", "aqs_batch_autotune", "aqs_batch_run", "aqs_batch_state", "aqs_batch_info",
-    "aqs_batch_destroy", "aqs_autotune", "aqs_server_create", "aqs_server_set_starvation_limit",
+    "aqs_batch_destroy", "aqs_autotune", "aqs_tune_set", "aqs_server_create", "aqs_server_set_starvation_limit",
     "aqs_server_submit", "aqs_server_wait", "aqs_server_stats", "aqs_server_destroy",
 ]
 
@@ -162,6 +162,7 @@ def load(path: str = LIB_PATH):
                            ctypes.POINTER(i32)],
         "aqs_batch_destroy": [vp],
         "aqs_autotune": [vp, i32, ctypes.POINTER(i32)],
+        "aqs_tune_set": [vp, i32, i32],
         "aqs_server_create": [sz, i32, i32, ctypes.POINTER(vp)],
         "aqs_server_set_starvation_limit": [vp, ctypes.c_double],
         "aqs_server_submit": [vp, u64, i32, ctypes.POINTER(Op), sz, dp, ctypes.c_uint32, i32],
@@ -269,6 +270,10 @@ class State:
         out = (ctypes.c_int * self.n)()
         check(_lib.aqs_autotune(self.h, reps, out))
         return list(out)
+
+    def tune_set(self, qubit: int, variant: int):
+        """Pin the one-qubit launch shape (0..5) for this size and target qubit."""
+        check(_lib.aqs_tune_set(self.h, qubit, variant))
 
     def apply_schedule(self, sch: "Schedule"):
         check(_lib.aqs_apply_schedule(self.h, sch.h))

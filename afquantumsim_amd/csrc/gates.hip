@@ -30,7 +30,12 @@ using namespace aqs_hip;
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kUnroll = 4;  // independent units per thread (memory-level parallelism)
+
+// One-wave blocks with one unit per lane are the fastest streaming shape;
+// they are used while the grid stays within 2^31 threads (and, for 32-bit
+// index variants, the float4 index space within 2^32), else 256-thread
+// blocks of several units with 64-bit indices.
+inline bool one_wave_shape(uint64_t units) { return units <= (1ull << 31); }
 
 // Sorted bit positions that are held fixed while enumerating work units.
 struct Fixed {
@@ -177,18 +182,19 @@ int tuned_variant(int n, int t) {
 }
 
 // MAT1, complex64, target bit 0 (controls >= 1): the pair is one float4.
-__global__ __launch_bounds__(kBlock) void k_mat1_b0(Mat1V4Args a) {
-    const uint64_t base = (uint64_t)blockIdx.x * (kBlock * kUnroll) + threadIdx.x;
-    float4 v[kUnroll];
-    uint64_t idx[kUnroll];
+template <int U, int BS>
+__global__ __launch_bounds__(BS) void k_mat1_b0(Mat1V4Args a) {
+    const uint64_t base = (uint64_t)blockIdx.x * (BS * U) + threadIdx.x;
+    float4 v[U];
+    uint64_t idx[U];
 #pragma unroll
-    for (int j = 0; j < kUnroll; ++j) {
-        const uint64_t u = base + (uint64_t)j * kBlock;
+    for (int j = 0; j < U; ++j) {
+        const uint64_t u = base + (uint64_t)j * BS;
         idx[j] = (u < a.units) ? (insert_zeros(u, a.fixed) | a.cmask) : ~0ull;
         if (idx[j] != ~0ull) v[j] = ld_nt(&a.sv[idx[j]]);
     }
 #pragma unroll
-    for (int j = 0; j < kUnroll; ++j) {
+    for (int j = 0; j < U; ++j) {
         if (idx[j] == ~0ull) continue;
         float4 x = v[j], n;
         n.x = a.m[0] * x.x - a.m[1] * x.y + a.m[2] * x.z - a.m[3] * x.w;
@@ -253,13 +259,14 @@ struct XchgV4Args {
     uint64_t partner;  // xor to the partner float4
 };
 
-__global__ __launch_bounds__(kBlock) void k_xchg_v4(XchgV4Args a) {
-    const uint64_t base = (uint64_t)blockIdx.x * (kBlock * kUnroll) + threadIdx.x;
-    float4 lo[kUnroll], hi[kUnroll];
-    uint64_t idx[kUnroll];
+template <int U, int BS>
+__global__ __launch_bounds__(BS) void k_xchg_v4(XchgV4Args a) {
+    const uint64_t base = (uint64_t)blockIdx.x * (BS * U) + threadIdx.x;
+    float4 lo[U], hi[U];
+    uint64_t idx[U];
 #pragma unroll
-    for (int j = 0; j < kUnroll; ++j) {
-        const uint64_t u = base + (uint64_t)j * kBlock;
+    for (int j = 0; j < U; ++j) {
+        const uint64_t u = base + (uint64_t)j * BS;
         idx[j] = (u < a.units) ? (insert_zeros(u, a.fixed) | a.setmask) : ~0ull;
         if (idx[j] != ~0ull) {
             lo[j] = ld_nt(&a.sv[idx[j]]);
@@ -267,11 +274,18 @@ __global__ __launch_bounds__(kBlock) void k_xchg_v4(XchgV4Args a) {
         }
     }
 #pragma unroll
-    for (int j = 0; j < kUnroll; ++j) {
+    for (int j = 0; j < U; ++j) {
         if (idx[j] == ~0ull) continue;
         st_nt(hi[j], &a.sv[idx[j]]);
         st_nt(lo[j], &a.sv[idx[j] ^ a.partner]);
     }
+}
+
+void launch_xchg_v4(const XchgV4Args& a, hipStream_t st) {
+    if (one_wave_shape(a.units))
+        hipLaunchKernelGGL((k_xchg_v4<1, 64>), dim3(blocks_for(a.units, 1, 64)), dim3(64), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_xchg_v4<4, 256>), dim3(blocks_for(a.units, 4, 256)), dim3(256), 0, st, a);
 }
 
 // X on bit 0 (complex64): swap the halves of each float4 (controls >= 1).
@@ -306,18 +320,19 @@ __global__ __launch_bounds__(kBlock) void k_phase(PhaseArgs<T2, R> a) {
     }
 }
 
-// U float4s per lane; IT = index type (32-bit when the float4 index space
-// fits, the shape that measured fastest for streaming passes).
-template <int U, typename IT>
-__global__ __launch_bounds__(kBlock) void k_phase_v4(PhaseArgs<float4, float> a) {
+// U float4s per lane, BS threads per block; IT = index type (32-bit when
+// the float4 index space fits).  One-wave blocks with one float4 per lane
+// are the fastest streaming shape.
+template <int U, typename IT, int BS>
+__global__ __launch_bounds__(BS) void k_phase_v4(PhaseArgs<float4, float> a) {
     const IT units = (IT)a.units, setmask = (IT)a.setmask;
-    const IT base = (IT)blockIdx.x * (IT)(kBlock * U) + (IT)threadIdx.x;
+    const IT base = (IT)blockIdx.x * (IT)(BS * U) + (IT)threadIdx.x;
     float4 v[U];
     IT idx[U];
     bool ok[U];
 #pragma unroll
     for (int j = 0; j < U; ++j) {
-        const IT u = base + (IT)j * kBlock;
+        const IT u = base + (IT)j * BS;
         ok[j] = u < units;
         idx[j] = insert_zeros_t<IT>(u, a.fixed) | setmask;
         if (ok[j]) v[j] = ld_nt(&a.sv[idx[j]]);
@@ -347,16 +362,16 @@ struct PhaseLineArgs {
     float pr, pi;
 };
 
-template <int U, typename IT>
-__global__ __launch_bounds__(kBlock) void k_phase_lines(PhaseLineArgs a) {
+template <int U, typename IT, int BS>
+__global__ __launch_bounds__(BS) void k_phase_lines(PhaseLineArgs a) {
     const IT units = (IT)a.units, setmask = (IT)a.setmask;
-    const IT base = (IT)blockIdx.x * (IT)(kBlock * U) + (IT)threadIdx.x;
+    const IT base = (IT)blockIdx.x * (IT)(BS * U) + (IT)threadIdx.x;
     float4 v[U];
     IT idx[U];
     bool ok[U];
 #pragma unroll
     for (int j = 0; j < U; ++j) {
-        const IT u = base + (IT)j * kBlock;
+        const IT u = base + (IT)j * BS;
         ok[j] = u < units;
         idx[j] = insert_zeros_t<IT>(u, a.fixed) | setmask;
         if (ok[j]) v[j] = ld_nt(&a.sv[idx[j]]);
@@ -389,18 +404,19 @@ struct Diag1V4Args {
     float d0r, d0i, d1r, d1i;
 };
 
-__global__ __launch_bounds__(kBlock) void k_diag1_v4(Diag1V4Args a) {
-    const uint64_t base = (uint64_t)blockIdx.x * (kBlock * kUnroll) + threadIdx.x;
-    float4 v[kUnroll];
-    uint64_t idx[kUnroll];
+template <int U, int BS>
+__global__ __launch_bounds__(BS) void k_diag1_v4(Diag1V4Args a) {
+    const uint64_t base = (uint64_t)blockIdx.x * (BS * U) + threadIdx.x;
+    float4 v[U];
+    uint64_t idx[U];
 #pragma unroll
-    for (int j = 0; j < kUnroll; ++j) {
-        const uint64_t u = base + (uint64_t)j * kBlock;
+    for (int j = 0; j < U; ++j) {
+        const uint64_t u = base + (uint64_t)j * BS;
         idx[j] = (u < a.units) ? (insert_zeros(u, a.fixed) | a.cmask) : ~0ull;
         if (idx[j] != ~0ull) v[j] = ld_nt(&a.sv[idx[j]]);
     }
 #pragma unroll
-    for (int j = 0; j < kUnroll; ++j) {
+    for (int j = 0; j < U; ++j) {
         if (idx[j] == ~0ull) continue;
         float ar, ai, br, bi;
         if (a.tbit == 0) {
@@ -638,7 +654,12 @@ void apply_c64_fast_or_generic(aqs_state s, const aqs_op& op) {
                 } else {
                     a.units = s->N >> (1 + cb.cpos.size());
                     a.fixed = make_fixed(fp);
-                    hipLaunchKernelGGL(k_mat1_b0, dim3(blocks_for(a.units, kUnroll)), dim3(kBlock), 0, s->stream, a);
+                    if (one_wave_shape(a.units))
+                        hipLaunchKernelGGL((k_mat1_b0<1, 64>), dim3(blocks_for(a.units, 1, 64)), dim3(64), 0,
+                                           s->stream, a);
+                    else
+                        hipLaunchKernelGGL((k_mat1_b0<4, 256>), dim3(blocks_for(a.units, 4, 256)), dim3(256), 0,
+                                           s->stream, a);
                 }
             } else {
                 std::vector<int> fp = any ? std::vector<int>{} : cb.cpos;
@@ -661,7 +682,11 @@ void apply_c64_fast_or_generic(aqs_state s, const aqs_op& op) {
                 a.units = s->N >> (1 + cb.cpos.size());
                 a.tbit = t;
                 a.d0r = mf[0]; a.d0i = mf[1]; a.d1r = mf[6]; a.d1i = mf[7];
-                hipLaunchKernelGGL(k_diag1_v4, dim3(blocks_for(a.units, kUnroll)), dim3(kBlock), 0, s->stream, a);
+                if (one_wave_shape(a.units))
+                    hipLaunchKernelGGL((k_diag1_v4<1, 64>), dim3(blocks_for(a.units, 1, 64)), dim3(64), 0, s->stream, a);
+                else
+                    hipLaunchKernelGGL((k_diag1_v4<4, 256>), dim3(blocks_for(a.units, 4, 256)), dim3(256), 0,
+                                       s->stream, a);
             } else {
                 std::vector<int> fp = any ? std::vector<int>{} : cb.cpos;
                 fp.push_back(t);
@@ -697,12 +722,12 @@ void apply_c64_fast_or_generic(aqs_state s, const aqs_op& op) {
                 a.units = s->N >> (1 + high.size());
                 a.pr = mf[0];
                 a.pi = mf[1];
-                if (s->N / 2 <= (1ull << 32))
-                    hipLaunchKernelGGL((k_phase_lines<1, uint32_t>), dim3(blocks_for(a.units, 1)), dim3(kBlock), 0,
-                                       s->stream, a);
+                if (s->N / 2 <= (1ull << 32) && one_wave_shape(a.units))
+                    hipLaunchKernelGGL((k_phase_lines<1, uint32_t, 64>), dim3(blocks_for(a.units, 1, 64)), dim3(64),
+                                       0, s->stream, a);
                 else
-                    hipLaunchKernelGGL((k_phase_lines<2, uint64_t>), dim3(blocks_for(a.units, 2)), dim3(kBlock), 0,
-                                       s->stream, a);
+                    hipLaunchKernelGGL((k_phase_lines<4, uint64_t, 256>), dim3(blocks_for(a.units, 4, 256)),
+                                       dim3(256), 0, s->stream, a);
             } else if (all_ge1(fp) && n >= 2) {
                 PhaseArgs<float4, float> a{};
                 a.sv = (float4*)s->amps;
@@ -713,12 +738,12 @@ void apply_c64_fast_or_generic(aqs_state s, const aqs_op& op) {
                 a.units = s->N >> (1 + fp.size());
                 a.pr = mf[0];
                 a.pi = mf[1];
-                if (s->N / 2 <= (1ull << 32))
-                    hipLaunchKernelGGL((k_phase_v4<1, uint32_t>), dim3(blocks_for(a.units, 1)), dim3(kBlock), 0,
+                if (s->N / 2 <= (1ull << 32) && one_wave_shape(a.units))
+                    hipLaunchKernelGGL((k_phase_v4<1, uint32_t, 64>), dim3(blocks_for(a.units, 1, 64)), dim3(64), 0,
                                        s->stream, a);
                 else
-                    hipLaunchKernelGGL((k_phase_v4<2, uint64_t>), dim3(blocks_for(a.units, 2)), dim3(kBlock), 0,
-                                       s->stream, a);
+                    hipLaunchKernelGGL((k_phase_v4<4, uint64_t, 256>), dim3(blocks_for(a.units, 4, 256)), dim3(256),
+                                       0, s->stream, a);
             } else {
                 PhaseArgs<float2, float> a{};
                 a.sv = (float2*)s->amps;
@@ -746,7 +771,7 @@ void apply_c64_fast_or_generic(aqs_state s, const aqs_op& op) {
                     a.fixed = make_fixed(fp);
                     a.partner = 1ull << (t - 1);
                     a.units = s->N >> (2 + cb.cpos.size());
-                    hipLaunchKernelGGL(k_xchg_v4, dim3(blocks_for(a.units, kUnroll)), dim3(kBlock), 0, s->stream, a);
+                    launch_xchg_v4(a, s->stream);
                 } else {
                     a.fixed = make_fixed(fp);
                     a.units = s->N >> (1 + cb.cpos.size());
@@ -777,7 +802,7 @@ void apply_c64_fast_or_generic(aqs_state s, const aqs_op& op) {
                 a.setmask = set >> 1;
                 a.partner = partner >> 1;
                 a.units = s->N >> (1 + fp.size());
-                hipLaunchKernelGGL(k_xchg_v4, dim3(blocks_for(a.units, kUnroll)), dim3(kBlock), 0, s->stream, a);
+                launch_xchg_v4(a, s->stream);
             } else {
                 launch_pair<float2, float>(s, 2, op, fp, set, partner, any ? cb.cmask : 0, s->N >> fp.size());
             }

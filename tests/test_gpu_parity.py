@@ -517,6 +517,31 @@ def test_every_launch_shape(n):
     st.close()
 
 
+def test_33_qubit_launch_sizes():
+    """A 33-qubit complex64 state (64 GiB): 2^31..2^32 work units per pass,
+    where the one-wave launch shapes give way to 64-bit multi-unit shapes.
+    H on bits 0 and 1, CZ on them (a low-bit phase pass over 2^32 float4s),
+    RotZ on bit 32 and X on bit 27 from |0..0> leave four nonzero
+    amplitudes, known in closed form."""
+    n = 33
+    st = hip.State(n, hip.C64)
+    st.set_basis(0)
+    theta = 0.6
+    gates = [oracle.gate("H", 32), oracle.gate("H", 31), oracle.gate("CZ", 31, 32),
+             oracle.gate("RotZ", 0, angle=theta), oracle.gate("X", 5)]
+    st.apply(ops.lower_all(gates))
+    c, s = math.cos(np.float32(theta) / 2), math.sin(np.float32(theta) / 2)
+    rz0 = complex(c, -s)
+    base = 1 << 27
+    for low, sign in ((0, 1), (1, 1), (2, 1), (3, -1)):
+        got = st.amplitude(base | low)
+        assert abs(got - 0.5 * sign * rz0) < 1e-6, (low, got)
+    for idx in (0, 3, base | 4, (1 << 32) | base | 3, (1 << 33) - 1):
+        assert abs(st.amplitude(idx)) == 0.0, idx
+    assert abs(st.norm2() - 1.0) < 1e-6
+    st.close()
+
+
 @pytest.mark.parametrize("n", [26])
 def test_large_state_round_trip(n):
     """Size-independent properties at a large state: a brickwork layer

@@ -138,13 +138,44 @@ __global__ __launch_bounds__(BS) void k_mat1_v4(Mat1V4Args a) {
     }
 }
 
+// Uncontrolled MAT1 whose partner float4 is 1..32 float4s away (target
+// bits 1..6): every wave streams 64 consecutive float4s and takes each
+// partner from lane (lane ^ stride), so loads and stores are plain 1 KiB
+// runs instead of interleaved pairs.
+template <typename IT>
+__global__ __launch_bounds__(64) void k_mat1_lx(Mat1V4Args a) {
+    const IT f = (IT)blockIdx.x * (IT)64 + (IT)threadIdx.x;
+    const int s = (int)a.stride;
+    const float4 x = ld_nt(&a.sv[f]);
+    float4 y;
+    y.x = __shfl_xor(x.x, s);
+    y.y = __shfl_xor(x.y, s);
+    y.z = __shfl_xor(x.z, s);
+    y.w = __shfl_xor(x.w, s);
+    // lower member: m00 x + m01 y; upper member: m10 y + m11 x
+    const bool upper = (threadIdx.x & (unsigned)s) != 0u;
+    const float ar = upper ? a.m[6] : a.m[0], ai = upper ? a.m[7] : a.m[1];
+    const float br = upper ? a.m[4] : a.m[2], bi = upper ? a.m[5] : a.m[3];
+    float4 n;
+    n.x = ar * x.x - ai * x.y + br * y.x - bi * y.y;
+    n.y = ar * x.y + ai * x.x + br * y.y + bi * y.x;
+    n.z = ar * x.z - ai * x.w + br * y.z - bi * y.w;
+    n.w = ar * x.w + ai * x.z + br * y.w + bi * y.z;
+    st_nt(n, &a.sv[f]);
+}
+
 // Launch-shape variants of the float4 MAT1 pass: (units per thread, index
 // type, block size) = 0 (1, 32-bit, 256), 1 (2, 64-bit, 256), 2 (4, 64-bit,
-// 256), 3 (2, 32-bit, 256), 4 (1, 32-bit, 64), 5 (2, 32-bit, 64).
-constexpr int kMat1Variants = 6;
+// 256), 3 (2, 32-bit, 256), 4 (1, 32-bit, 64), 5 (2, 32-bit, 64); 6 = the
+// lane-exchange pass (uncontrolled, partner 1..32 float4s away, 2^6..2^31
+// float4s; otherwise variant 4 or 1).
+constexpr int kMat1Variants = 7;
 
 void launch_mat1_v4(const Mat1V4Args& a, int variant, uint64_t n_float4, hipStream_t st) {
     const bool fits32 = n_float4 <= (1ull << 32);
+    if (variant == 6 && !(a.cmask == 0 && a.fixed.n == 1 && a.stride <= 32 && n_float4 >= 64 &&
+                          n_float4 <= (1ull << 31) && 2 * a.units == n_float4))
+        variant = 4;
     if (!fits32 && variant != 1 && variant != 2) variant = 1;
     switch (variant) {
         case 0:
@@ -161,6 +192,9 @@ void launch_mat1_v4(const Mat1V4Args& a, int variant, uint64_t n_float4, hipStre
             break;
         case 4:
             hipLaunchKernelGGL((k_mat1_v4<1, uint32_t, 64>), dim3(blocks_for(a.units, 1, 64)), dim3(64), 0, st, a);
+            break;
+        case 6:
+            hipLaunchKernelGGL((k_mat1_lx<uint32_t>), dim3((unsigned)(n_float4 / 64)), dim3(64), 0, st, a);
             break;
         default:
             hipLaunchKernelGGL((k_mat1_v4<2, uint32_t, 64>), dim3(blocks_for(a.units, 2, 64)), dim3(64), 0, st, a);

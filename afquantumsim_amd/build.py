@@ -35,7 +35,7 @@ HIP_FLAGS = [
 
 HIP_SOURCES = ["state.hip", "gates.hip", "mfma.hip", "diag.hip", "perm.hip", "tile.hip", "measure.hip", "batch.hip",
                "fusion.cpp", "tiling.cpp", "serve.cpp"]
-CPP_SOURCES = ["quantum.cpp", "quantum_algo.cpp", "quantum_gates.cpp"]
+CPP_SOURCES = ["quantum.cpp", "quantum_algo.cpp", "quantum_gates.cpp"]  # algo_bench.cpp is a program
 
 
 def _run(cmd):
@@ -92,6 +92,21 @@ def build_cpp_tests() -> str:
     return CPP_TEST_BIN
 
 
+ALGO_BENCH_SRC = os.path.join(CPPSRC, "algo_bench.cpp")
+ALGO_BENCH_BIN = os.path.join(PKG, "algo_bench")
+
+
+def build_algo_bench() -> str:
+    """The reference's published algorithm benchmarks over the aqs:: API (run by bench.py)."""
+    if not os.path.exists(ALGO_BENCH_SRC):
+        return ""
+    tmp = ALGO_BENCH_BIN + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", INCLUDE, "-o", tmp, ALGO_BENCH_SRC, "-L", PKG, "-laqs",
+          "-laqs_hip", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, ALGO_BENCH_BIN)
+    return ALGO_BENCH_BIN
+
+
 TILE_EMU_SRC = os.path.join(ROOT, "tests", "cpp", "tile_emu.cpp")
 TILE_EMU_LIB = os.path.join(ROOT, "tests", "cpp", "libtile_emu.so")
 
@@ -113,6 +128,7 @@ def build_all() -> None:
     build_hip()
     build_cpp()
     build_cpp_tests()
+    build_algo_bench()
     build_test_helpers()
 
 

@@ -260,7 +260,8 @@ QCircuit::QCircuit(const QCircuit& o)
       cached_index_{o.cached_index_},
       fusion_width_{o.fusion_width_},
       compiled_{o.compiled_},
-      matrix_{o.matrix_} {}
+      matrix_{o.matrix_},
+      unitary64_{o.unitary64_} {}
 
 QCircuit::QCircuit(QCircuit&& o) noexcept
     : gate_list_{std::move(o.gate_list_)},
@@ -269,7 +270,8 @@ QCircuit::QCircuit(QCircuit&& o) noexcept
       cached_index_{o.cached_index_},
       fusion_width_{o.fusion_width_},
       compiled_{std::move(o.compiled_)},
-      matrix_{std::move(o.matrix_)} {}
+      matrix_{std::move(o.matrix_)},
+      unitary64_{std::move(o.unitary64_)} {}
 
 QCircuit& QCircuit::operator=(const QCircuit& o) {
     if (this != &o) {
@@ -280,6 +282,7 @@ QCircuit& QCircuit::operator=(const QCircuit& o) {
         fusion_width_ = o.fusion_width_;
         compiled_ = o.compiled_;
         matrix_ = o.matrix_;
+        unitary64_ = o.unitary64_;
         sink_ = nullptr;
     }
     return *this;
@@ -294,6 +297,7 @@ QCircuit& QCircuit::operator=(QCircuit&& o) noexcept {
         fusion_width_ = o.fusion_width_;
         compiled_ = std::move(o.compiled_);
         matrix_ = std::move(o.matrix_);
+        unitary64_ = std::move(o.unitary64_);
         sink_ = nullptr;
     }
     return *this;
@@ -310,6 +314,7 @@ void QCircuit::clear_cache() {
     cached_index_ = 0;
     compiled_.clear();
     matrix_.reset();
+    unitary64_.reset();
 }
 
 bool operator==(const QCircuit& lhs, const QCircuit& rhs) {
@@ -336,6 +341,7 @@ void QCircuit::emit(const aqs_op& op) {
     if (compiled_.empty() || compiled_.back()->sch) compiled_.push_back(std::make_shared<detail::Segment>());
     compiled_.back()->ops.push_back(detail::own(op));
     matrix_.reset();
+    unitary64_.reset();
 }
 
 void QCircuit::compile() {
@@ -365,6 +371,7 @@ void QCircuit::compile() {
     if (seg->sch) compiled_.push_back(seg);
     cached_index_ = gate_list_.size();
     matrix_.reset();
+    unitary64_.reset();
 }
 
 size_t QCircuit::compiled_passes() const {
@@ -1098,6 +1105,7 @@ namespace detail {
 struct Access {
     static void set_sink(QCircuit& c, Sink* s) { c.sink_ = s; }
     static void run_compiled(const QCircuit& c, aqs_state s) { c.run_compiled(s); }
+    static std::shared_ptr<const std::vector<double>>& unitary_cache(const QCircuit& c) { return c.unitary64_; }
 };
 }  // namespace detail
 
@@ -1112,11 +1120,14 @@ void emit_flattened(QCircuit& qc, const QCircuit& internal, uint32_t offset, int
         aqs_op o = op;
         for (int j = 0; j < o.ntargets; ++j) o.targets[j] += (int)offset;
         for (int j = 0; j < o.ncontrols; ++j) o.controls[j] += (int)offset;
+        o.more_controls <<= offset;
         if (control >= 0) {
-            if (o.ncontrols >= 3) throw std::invalid_argument{"controlled sub-circuit gate exceeds three controls"};
             if (o.flags & AQS_CTRL_ANY)
                 throw std::invalid_argument{"an Or gate cannot take an extra control in a wide ControlGate"};
-            o.controls[o.ncontrols++] = control;
+            if (o.ncontrols < 3)
+                o.controls[o.ncontrols++] = control;
+            else
+                o.more_controls |= 1ull << control;  // the fourth control and beyond
         }
         qc.emit(o);
     };
@@ -1160,7 +1171,9 @@ void emit_subcircuit(QCircuit& qc, QCircuit& internal, uint32_t begin, int contr
     internal.compile();
     const uint32_t k = internal.qubit_count();
     if (k <= 5) {
-        std::vector<double> u = unitary64(internal);
+        auto& cached = detail::Access::unitary_cache(internal);
+        if (!cached) cached = std::make_shared<const std::vector<double>>(unitary64(internal));
+        const std::vector<double>& u = *cached;
         aqs_op op{};
         op.kind = AQS_OP_MATK;
         op.ntargets = (int)k;

@@ -334,7 +334,8 @@ aqs_status aqs_batch_create(int count, const int* nqubits, const aqs_op* ops, co
                     const aqs_op& op = ops[i];
                     validate_op(op, n);
                     if (op.kind == AQS_OP_MATK) fail(AQS_ERR_INVALID_ARGUMENT, "batched states take no MATK ops");
-                    const bool one_q = (op.kind == AQS_OP_MAT1 || op.kind == AQS_OP_DIAG1) && op.ncontrols == 0;
+                    const bool one_q =
+                        (op.kind == AQS_OP_MAT1 || op.kind == AQS_OP_DIAG1) && op.ncontrols == 0 && !op.more_controls;
                     if (one_q) {
                         const int q = op.targets[0];
                         M2 m = mat_of(op);
@@ -348,12 +349,16 @@ aqs_status aqs_batch_create(int count, const int* nqubits, const aqs_op* ops, co
                     }
                     for (int j = 0; j < op.ntargets; ++j) flush(op.targets[j]);
                     for (int j = 0; j < op.ncontrols; ++j) flush(op.controls[j]);
+                    for (int q = 0; q < n; ++q)
+                        if ((op.more_controls >> q) & 1ull) flush(q);
                     BOp o{};
                     o.kind = op.kind;
                     o.t0 = n - 1 - op.targets[0];
                     o.t1 = op.ntargets > 1 ? n - 1 - op.targets[1] : 0;
                     o.any = (op.flags & AQS_CTRL_ANY) ? 1 : 0;
                     for (int j = 0; j < op.ncontrols; ++j) o.cmask |= 1u << (n - 1 - op.controls[j]);
+                    for (int q = 0; q < n; ++q)
+                        if ((op.more_controls >> q) & 1ull) o.cmask |= 1u << (n - 1 - q);
                     if (op.kind == AQS_OP_PHASE) {
                         o.pmask = o.cmask;
                         for (int j = 0; j < op.ntargets; ++j) o.pmask |= 1u << (n - 1 - op.targets[j]);

@@ -126,7 +126,17 @@ std::vector<int> op_qubits(const aqs_op& op) {
     for (int j = 0; j < op.ntargets; ++j) q.push_back(op.targets[j]);
     for (int j = 0; j < op.ncontrols; ++j)
         if (std::find(q.begin(), q.end(), op.controls[j]) == q.end()) q.push_back(op.controls[j]);
+    for (int b = 0; b < 64; ++b)
+        if ((op.more_controls >> b) & 1ull) q.push_back(b);
     return q;
+}
+
+// A qubit mask with every qubit q moved to L[q].
+uint64_t relabel_mask(uint64_t mask, const std::vector<int>& L) {
+    uint64_t r = 0;
+    for (size_t q = 0; q < L.size(); ++q)
+        if ((mask >> q) & 1ull) r |= 1ull << L[q];
+    return r;
 }
 
 bool is_diagonal(const aqs_op& op) {
@@ -223,6 +233,7 @@ class Fuser {
             }
             for (int j = 0; j < op.ntargets; ++j) op.targets[j] = L[op.targets[j]];
             for (int j = 0; j < op.ncontrols; ++j) op.controls[j] = L[op.controls[j]];
+            op.more_controls = relabel_mask(op.more_controls, L);
             ops.push_back(op);
         }
         count = ops.size();
@@ -390,7 +401,7 @@ class Fuser {
                 if (std::find(S.begin(), S.end(), q) != S.end()) need = true;
         if (need) flush_terms();
         auto hit = touching(S);
-        if ((int)S.size() > kmax_) {
+        if ((int)S.size() > kmax_ || op.more_controls) {
             close_touching(S);
             raw(op);
             return;
@@ -414,6 +425,11 @@ class Fuser {
 
     void diag(const aqs_op& op) {
         std::vector<int> S = op_qubits(op);
+        if (op.more_controls) {  // many-controlled: its own pass
+            close_touching(S);
+            raw(op);
+            return;
+        }
         auto hit = touching(S);
         std::vector<int> U = union_with(S, hit);
         if ((int)U.size() <= kmax_) {

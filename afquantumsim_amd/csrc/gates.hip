@@ -249,6 +249,7 @@ struct PairArgs {
     uint64_t cmask;   // bits forced to 1
     uint64_t tmask;   // partner offset (target bit, or pa|pb for SWAP)
     uint64_t anymask; // AQS_CTRL_ANY: at least one of these bits must be 1 (0 = unused)
+    uint64_t reqmask; // further controls: all of these bits must be 1 (0 = unused)
     R m[8];
 };
 
@@ -259,6 +260,7 @@ __global__ __launch_bounds__(kBlock) void k_pair(PairArgs<T2, R> a) {
     for (uint64_t u = (uint64_t)blockIdx.x * kBlock + threadIdx.x; u < a.units; u += stride) {
         const uint64_t i0 = insert_zeros(u, a.fixed) | a.cmask;
         if (a.anymask && !(i0 & a.anymask)) continue;
+        if ((i0 & a.reqmask) != a.reqmask) continue;
         const uint64_t i1 = i0 ^ a.tmask;
         T2 x = a.sv[i0], y = a.sv[i1];
         if (OP == 0) {
@@ -339,6 +341,7 @@ struct PhaseArgs {
     uint64_t units;
     Fixed fixed;
     uint64_t setmask;
+    uint64_t reqmask;  // generic kernel: further controls, all must be 1 (0 = unused)
     R pr, pi;
 };
 
@@ -347,6 +350,7 @@ __global__ __launch_bounds__(kBlock) void k_phase(PhaseArgs<T2, R> a) {
     const uint64_t stride = (uint64_t)gridDim.x * kBlock;
     for (uint64_t u = (uint64_t)blockIdx.x * kBlock + threadIdx.x; u < a.units; u += stride) {
         const uint64_t i = insert_zeros(u, a.fixed) | a.setmask;
+        if ((i & a.reqmask) != a.reqmask) continue;
         T2 x = a.sv[i], y;
         y.x = a.pr * x.x - a.pi * x.y;
         y.y = a.pr * x.y + a.pi * x.x;
@@ -479,6 +483,7 @@ struct MatkArgs {
     uint64_t units;
     Fixed fixed;
     uint64_t cmask;
+    uint64_t reqmask;  // further controls, all must be 1 (0 = unused)
     uint64_t off[K];   // amplitude offset of sub-index r
     const R* u;        // K x K complex, row-major, interleaved
 };
@@ -491,6 +496,7 @@ __global__ __launch_bounds__(kBlock) void k_matk(MatkArgs<T2, R, K> a) {
     const uint64_t stride = (uint64_t)gridDim.x * kBlock;
     for (uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x; t < a.units; t += stride) {
         const uint64_t base = insert_zeros(t, a.fixed) | a.cmask;
+        if ((base & a.reqmask) != a.reqmask) continue;
         R xr[K], xi[K];
 #pragma unroll
         for (int r = 0; r < K; ++r) {
@@ -530,7 +536,7 @@ unsigned grid_stride_blocks(uint64_t units) {
 
 template <typename T2, typename R>
 void launch_pair(aqs_state s, int op_code, const aqs_op& op, std::vector<int> fixed_pos, uint64_t cmask,
-                 uint64_t tmask, uint64_t anymask, uint64_t units) {
+                 uint64_t tmask, uint64_t anymask, uint64_t units, uint64_t reqmask = 0) {
     PairArgs<T2, R> a{};
     a.sv = (T2*)s->amps;
     a.units = units;
@@ -538,6 +544,7 @@ void launch_pair(aqs_state s, int op_code, const aqs_op& op, std::vector<int> fi
     a.cmask = cmask;
     a.tmask = tmask;
     a.anymask = anymask;
+    a.reqmask = reqmask;
     for (int i = 0; i < 8; ++i) a.m[i] = (R)op.m[i];
     dim3 g(grid_stride_blocks(units)), b(kBlock);
     switch (op_code) {
@@ -548,11 +555,20 @@ void launch_pair(aqs_state s, int op_code, const aqs_op& op, std::vector<int> fi
     }
 }
 
+// Amplitude-index bits of an op's further controls (more_controls).
+uint64_t more_control_bits(int n, const aqs_op& op) {
+    uint64_t r = 0;
+    for (int q = 0; q < n; ++q)
+        if ((op.more_controls >> q) & 1ull) r |= qbit(n, q);
+    return r;
+}
+
 template <typename T2, typename R, int K>
 void launch_matk(aqs_state s, const aqs_op& op, const void* dev_u) {
     const int n = s->n, k = op.ntargets;
     MatkArgs<T2, R, K> a{};
     a.sv = (T2*)s->amps;
+    a.reqmask = more_control_bits(n, op);
     std::vector<int> fp;
     for (int j = 0; j < k; ++j) fp.push_back(qpos(n, op.targets[j]));
     uint64_t cmask = 0;
@@ -636,6 +652,12 @@ void validate_op(const aqs_op& op, int n) {
         if (seen & b) fail(AQS_ERR_INVALID_ARGUMENT, "Control qubit cannot be the same as the target qubit");
         if (cseen & b) fail(AQS_ERR_INVALID_ARGUMENT, "repeated control qubit");
         cseen |= b;
+    }
+    if (op.more_controls) {
+        if (n < 64 && (op.more_controls >> n)) fail(AQS_ERR_OUT_OF_RANGE, "Cannot add gate at the given qubit position");
+        if (op.more_controls & seen) fail(AQS_ERR_INVALID_ARGUMENT, "Control qubit cannot be the same as the target qubit");
+        if (op.more_controls & cseen) fail(AQS_ERR_INVALID_ARGUMENT, "repeated control qubit");
+        if (op.flags & AQS_CTRL_ANY) fail(AQS_ERR_INVALID_ARGUMENT, "further controls cannot take AQS_CTRL_ANY");
     }
 }
 
@@ -900,6 +922,60 @@ void apply_c128(aqs_state s, const aqs_op& op) {
     AQS_HIP(hipGetLastError());
 }
 
+// Ops with further controls (more_controls): the generic kernels over the
+// listed controls and targets, skipping indices that lack a further control.
+template <typename T2, typename R>
+void apply_masked_t(aqs_state s, const aqs_op& op) {
+    const int n = s->n;
+    OpBits cb = control_bits(n, op);
+    const uint64_t req = more_control_bits(n, op);
+    ProfScope ps(s, op.kind);
+    switch (op.kind) {
+        case AQS_OP_MAT1:
+        case AQS_OP_DIAG1:
+        case AQS_OP_X: {
+            const int t = qpos(n, op.targets[0]);
+            std::vector<int> fp = cb.cpos;
+            fp.push_back(t);
+            const int code = op.kind == AQS_OP_MAT1 ? 0 : (op.kind == AQS_OP_X ? 1 : 3);
+            launch_pair<T2, R>(s, code, op, fp, cb.cmask, 1ull << t, 0, s->N >> fp.size(), req);
+            break;
+        }
+        case AQS_OP_SWAP: {
+            const int pa = qpos(n, op.targets[0]), pb = qpos(n, op.targets[1]);
+            std::vector<int> fp = cb.cpos;
+            fp.push_back(pa);
+            fp.push_back(pb);
+            launch_pair<T2, R>(s, 2, op, fp, cb.cmask | (1ull << pa), (1ull << pa) | (1ull << pb), 0,
+                               s->N >> fp.size(), req);
+            break;
+        }
+        case AQS_OP_PHASE: {
+            std::vector<int> fp = cb.cpos;
+            uint64_t mask = cb.cmask;
+            for (int j = 0; j < op.ntargets; ++j) {
+                const uint64_t m = qbit(n, op.targets[j]);
+                if (mask & m) continue;
+                mask |= m;
+                fp.push_back(qpos(n, op.targets[j]));
+            }
+            PhaseArgs<T2, R> a{};
+            a.sv = (T2*)s->amps;
+            a.fixed = make_fixed(fp);
+            a.setmask = mask;
+            a.reqmask = req;
+            a.units = s->N >> fp.size();
+            a.pr = (R)op.m[0];
+            a.pi = (R)op.m[1];
+            hipLaunchKernelGGL((k_phase<T2, R>), dim3(grid_stride_blocks(a.units)), dim3(kBlock), 0, s->stream, a);
+            break;
+        }
+        default:
+            break;
+    }
+    AQS_HIP(hipGetLastError());
+}
+
 }  // namespace
 
 namespace aqs_hip {
@@ -911,7 +987,7 @@ void apply_op(aqs_state s, const aqs_op& op) {
         const size_t K = (size_t)1 << k;
         if (op.flags & AQS_CTRL_ANY) fail(AQS_ERR_INVALID_ARGUMENT, "MATK does not take AQS_CTRL_ANY");
         if (s->precision == AQS_C64) {
-            if (k >= 4 && op.ncontrols == 0) {
+            if (k >= 4 && op.ncontrols == 0 && !op.more_controls) {
                 std::vector<float> pk = pack_mfma_operand(k, op.u);
                 ensure_staging(s, pk.size() * sizeof(float));
                 AQS_HIP(hipMemcpyAsync(s->staging, pk.data(), pk.size() * sizeof(float), hipMemcpyHostToDevice,
@@ -931,6 +1007,13 @@ void apply_op(aqs_state s, const aqs_op& op) {
             dispatch_matk<double2, double>(s, op, s->staging);
         }
         AQS_HIP(hipGetLastError());
+        return;
+    }
+    if (op.more_controls) {
+        if (s->precision == AQS_C64)
+            apply_masked_t<float2, float>(s, op);
+        else
+            apply_masked_t<double2, double>(s, op);
         return;
     }
     if (s->precision == AQS_C64)

@@ -282,7 +282,7 @@ std::vector<float> pack_mfma_operand(int k, const double* u) {
 bool apply_matk_mfma(aqs_state s, const aqs_op& op, const float* dev_apack, const DiagEpilogueDesc* epi,
                      bool before) {
     const int k = op.ntargets;
-    if (s->precision != AQS_C64 || op.ncontrols != 0 || (k != 4 && k != 5)) return false;
+    if (s->precision != AQS_C64 || op.ncontrols != 0 || op.more_controls || (k != 4 && k != 5)) return false;
     if (s->n < k + 5 || s->n > 36) return false;
     const int n = s->n;
     // target bit positions (op order: targets[0] = most significant sub-index bit)

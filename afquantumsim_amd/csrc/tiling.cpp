@@ -75,6 +75,10 @@ class Tiler {
             }
             for (int j = 0; j < op.ntargets; ++j) op.targets[j] = L[op.targets[j]];
             for (int j = 0; j < op.ncontrols; ++j) op.controls[j] = L[op.controls[j]];
+            uint64_t more = 0;
+            for (int q = 0; q < n_; ++q)
+                if ((op.more_controls >> q) & 1ull) more |= 1ull << L[q];
+            op.more_controls = more;
             lower(op);
         }
         std::vector<char> done(g_.size(), 0);
@@ -104,6 +108,8 @@ class Tiler {
         o.op = op;
         o.any = (op.flags & AQS_CTRL_ANY) != 0;
         for (int j = 0; j < op.ncontrols; ++j) o.ctrl.push_back(bit(op.controls[j]));
+        for (int q = 0; q < n_; ++q)
+            if ((op.more_controls >> q) & 1ull) o.ctrl.push_back(bit(q));
         switch (op.kind) {
             case AQS_OP_MAT1:
             case AQS_OP_X:
@@ -148,6 +154,7 @@ class Tiler {
                     t.targets[0] = b;
                     t.ncontrols = 0;
                     for (int j = 0; j < op.ncontrols; ++j) t.controls[t.ncontrols++] = op.controls[j];
+                    t.more_controls = op.more_controls;
                     lower_wide_x(t, a);
                     lower(x);
                     return;
@@ -175,6 +182,8 @@ class Tiler {
         o.tbit = bit(t.targets[0]);
         o.nd = {o.tbit};
         for (int j = 0; j < t.ncontrols; ++j) o.ctrl.push_back(bit(t.controls[j]));
+        for (int q = 0; q < n_; ++q)
+            if ((t.more_controls >> q) & 1ull) o.ctrl.push_back(bit(q));
         o.ctrl.push_back(bit(extra));
         o.dg = o.ctrl;
         g_.push_back(std::move(o));
@@ -524,6 +533,7 @@ double pass_cost(const aqs_schedule_s* sch, size_t p) {
         return c;
     }
     if (sch->diag_index[p] >= 0) return 1.25;
+    if (op.more_controls) return 0.4;  // a small fraction of the state, found by index tests
     switch (op.kind) {
         case AQS_OP_PERMB:
             return 1.8;
@@ -548,7 +558,8 @@ std::vector<int> mfma_tile_bits(const aqs_schedule_s* sch, size_t p) {
     if (p >= sch->passes.size()) return {};
     const aqs_op& op = sch->passes[p];
     const int n = sch->n;
-    if (op.kind != AQS_OP_MATK || op.ncontrols || (op.ntargets != 4 && op.ntargets != 5) || n < op.ntargets + 5)
+    if (op.kind != AQS_OP_MATK || op.ncontrols || op.more_controls || (op.ntargets != 4 && op.ntargets != 5) ||
+        n < op.ntargets + 5)
         return {};
     std::vector<int> T;
     for (int j = 0; j < op.ntargets; ++j) T.push_back(n - 1 - op.targets[j]);

@@ -70,6 +70,7 @@ class Op(ctypes.Structure):
         ("flags", ctypes.c_int32),
         ("m", ctypes.c_double * 8),
         ("u", ctypes.c_void_p),
+        ("more_controls", ctypes.c_uint64),
     ]
 
 
@@ -532,10 +533,14 @@ def pack_ops(ops):
         x.ntargets = len(t)
         for j, q in enumerate(t):
             x.targets[j] = q
-        c = o.get("controls", ())
-        x.ncontrols = len(c)
-        for j, q in enumerate(c):
+        c = list(o.get("controls", ()))
+        x.ncontrols = min(3, len(c))
+        for j, q in enumerate(c[:3]):
             x.controls[j] = q
+        more = int(o.get("more_controls", 0))
+        for q in c[3:]:  # the fourth control and beyond: a qubit mask
+            more |= 1 << q
+        x.more_controls = more
         x.flags = o.get("flags", 0)
         m = o.get("m")
         if m is not None:

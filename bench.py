@@ -191,6 +191,32 @@ def random20(args, n=20, depth=40, shots=1000):
     return out
 
 
+# The reference's published algorithm timings (benchmark/results.md:33-35:
+# QFT 10 qubits, Grover 10 qubits, Shor N = 15 on 12 qubits; build +
+# simulate per repetition, mean of 100, ArrayFire 3.8.2 OpenCL on a Radeon
+# Pro 5500M).
+REFERENCE_ALGO_MS = {"qft10": 6.05, "grover10": 192.0, "shor12": 106.0}
+
+
+def reference_algorithms(args):
+    """The reference's own benchmark program (benchmark/benchmark.cpp)
+    restated over this library's aqs:: C++ API (afquantumsim_amd/algo_bench,
+    a child process), against the published numbers."""
+    import subprocess
+    exe = os.path.join(ROOT, "afquantumsim_amd", "algo_bench")
+    if not os.path.exists(exe):
+        raise RuntimeError(f"{exe} is missing: build it with `python -m afquantumsim_amd.build`")
+    r = subprocess.run([exe, str(args.algo_reps)], capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError(f"algo_bench failed ({r.returncode}): {r.stderr[-2000:]}")
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    res["reference_published_ms"] = REFERENCE_ALGO_MS
+    res["speedup_vs_published"] = {k: round(REFERENCE_ALGO_MS[k] / res["uncompiled"][f"{k}_ms"][0], 1)
+                                   for k in REFERENCE_ALGO_MS}
+    log(f"reference algorithms: {res}")
+    return res
+
+
 def qft30(args, n=30, seed=30, inverse=False):
     """30-qubit QFT (fourier_transform + final swaps, or
     inverse_fourier_transform) on a seeded Gaussian complex64 state
@@ -471,6 +497,7 @@ def run(args):
     qft = qft30(args) if not args.no_qft else None
     iqft = qft30(args, inverse=True) if not args.no_qft else None
     serve = served(args) if args.serve_requests > 0 else None
+    algos = reference_algorithms(args) if args.algo_reps > 0 else None
 
     cpu = cpu_baseline(gates, n) if not args.no_cpu else None
     if cpu:
@@ -501,6 +528,7 @@ def run(args):
         "qft30": qft,
         "iqft30": iqft,
         "served": serve,
+        "reference_algorithms": algos,
         "cpu_baseline": cpu,
         "device": dev,
     }
@@ -517,6 +545,8 @@ def main():
     p.add_argument("--no-qft", action="store_true", help="skip the 30-qubit QFT section")
     p.add_argument("--serve-requests", type=int, default=120,
                    help="requests in the served-stream section (0 skips it)")
+    p.add_argument("--algo-reps", type=int, default=100,
+                   help="repetitions of the reference's algorithm benchmarks (0 skips them)")
     p.add_argument("--probe", action="store_true",
                    help="run a short fixed kernel sequence for hardware-counter collection and exit")
     args = p.parse_args()

@@ -201,6 +201,9 @@ class QCircuit {
     std::vector<std::shared_ptr<detail::Segment>> compiled_;
     detail::Sink* sink_ = nullptr;
     mutable std::shared_ptr<CircuitMatrix> matrix_;
+    /* FP64 unitary of the compiled part (up to 5 qubits), computed once for
+     * Gate / ControlGate emission; dropped with matrix_. */
+    mutable std::shared_ptr<const std::vector<double>> unitary64_;
 
     void run_compiled(aqs_state s) const;
 };

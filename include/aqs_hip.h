@@ -93,6 +93,9 @@ typedef struct aqs_op {
     int32_t flags;       /* AQS_CTRL_ANY */
     double m[8];         /* 2x2 complex, row-major, interleaved */
     const double* u;     /* MATK: 2^k x 2^k complex, row-major, interleaved */
+    uint64_t more_controls; /* further controls as a qubit mask (bit q = qubit q), all of which must be 1,
+                             * for gates with more than 3 controls (the reference's NControl_Gate and nested
+                             * ControlGates, src/quantum_gates.cpp:80-114); 0 with AQS_CTRL_ANY */
 } aqs_op;
 
 typedef struct aqs_state_s* aqs_state;

@@ -787,6 +787,24 @@ static void test_grover() {
         c.simulate(it);
         CHECK(near(a.statevector().host(), c.statevector().host(), 1e-6f));
     }
+    // 10 qubits, 25 iterations (the reference benchmark's size): the
+    // 9-controlled Z of oracle and diffusion exceeds the three listed
+    // controls of an op and runs with the further-controls mask
+    {
+        const uint32_t n10 = 10, idx10 = (1u << 9) | (1u << 7);
+        const auto iters = static_cast<uint32_t>(pi * std::sqrt(1024.f) / 4.f);
+        CHECK(iters == 25);
+        QCircuit s10 = grover_search(n10, grover_oracle(n10, marked), iters);
+        const double th = std::asin(std::pow(2.0, -5.0));
+        const double expect10 = std::pow(std::sin((2 * iters + 1) * th), 2);
+        QSimulator q10(n10);
+        q10.simulate(s10);
+        CHECK(std::fabs(q10.state_probability(idx10) - expect10) < 1e-4);
+        s10.compile();
+        QSimulator q10c(n10);
+        q10c.simulate(s10);
+        CHECK(std::fabs(q10c.state_probability(idx10) - expect10) < 1e-4);
+    }
     // the reference benchmark's iteration count: floor(pi * sqrt(16) / 4) = 3
     QSimulator qs(n);
     qs.simulate(grover_search(n, oracle, 3));

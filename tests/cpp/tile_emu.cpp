@@ -28,6 +28,8 @@ void plain_apply(std::vector<cplx>& v, int n, const aqs_op& op) {
     auto bit = [&](int q) { return 1ull << (n - 1 - q); };
     uint64_t cmask = 0;
     for (int j = 0; j < op.ncontrols; ++j) cmask |= bit(op.controls[j]);
+    for (int q = 0; q < n; ++q)
+        if ((op.more_controls >> q) & 1ull) cmask |= bit(q);
     const bool any = (op.flags & AQS_CTRL_ANY) != 0;
     auto ok = [&](uint64_t i) { return !cmask || (any ? (i & cmask) != 0 : (i & cmask) == cmask); };
     const cplx m00(op.m[0], op.m[1]), m01(op.m[2], op.m[3]), m10(op.m[4], op.m[5]), m11(op.m[6], op.m[7]);

@@ -28,9 +28,15 @@ def test_hip_library_exports_every_declared_symbol():
 
 
 def test_op_struct_layout():
-    # int32 kind, ntargets, targets[5], ncontrols, controls[3], flags | double m[8] | pointer
-    assert ctypes.sizeof(hip.Op) == 12 * 4 + 8 * 8 + 8
-    assert hip.Op.m.offset == 48 and hip.Op.u.offset == 112
+    # int32 kind, ntargets, targets[5], ncontrols, controls[3], flags | double m[8] | pointer | uint64 mask
+    assert ctypes.sizeof(hip.Op) == 12 * 4 + 8 * 8 + 8 + 8
+    assert hip.Op.m.offset == 48 and hip.Op.u.offset == 112 and hip.Op.more_controls.offset == 120
+
+
+def test_more_than_three_controls_pack_into_the_mask():
+    arr, keep = hip.pack_ops([{"kind": hip.OP_X, "targets": (0,), "controls": (1, 2, 3, 5, 9)}])
+    assert arr[0].ncontrols == 3 and list(arr[0].controls) == [1, 2, 3]
+    assert arr[0].more_controls == (1 << 5) | (1 << 9)
 
 
 def test_missing_library_fails_loudly(tmp_path):

@@ -1088,6 +1088,21 @@ std::string named_representation(const std::string& name, uint32_t k, uint32_t b
     return b.str();
 }
 
+// Same gates by value (each gate's operator==), not only the same gate
+// objects as QCircuit's operator== requires.
+bool same_content(const QCircuit& a, const QCircuit& b) {
+    if (a.qubit_count() != b.qubit_count() || a.gate_list().size() != b.gate_list().size() ||
+        a.representation() != b.representation())
+        return false;
+    for (size_t i = 0; i < a.gate_list().size(); ++i)
+        if (!(*a.gate_list()[i] == *b.gate_list()[i])) return false;
+    return true;
+}
+
+// The circuit cache keyed by representation (src/quantum.cpp:1710-1724).  A
+// circuit rebuilt with the same content (NControl_Gate called again, as in
+// every Grover diffusion step) keeps the cached entry, and with it the
+// entry's compiled passes and unitary; other circuits replace the entry.
 std::shared_ptr<QCircuit> cache_circuit(const QCircuit& circuit) {
     auto it = g_cached_circuits.find(circuit.representation());
     if (it == g_cached_circuits.end()) {
@@ -1095,7 +1110,8 @@ std::shared_ptr<QCircuit> cache_circuit(const QCircuit& circuit) {
         if (!res.second) throw std::runtime_error{"Could not store circuit"};
         return res.first->second;
     }
-    if (!(circuit == *(it->second))) g_cached_circuits[circuit.representation()] = std::make_shared<QCircuit>(circuit);
+    if (!(circuit == *(it->second)) && !same_content(circuit, *(it->second)))
+        g_cached_circuits[circuit.representation()] = std::make_shared<QCircuit>(circuit);
     return g_cached_circuits.find(circuit.representation())->second;
 }
 

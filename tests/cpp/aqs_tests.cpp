@@ -829,6 +829,35 @@ static void test_grover() {
     CHECK(threw);
 }
 
+// The circuit cache keyed by representation: a rebuilt circuit with the same
+// gates by value shares the cached entry (and its compiled passes); circuits
+// whose representations collide but whose gates differ (a named inner Gate
+// over different circuits) still replace it, and each simulates its own gates.
+static void test_circuit_cache() {
+    clear_circuit_cache();
+    QCircuit a = NControl_Gate(6, 0, 5, 5, Z::gate());
+    QCircuit b = NControl_Gate(6, 0, 5, 5, Z::gate());
+    Gate ga{a, 0}, gb{b, 0};
+    CHECK(ga.internal_circuit == gb.internal_circuit);
+    QCircuit o1(3), o2(3);
+    o1 << X{0};
+    o2 << H{1};
+    QCircuit c1(3), c2(3);
+    c1 << Gate{o1, 0, "Orc"};
+    c2 << Gate{o2, 0, "Orc"};
+    CHECK(c1.representation() == c2.representation());
+    QCircuit w1(3), w2(3);
+    w1 << Gate{c1, 0};
+    w2 << Gate{c2, 0};
+    QSimulator s1(3), s2(3);
+    s1.simulate(w1);
+    s2.simulate(w2);
+    // X on qubit 0: |100>; H on qubit 1: (|000> + |010>) / sqrt 2
+    CHECK(fequal(s1.state(4).real(), 1.f, 1e-6f));
+    CHECK(fequal(s2.state(0).real(), 0.70710678f, 1e-6f) && fequal(s2.state(2).real(), 0.70710678f, 1e-6f));
+    clear_circuit_cache();
+}
+
 int main(int argc, char** argv) {
     initialize(argc, argv);
     seed(20240229u);
@@ -847,6 +876,7 @@ int main(int argc, char** argv) {
     test_errors();
     test_compile_equivalence();
     test_grover();
+    test_circuit_cache();
     std::cout << "ALL PASSED" << std::endl;
     return 0;
 }

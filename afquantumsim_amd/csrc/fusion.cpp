@@ -678,6 +678,24 @@ void emit_restore_order(aqs_schedule_s* sch, const std::vector<int>& L) {
 
 }  // namespace aqs_hip
 
+namespace {
+
+// The bit involution of pass p when p is a bit-permutation pass made of
+// exactly one involution, else null.
+const std::vector<int>* single_involution(const aqs_schedule_s* sch, size_t p) {
+    if (p >= sch->passes.size() || sch->passes[p].kind != AQS_OP_PERMB) return nullptr;
+    const std::vector<int>* beta = nullptr;
+    int count = 0;
+    for (const auto& pi : sch->perm_index)
+        if (pi.first == (int)p) {
+            beta = &sch->perms[pi.second];
+            ++count;
+        }
+    return count == 1 ? beta : nullptr;
+}
+
+}  // namespace
+
 extern "C" {
 
 aqs_status aqs_fuse(const aqs_op* ops, size_t count, int nqubits, int max_qubits, aqs_schedule* out) {
@@ -832,7 +850,8 @@ aqs_status aqs_apply_schedule(aqs_state s, aqs_schedule sch) {
             if (sch->diag_index[p] >= 0) {
                 const auto& dp = sch->diags[sch->diag_index[p]];
                 // or in the prologue of the next MFMA block when its anchor
-                // lies in that block's tile
+                // lies in that block's tile (with a restore permutation after
+                // the block folded into its store when the tile allows)
                 if (p + 1 < sch->passes.size() && sch->dev_mfma[p + 1] && sch->passes[p + 1].kind == AQS_OP_MATK &&
                     sch->tile_index[p + 1] < 0 && sch->diag_index[p + 1] < 0) {
                     DiagEpilogueDesc epi;
@@ -840,8 +859,13 @@ aqs_status aqs_apply_schedule(aqs_state s, aqs_schedule sch) {
                     epi.anchor = dp.anchor;
                     epi.chunks = dp.chunks;
                     epi.dev_tab = d;
-                    if (apply_matk_mfma(s, sch->passes[p + 1], (const float*)((const char*)sch->dev + sch->dev_off[p + 1]),
-                                        &epi, true)) {
+                    const float* apack = (const float*)((const char*)sch->dev + sch->dev_off[p + 1]);
+                    const std::vector<int>* beta = single_involution(sch, p + 2);
+                    if (beta && apply_matk_mfma(s, sch->passes[p + 1], apack, &epi, true, beta)) {
+                        p += 2;
+                        continue;
+                    }
+                    if (apply_matk_mfma(s, sch->passes[p + 1], apack, &epi, true)) {
                         ++p;
                         continue;
                     }
@@ -869,6 +893,14 @@ aqs_status aqs_apply_schedule(aqs_state s, aqs_schedule sch) {
                     epi.chunks = dp.chunks;
                     epi.dev_tab = (const char*)sch->dev + sch->dev_off[p + 1];
                     if (apply_matk_mfma(s, op, (const float*)d, &epi)) {
+                        ++p;
+                        continue;
+                    }
+                }
+                // a restore permutation right after the block: folded into
+                // its store when it maps the block's tile onto itself
+                if (const std::vector<int>* beta = single_involution(sch, p + 1)) {
+                    if (apply_matk_mfma(s, op, (const float*)d, nullptr, false, beta)) {
                         ++p;
                         continue;
                     }

@@ -139,9 +139,12 @@ struct DiagEpilogueDesc {
 // k-qubit dense block on MFMA (k = 4 or 5, complex64); returns false when
 // the op shape is not supported by the MFMA path, or when `epi` is given
 // and its anchor bits do not lie in the block's tile.  `epi` is applied
-// after the block, or before it when `before` is set.
+// after the block, or before it when `before` is set.  `beta` (a bit
+// involution over amplitude bit positions) is applied after the block and
+// any table before it, when it maps the block's tile bits onto themselves
+// and no table follows; otherwise the call returns false.
 bool apply_matk_mfma(aqs_state s, const aqs_op& op, const float* dev_apack, const DiagEpilogueDesc* epi = nullptr,
-                     bool before = false);
+                     bool before = false, const std::vector<int>* beta = nullptr);
 // MATK with the matrix already on the device (FP32 for complex64, FP64 for
 // complex128, plain row-major).
 void apply_matk_device(aqs_state s, const aqs_op& op, const void* dev_u);

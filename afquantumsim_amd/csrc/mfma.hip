@@ -62,6 +62,16 @@ struct MfmaEpilogue {
     int cpos[6][5];
 };
 
+// Bit involution applied while the block's tiles are written out: a restore
+// pass right after the block whose permutation maps the tile bits onto
+// themselves moves every amplitude i to P(i) (bit b to bit beta(b)).  Tile
+// t (outer bits o) then lands in tile beta(o), its local index permuted.
+struct MfmaPerm {
+    int nr;          // outer (non-tile) bits, ascending positions
+    int rmap[40];    // outer bit j -> outer bit rmap[j]
+    int tmap[10];    // tile bit j -> tile bit tmap[j]
+};
+
 struct MfmaArgs {
     float4* sv4;
     uint64_t tiles;
@@ -73,6 +83,7 @@ struct MfmaArgs {
     uint32_t tc[32];      // tile-index bits of column c
     const float* apack;
     MfmaEpilogue epi;
+    MfmaPerm perm;
 };
 
 __device__ __forceinline__ uint32_t bits_of(uint64_t i, int start, const int* pos, int len) {
@@ -116,6 +127,70 @@ __device__ __forceinline__ uint64_t tile_base(uint64_t t, const MfmaArgs& a) {
     return t;  // amplitude index of the tile's first element (even)
 }
 
+// Per-tile factors of the diagonal table's chunks outside the tile (one per
+// anchor value, first lanes) into pg.
+__device__ __forceinline__ void outside_factors(const MfmaArgs& a, float2* pg, uint64_t base, int lane) {
+    if (lane >= (1 << a.epi.ng)) return;
+    float2 f = make_float2(1.f, 0.f);
+    for (int c = 0; c < a.epi.nc; ++c) {
+        if (!((a.epi.outside >> c) & 1u)) continue;
+        const float2 v = a.epi.tab[(lane * a.epi.nc + c) * 32 +
+                                   bits_of(base, a.epi.cstart[c], a.epi.cpos[c], a.epi.clen[c])];
+        f = make_float2(f.x * v.x - f.y * v.y, f.x * v.y + f.y * v.x);
+    }
+    pg[lane] = f;
+}
+
+// Diagonal table applied to a tile's amplitudes as they are read in.
+template <int CHUNKS>
+__device__ __forceinline__ void prologue(const MfmaArgs& a, const float2* pg, const float2* tin, float4 (&pf)[CHUNKS],
+                                         uint64_t b4, uint32_t glane) {
+#pragma unroll
+    for (int j = 0; j < CHUNKS; ++j) {
+        const uint64_t i0 = (b4 + a.gj[j] + glane) << 1;
+        const float2 f0 = epi_factor(a.epi, pg, tin, i0), f1 = epi_factor(a.epi, pg, tin, i0 | 1ull);
+        const float2 y0 = cmul2(f0, pf[j].x, pf[j].y), y1 = cmul2(f1, pf[j].z, pf[j].w);
+        pf[j] = make_float4(y0.x, y0.y, y1.x, y1.y);
+    }
+}
+
+// The block's product on the tile held in LDS (memory order), result back
+// into LDS.  Ends before the final barrier.
+template <int K, int NA>
+__device__ __forceinline__ void tile_product(float2* lds, const float (&A)[NA], const MfmaArgs& a, uint32_t rd_lane,
+                                             uint32_t wr_lane) {
+    if (K == 32) {
+        floatx16 t1 = {0}, t2 = {0}, t3 = {0};
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            const float2 x = lds[padded(a.tr[2 * s] | rd_lane)];
+            t1 = __builtin_amdgcn_mfma_f32_32x32x2f32(A[s], x.x, t1, 0, 0, 0);
+            t2 = __builtin_amdgcn_mfma_f32_32x32x2f32(A[16 + s], x.y, t2, 0, 0, 0);
+            t3 = __builtin_amdgcn_mfma_f32_32x32x2f32(A[32 + s], x.x + x.y, t3, 0, 0, 0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const uint32_t tau = a.tr[8 * (r / 4) + (r % 4)] | wr_lane;
+            lds[padded(tau)] = make_float2(t1[r] - t2[r], t3[r] - t1[r] - t2[r]);
+        }
+    } else {
+        floatx16 p = {0}, q = {0};
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const float2 x = lds[padded(a.tr[2 * s] | rd_lane)];
+            p = __builtin_amdgcn_mfma_f32_32x32x2f32(A[s], x.x, p, 0, 0, 0);
+            q = __builtin_amdgcn_mfma_f32_32x32x2f32(A[s], x.y, q, 0, 0, 0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const uint32_t tau = a.tr[8 * (r / 4) + (r % 4)] | wr_lane;
+            lds[padded(tau)] = make_float2(p[r] - q[r + 8], p[r + 8] + q[r]);
+        }
+    }
+}
+
 // DIAG: 0 none, 1 diagonal table after the block (epilogue), 2 before it (prologue).
 template <int K, int DIAG>
 __global__ __launch_bounds__(64) void k_matk_mfma(MfmaArgs a) {
@@ -154,27 +229,10 @@ __global__ __launch_bounds__(64) void k_matk_mfma(MfmaArgs a) {
     }
     while (t < a.tiles) {
         const uint64_t b4 = tile_base(t, a) >> 1;
-        if (EPI && lane < (1 << a.epi.ng)) {
-            // chunks outside the tile: fixed by the tile's outer bits
-            const uint64_t base = b4 << 1;
-            float2 f = make_float2(1.f, 0.f);
-            for (int c = 0; c < a.epi.nc; ++c) {
-                if (!((a.epi.outside >> c) & 1u)) continue;
-                const float2 v = a.epi.tab[(lane * a.epi.nc + c) * 32 +
-                                           bits_of(base, a.epi.cstart[c], a.epi.cpos[c], a.epi.clen[c])];
-                f = make_float2(f.x * v.x - f.y * v.y, f.x * v.y + f.y * v.x);
-            }
-            pg[lane] = f;
-        }
+        if (EPI) outside_factors(a, pg, b4 << 1, lane);
         if (DIAG == 2) {
             __syncthreads();  // pg of this tile before the prologue reads it
-#pragma unroll
-            for (int j = 0; j < CHUNKS; ++j) {
-                const uint64_t i0 = (b4 + a.gj[j] + glane) << 1;
-                const float2 f0 = epi_factor(a.epi, pg, tin, i0), f1 = epi_factor(a.epi, pg, tin, i0 | 1ull);
-                const float2 y0 = cmul2(f0, pf[j].x, pf[j].y), y1 = cmul2(f1, pf[j].z, pf[j].w);
-                pf[j] = make_float4(y0.x, y0.y, y1.x, y1.y);
-            }
+            prologue(a, pg, tin, pf, b4, glane);
         }
         // tile -> LDS (memory order)
 #pragma unroll
@@ -190,36 +248,7 @@ __global__ __launch_bounds__(64) void k_matk_mfma(MfmaArgs a) {
 #pragma unroll
             for (int j = 0; j < CHUNKS; ++j) pf[j] = ld_nt(&a.sv4[n4 + a.gj[j] + glane]);
         }
-        if (K == 32) {
-            floatx16 t1 = {0}, t2 = {0}, t3 = {0};
-#pragma unroll
-            for (int s = 0; s < 16; ++s) {
-                const float2 x = lds[padded(a.tr[2 * s] | rd_lane)];
-                t1 = __builtin_amdgcn_mfma_f32_32x32x2f32(A[s], x.x, t1, 0, 0, 0);
-                t2 = __builtin_amdgcn_mfma_f32_32x32x2f32(A[16 + s], x.y, t2, 0, 0, 0);
-                t3 = __builtin_amdgcn_mfma_f32_32x32x2f32(A[32 + s], x.x + x.y, t3, 0, 0, 0);
-            }
-            __syncthreads();
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const uint32_t tau = a.tr[8 * (r / 4) + (r % 4)] | wr_lane;
-                lds[padded(tau)] = make_float2(t1[r] - t2[r], t3[r] - t1[r] - t2[r]);
-            }
-        } else {
-            floatx16 p = {0}, q = {0};
-#pragma unroll
-            for (int s = 0; s < 8; ++s) {
-                const float2 x = lds[padded(a.tr[2 * s] | rd_lane)];
-                p = __builtin_amdgcn_mfma_f32_32x32x2f32(A[s], x.x, p, 0, 0, 0);
-                q = __builtin_amdgcn_mfma_f32_32x32x2f32(A[s], x.y, q, 0, 0, 0);
-            }
-            __syncthreads();
-#pragma unroll
-            for (int r = 0; r < 8; ++r) {
-                const uint32_t tau = a.tr[8 * (r / 4) + (r % 4)] | wr_lane;
-                lds[padded(tau)] = make_float2(p[r] - q[r + 8], p[r + 8] + q[r]);
-            }
-        }
+        tile_product<K>(lds, A, a, rd_lane, wr_lane);
         __syncthreads();
         // LDS -> tile (memory order)
 #pragma unroll
@@ -236,6 +265,85 @@ __global__ __launch_bounds__(64) void k_matk_mfma(MfmaArgs a) {
         }
         __syncthreads();
         t = tn;
+    }
+}
+
+__device__ __forceinline__ uint64_t permute_index(uint64_t v, const int* map, int len) {
+    uint64_t r = 0;
+    for (int j = 0; j < len; ++j) r |= ((v >> j) & 1ull) << map[j];
+    return r;
+}
+
+// The block followed by a bit involution (a.perm), DIAG 0 or 2 (prologue).
+// A wave takes an outer index t and its image t2 = beta(t) (skipping t2 < t)
+// and handles both tiles with one LDS buffer: tile A goes to LDS while tile
+// B's loads are in flight into registers, A's product is stored into B's
+// place (B is already read: every lane stores exactly the addresses it
+// loaded), then B's product into A's place.  The store gathers from LDS
+// through the tile-local permutation, so HBM writes stay coalesced runs.
+template <int K, int DIAG>
+__global__ __launch_bounds__(64) void k_matk_mfma_perm(MfmaArgs a) {
+    static_assert(DIAG == 0 || DIAG == 2, "the permuted store takes no epilogue");
+    constexpr bool EPI = DIAG != 0;
+    constexpr int TILE = 32 * K;
+    constexpr int CHUNKS = TILE / 128;
+    constexpr int LDS_F2 = TILE + TILE / 16;
+    __shared__ __attribute__((aligned(16))) float2 lds[LDS_F2];
+    __shared__ uint16_t pmap[TILE];         // destination tile element -> source element
+    __shared__ float2 pg[EPI ? 32 : 1];
+    __shared__ float2 tin[EPI ? 1024 : 1];
+    const int lane = threadIdx.x;
+    if (EPI && a.epi.inside >= 0) {
+        for (int e = lane; e < (32 << a.epi.ng); e += 64)
+            tin[e] = a.epi.tab[((e >> 5) * a.epi.nc + a.epi.inside) * 32 + (e & 31)];
+    }
+    for (int e = lane; e < TILE; e += 64) pmap[e] = (uint16_t)permute_index((uint64_t)e, a.perm.tmap, a.ntile_bits);
+    const int col = lane & 31;
+    const int half = lane >> 5;
+    constexpr int NA = (K == 32) ? 48 : 8;
+    float A[NA];
+#pragma unroll
+    for (int i = 0; i < NA; ++i) A[i] = a.apack[i * 64 + lane];
+    const uint32_t glane = a.glane[lane];
+    const uint32_t rd_lane = a.tr[half] | a.tc[col];
+    const uint32_t wr_lane = a.tr[4 * half] | a.tc[col];
+    __syncthreads();
+
+    for (uint64_t t = blockIdx.x; t < a.tiles; t += gridDim.x) {
+        const uint64_t t2 = permute_index(t, a.perm.rmap, a.perm.nr);
+        if (t2 < t) continue;  // the wave that draws t2 handles this pair
+        const int ntiles = t2 == t ? 1 : 2;
+        const uint64_t b4[2] = {tile_base(t, a) >> 1, tile_base(t2, a) >> 1};
+        float4 pf[CHUNKS];
+#pragma unroll
+        for (int j = 0; j < CHUNKS; ++j) pf[j] = ld_nt(&a.sv4[b4[0] + a.gj[j] + glane]);
+        for (int s = 0; s < ntiles; ++s) {
+            if (EPI) {
+                outside_factors(a, pg, b4[s] << 1, lane);
+                __syncthreads();
+                prologue(a, pg, tin, pf, b4[s], glane);
+            }
+#pragma unroll
+            for (int j = 0; j < CHUNKS; ++j) {
+                const uint32_t tau = 2u * (uint32_t)(j * 64 + lane);
+                *reinterpret_cast<float4*>(&lds[padded(tau)]) = pf[j];
+            }
+            __syncthreads();
+            if (s + 1 < ntiles) {  // the partner tile, read before this tile's result lands on it
+#pragma unroll
+                for (int j = 0; j < CHUNKS; ++j) pf[j] = ld_nt(&a.sv4[b4[1] + a.gj[j] + glane]);
+            }
+            tile_product<K>(lds, A, a, rd_lane, wr_lane);
+            __syncthreads();
+            const uint64_t dst = b4[ntiles - 1 - s];
+#pragma unroll
+            for (int j = 0; j < CHUNKS; ++j) {
+                const uint32_t e = 2u * (uint32_t)(j * 64 + lane);
+                const float2 v0 = lds[padded(pmap[e])], v1 = lds[padded(pmap[e + 1])];
+                st_nt(make_float4(v0.x, v0.y, v1.x, v1.y), &a.sv4[dst + a.gj[j] + glane]);
+            }
+            __syncthreads();
+        }
     }
 }
 
@@ -280,7 +388,7 @@ std::vector<float> pack_mfma_operand(int k, const double* u) {
 }
 
 bool apply_matk_mfma(aqs_state s, const aqs_op& op, const float* dev_apack, const DiagEpilogueDesc* epi,
-                     bool before) {
+                     bool before, const std::vector<int>* beta) {
     const int k = op.ntargets;
     if (s->precision != AQS_C64 || op.ncontrols != 0 || op.more_controls || (k != 4 && k != 5)) return false;
     if (s->n < k + 5 || s->n > 36) return false;
@@ -323,6 +431,19 @@ bool apply_matk_mfma(aqs_state s, const aqs_op& op, const float* dev_apack, cons
         a.tc[c] = v;
     }
     a.apack = dev_apack;
+    if (beta) {
+        if ((int)beta->size() != n || (epi && !before)) return false;
+        std::vector<int> R;
+        for (int b = 0; b < n; ++b)
+            if (std::find(T.begin(), T.end(), b) == T.end()) R.push_back(b);
+        for (int b : T)
+            if (std::find(T.begin(), T.end(), (*beta)[b]) == T.end()) return false;
+        MfmaPerm& pm = a.perm;
+        pm.nr = (int)R.size();
+        for (size_t j = 0; j < T.size(); ++j) pm.tmap[j] = tidx((*beta)[T[j]]);
+        for (size_t j = 0; j < R.size(); ++j)
+            pm.rmap[j] = (int)(std::find(R.begin(), R.end(), (*beta)[R[j]]) - R.begin());
+    }
     if (epi) {
         // the anchor must lie in the tile (it indexes the per-tile factors);
         // a chunk is either wholly outside the tile or looked up per amplitude
@@ -364,6 +485,17 @@ bool apply_matk_mfma(aqs_state s, const aqs_op& op, const float* dev_apack, cons
     ProfScope ps(s, kSlotMfma);
     const int mode = !epi ? 0 : before ? 2 : 1;
     const dim3 grid((unsigned)blocks), block(64);
+    if (beta) {
+        if (k == 5) {
+            if (mode == 0) hipLaunchKernelGGL((k_matk_mfma_perm<32, 0>), grid, block, 0, s->stream, a);
+            if (mode == 2) hipLaunchKernelGGL((k_matk_mfma_perm<32, 2>), grid, block, 0, s->stream, a);
+        } else {
+            if (mode == 0) hipLaunchKernelGGL((k_matk_mfma_perm<16, 0>), grid, block, 0, s->stream, a);
+            if (mode == 2) hipLaunchKernelGGL((k_matk_mfma_perm<16, 2>), grid, block, 0, s->stream, a);
+        }
+        AQS_HIP(hipGetLastError());
+        return true;
+    }
     if (k == 5) {
         if (mode == 0) hipLaunchKernelGGL((k_matk_mfma<32, 0>), grid, block, 0, s->stream, a);
         if (mode == 1) hipLaunchKernelGGL((k_matk_mfma<32, 1>), grid, block, 0, s->stream, a);

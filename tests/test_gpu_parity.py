@@ -260,6 +260,47 @@ def test_fused_swaps_are_relabelled_bit_exact(n):
         assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
 
 
+def restore_case(name, n):
+    """Circuits whose schedule ends with an MFMA block then the restore
+    permutation: the QFT with its swaps (block with a table prologue, tile
+    pairs), a bit reversal before a 5-qubit block (tile pairs), a swap of
+    bits 0 and 5 before a block on bits 1..5 (every tile its own partner)."""
+    U = np.linalg.qr(np.random.default_rng(n).normal(size=(32, 32)) + 1j * np.random.default_rng(n + 1).normal(size=(32, 32)))[0]
+    if name == "qft":
+        return circuits.fourier_transform(n, final_swaps=True)
+    if name == "reverse":
+        g = [oracle.gate("H", q) for q in range(n)]
+        g += [oracle.gate("Swap", i, n - 1 - i) for i in range(n // 2)]
+        return g + [oracle.gate("Circuit", n - 5, k=5, u=U)]
+    g = [oracle.gate("RotY", q, angle=0.1 + 0.2 * q) for q in range(n)]
+    g.append(oracle.gate("Swap", n - 1, n - 6))
+    return g + [oracle.gate("Circuit", n - 5, k=5, u=U)]
+
+
+@pytest.mark.parametrize("name,n,folded", [("qft", 20, True), ("qft", 14, False), ("reverse", 16, True),
+                                           ("swap05", 16, True)])
+def test_mfma_store_folds_restore_permutation(name, n, folded):
+    """A restore permutation that maps the last MFMA block's tile onto itself
+    runs in that block's store (no permutation launch); the state equals the
+    oracle's."""
+    gates = restore_case(name, n)
+    sv = random_state(n, 2 * n)
+    want = oracle.run(sv, n, gates, threads=8)
+    sch = hip.Schedule(ops.lower_all(gates), n, 5)
+    kinds = [k for k, _ in sch.passes()]
+    assert kinds[-1] == hip.OP_PERMB and kinds[-2] == hip.OP_MATK
+    st = hip.State(n, hip.C64)
+    st.upload(sv.astype(st.dtype))
+    st.profile(True)
+    st.apply_schedule(sch)
+    prof = st.profile_read()
+    got = st.download()
+    st.close()
+    assert (prof.get("PERMB", (0.0, 0))[1] == 0) == folded, prof
+    assert oracle.infidelity(got, want) < 1e-9
+    np.testing.assert_allclose(got, want, atol=5e-6)
+
+
 def structured_swaps(n, name):
     """Swap lists whose restore permutations need the bit-permutation tile
     grown beyond bits 0..4 and their partners."""

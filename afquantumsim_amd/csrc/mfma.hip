@@ -313,15 +313,16 @@ __global__ __launch_bounds__(64) void k_matk_mfma_perm(MfmaArgs a) {
         const uint64_t t2 = permute_index(t, a.perm.rmap, a.perm.nr);
         if (t2 < t) continue;  // the wave that draws t2 handles this pair
         const int ntiles = t2 == t ? 1 : 2;
-        const uint64_t b4[2] = {tile_base(t, a) >> 1, tile_base(t2, a) >> 1};
+        const uint64_t b4a = tile_base(t, a) >> 1, b4b = tile_base(t2, a) >> 1;
         float4 pf[CHUNKS];
 #pragma unroll
-        for (int j = 0; j < CHUNKS; ++j) pf[j] = ld_nt(&a.sv4[b4[0] + a.gj[j] + glane]);
+        for (int j = 0; j < CHUNKS; ++j) pf[j] = ld_nt(&a.sv4[b4a + a.gj[j] + glane]);
         for (int s = 0; s < ntiles; ++s) {
+            const uint64_t src = s ? b4b : b4a;
             if (EPI) {
-                outside_factors(a, pg, b4[s] << 1, lane);
+                outside_factors(a, pg, src << 1, lane);
                 __syncthreads();
-                prologue(a, pg, tin, pf, b4[s], glane);
+                prologue(a, pg, tin, pf, src, glane);
             }
 #pragma unroll
             for (int j = 0; j < CHUNKS; ++j) {
@@ -331,11 +332,11 @@ __global__ __launch_bounds__(64) void k_matk_mfma_perm(MfmaArgs a) {
             __syncthreads();
             if (s + 1 < ntiles) {  // the partner tile, read before this tile's result lands on it
 #pragma unroll
-                for (int j = 0; j < CHUNKS; ++j) pf[j] = ld_nt(&a.sv4[b4[1] + a.gj[j] + glane]);
+                for (int j = 0; j < CHUNKS; ++j) pf[j] = ld_nt(&a.sv4[b4b + a.gj[j] + glane]);
             }
             tile_product<K>(lds, A, a, rd_lane, wr_lane);
             __syncthreads();
-            const uint64_t dst = b4[ntiles - 1 - s];
+            const uint64_t dst = (ntiles - 1 - s) ? b4b : b4a;
 #pragma unroll
             for (int j = 0; j < CHUNKS; ++j) {
                 const uint32_t e = 2u * (uint32_t)(j * 64 + lane);

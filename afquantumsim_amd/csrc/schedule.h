@@ -35,6 +35,7 @@ enum TileOpKind : int32_t {
     TILE_MAT = 0,     // 2x2 on register slot `slot`
     TILE_XCH = 1,     // exchange across register slot `slot`
     TILE_PHASES = 2,  // a run of diagonal gates as phase terms [term_begin, +term_count)
+    TILE_MATR = 3,    // real 2x2 on register slot `slot` (half the multiplies of TILE_MAT)
 };
 
 // Kernel dispatch code of an op (one switch per op, no divergent branch
@@ -43,12 +44,20 @@ enum TileOpKind : int32_t {
 //   form 1  controls on register slots only (one uniform test per pair)
 //   form 2  controls on thread-held positions, or "any" controls (per-thread
 //           predicate, results selected)
-// MAT: 0 + 4*form + slot, XCH: 12 + 4*form + slot, PHASES: 24 direct, 25 zeta.
+// MAT: 0 + 4*form + slot, XCH: 12 + 4*form + slot, PHASES: 24 direct, 25 zeta,
+// MATR: 26 + 4*form + slot.
 constexpr int kTileCodePhases = 24;
+constexpr int kTileCodeMatR = 26;
+constexpr int kTileCodeCount = 38;
 inline int tile_code(int kind, int form, int slot, bool zeta) {
     if (kind == TILE_PHASES) return kTileCodePhases + (zeta ? 1 : 0);
+    if (kind == TILE_MATR) return kTileCodeMatR + 4 * form + slot;
     return (kind == TILE_XCH ? 12 : 0) + 4 * form + slot;
 }
+constexpr bool tile_code_is_phases(int code) { return code == kTileCodePhases || code == kTileCodePhases + 1; }
+// form and slot of a MAT / XCH / MATR code
+constexpr int tile_code_form(int code) { return code >= kTileCodeMatR ? (code - kTileCodeMatR) / 4 : (code % 12) / 4; }
+constexpr int tile_code_slot(int code) { return (code >= kTileCodeMatR ? code - kTileCodeMatR : code) % 4; }
 
 // 64 bytes, read with scalar loads.  Complex64 kernels read the FP32 matrix
 // here; complex128 kernels read TilePass::m128[8 * op].
@@ -59,7 +68,7 @@ struct TileOpDev {
     uint32_t c_thr;   // control local positions held by the thread (mask over local positions)
     uint64_t c_glb;   // control index bits outside the tile
     uint32_t term_begin, term_count;
-    float mf[8];      // MAT: row-major 2x2 in FP32
+    float mf[8];      // MAT: row-major 2x2 in FP32 (complex); MATR: mf[0..3] = real row-major 2x2
 };
 static_assert(sizeof(TileOpDev) == 64, "compact tile op");
 

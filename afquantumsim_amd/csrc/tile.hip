@@ -77,11 +77,27 @@ __device__ __forceinline__ void butterfly(T2& p, T2& q, const R (&m)[8]) {
     q = v;
 }
 
-// Pair operations across register slot SL.  Form 0: every pair.  Form 1:
+// Real 2x2 (m[0..3] row-major) on complex amplitudes: 8 multiplies per pair
+// instead of 16.  A general unitary reaches the tile as phase terms around
+// such a real rotation (its ZYZ form), the phases merged into the runs of
+// diagonal gates around it.
+template <typename T2, typename R>
+__device__ __forceinline__ void butterfly_real(T2& p, T2& q, const R (&m)[8]) {
+    T2 u, v;
+    u.x = m[0] * p.x + m[1] * q.x;
+    u.y = m[0] * p.y + m[1] * q.y;
+    v.x = m[2] * p.x + m[3] * q.x;
+    v.y = m[2] * p.y + m[3] * q.y;
+    p = u;
+    q = v;
+}
+
+// Pair operations across register slot SL (OP: 0 complex 2x2, 1 exchange,
+// 2 real 2x2).  Form 0: every pair.  Form 1:
 // pairs whose register index holds the control slots `creg` (uniform test,
 // the partner has the same control bits since a control is never the
 // target).  Form 2: per-thread predicate `ok(j)`, results selected.
-template <int SL, bool XCH, int FORM, typename T2, typename R, typename OK>
+template <int SL, int OP, int FORM, typename T2, typename R, typename OK>
 __device__ __forceinline__ void pairs(T2 (&x)[16], const R (&m)[8], uint32_t creg, OK ok) {
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
@@ -89,10 +105,12 @@ __device__ __forceinline__ void pairs(T2 (&x)[16], const R (&m)[8], uint32_t cre
         const int k = j | (1 << SL);
         if (FORM == 1 && ((uint32_t)j & creg) != creg) continue;
         T2 p = x[j], q = x[k];
-        if (XCH) {
+        if (OP == 1) {
             const T2 t = p;
             p = q;
             q = t;
+        } else if (OP == 2) {
+            butterfly_real(p, q, m);
         } else {
             butterfly(p, q, m);
         }
@@ -315,7 +333,7 @@ __global__ __launch_bounds__(1 << W, (sizeof(T2) == 8 ? 4 : 2)) void k_tile(
             for (uint32_t o = 0; o < oc; ++o) {
                 const TileOpDev& op = ops[ob + o];
                 const int code = op.code;
-                if (code >= kTileCodePhases) {
+                if (tile_code_is_phases(code)) {
                     if (code == kTileCodePhases)
                         phases_direct<T2, R>(x, terms + op.term_begin, z128 + 2 * (size_t)op.term_begin, fold_val,
                                              op.term_count, lthr, base);
@@ -345,16 +363,19 @@ __global__ __launch_bounds__(1 << W, (sizeof(T2) == 8 ? 4 : 2)) void k_tile(
                         m[i] = m128[8 * (size_t)(ob + o) + i];
                 }
                 switch (code) {
-#define AQS_PAIRS(C, SL, XCH, FORM) \
-    case C: pairs<SL, XCH, FORM>(x, m, creg, ok); break;
-#define AQS_PAIRS4(C, XCH, FORM) \
-    AQS_PAIRS(C, 0, XCH, FORM) AQS_PAIRS(C + 1, 1, XCH, FORM) AQS_PAIRS(C + 2, 2, XCH, FORM) AQS_PAIRS(C + 3, 3, XCH, FORM)
-                    AQS_PAIRS4(0, false, 0)
-                    AQS_PAIRS4(4, false, 1)
-                    AQS_PAIRS4(8, false, 2)
-                    AQS_PAIRS4(12, true, 0)
-                    AQS_PAIRS4(16, true, 1)
-                    AQS_PAIRS4(20, true, 2)
+#define AQS_PAIRS(C, SL, OP, FORM) \
+    case C: pairs<SL, OP, FORM>(x, m, creg, ok); break;
+#define AQS_PAIRS4(C, OP, FORM) \
+    AQS_PAIRS(C, 0, OP, FORM) AQS_PAIRS(C + 1, 1, OP, FORM) AQS_PAIRS(C + 2, 2, OP, FORM) AQS_PAIRS(C + 3, 3, OP, FORM)
+                    AQS_PAIRS4(0, 0, 0)
+                    AQS_PAIRS4(4, 0, 1)
+                    AQS_PAIRS4(8, 0, 2)
+                    AQS_PAIRS4(12, 1, 0)
+                    AQS_PAIRS4(16, 1, 1)
+                    AQS_PAIRS4(20, 1, 2)
+                    AQS_PAIRS4(kTileCodeMatR, 2, 0)
+                    AQS_PAIRS4(kTileCodeMatR + 4, 2, 1)
+                    AQS_PAIRS4(kTileCodeMatR + 8, 2, 2)
 #undef AQS_PAIRS4
 #undef AQS_PAIRS
                     default: break;

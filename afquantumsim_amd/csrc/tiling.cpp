@@ -118,6 +118,7 @@ class Tiler {
                 o.nd = {o.tbit};
                 o.dg = o.ctrl;
                 std::memcpy(o.m, op.m, sizeof(o.m));
+                if (op.kind == AQS_OP_MAT1 && !o.any && lower_real(o)) return;
                 break;
             case AQS_OP_DIAG1: {
                 if (o.any) break;
@@ -171,6 +172,22 @@ class Tiler {
             for (int c : o.ctrl) o.nd.push_back(c);
         }
         g_.push_back(std::move(o));
+    }
+
+    // A one-qubit gate with a real matrix (H, RotY) as a real 2x2: half the
+    // multiplies of a complex one.  (Factoring complex gates into a real
+    // rotation between phases was measured 2x slower on the brickwork: the
+    // phase runs it adds between consecutive rotations cost more than the
+    // saved multiplies.)
+    bool lower_real(const GOp& o) {
+        const double* m = o.op.m;
+        if (m[1] != 0 || m[3] != 0 || m[5] != 0 || m[7] != 0) return false;
+        GOp r = o;
+        r.kind = TILE_MATR;
+        const double re[8] = {m[0], m[2], m[4], m[6], 0, 0, 0, 0};
+        std::memcpy(r.m, re, sizeof(r.m));
+        g_.push_back(std::move(r));
+        return true;
     }
 
     // X on `t.targets[0]` controlled by t's controls plus qubit `extra` (the
@@ -332,13 +349,13 @@ class Tiler {
             ok = ok && (uint64_t)s.op_begin + s.op_count <= tp.ops.size();
         }
         for (const auto& o : tp.ops) {
-            ok = ok && o.code >= 0 && o.code <= kTileCodePhases + 1 && o.c_reg < (1u << kTileRegBits) &&
+            ok = ok && o.code >= 0 && o.code < kTileCodeCount && o.c_reg < (1u << kTileRegBits) &&
                  o.c_thr < (1u << m) && (o.c_glb >> n_) == 0;
-            if (o.code < kTileCodePhases) {
-                const int form = (o.code % 12) / 4;
+            if (!tile_code_is_phases(o.code)) {
+                const int form = tile_code_form(o.code);
                 ok = ok && (form == 2 || (!o.any && o.c_thr == 0 && (form == 1) == (o.c_reg != 0)));
             }
-            if (o.code >= kTileCodePhases) {
+            if (tile_code_is_phases(o.code)) {
                 ok = ok && o.term_count > 0 && (uint64_t)o.term_begin + o.term_count <= tp.terms.size();
                 // groups tile the op's terms exactly
                 for (uint32_t k = 0; ok && k < o.term_count;) {
@@ -518,7 +535,9 @@ double pass_cost(const aqs_schedule_s* sch, size_t p) {
         for (const auto& o : t.ops) {
             if (o.code < 12) {
                 c += 0.107;
-            } else if (o.code < kTileCodePhases) {
+            } else if (o.code >= kTileCodeMatR) {
+                c += 0.054;
+            } else if (!tile_code_is_phases(o.code)) {
                 c += 0.034;
             } else {
                 const bool zeta = o.code == kTileCodePhases + 1;

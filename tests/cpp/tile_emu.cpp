@@ -164,7 +164,7 @@ void tile_apply(std::vector<cplx>& v, int n, const TilePass& tp) {
                 for (int j = 0; j < 16; ++j) x[j] = tile[addr(j)];
                 for (uint32_t o = 0; o < S.op_count; ++o) {
                     const TileOpDev& op = tp.ops[S.op_begin + o];
-                    if (op.code >= kTileCodePhases) {
+                    if (tile_code_is_phases(op.code)) {
                         // every term multiplies the amplitudes whose register
                         // index holds its register mask (both device forms
                         // compute this product)
@@ -182,11 +182,13 @@ void tile_apply(std::vector<cplx>& v, int n, const TilePass& tp) {
                     const bool g_all = (base & op.c_glb) == op.c_glb, g_any = (base & op.c_glb) != 0;
                     if (!any && !g_all) continue;
                     const bool t_all = (lthr & op.c_thr) == op.c_thr, t_any = (lthr & op.c_thr) != 0;
-                    // code = (XCH ? 12 : 0) + 4 * form + slot; the form only
-                    // selects how the kernel evaluates the same controls
-                    const int slot = op.code % 4;
+                    // code: MAT 4 * form + slot, XCH 12 + ..., MATR 26 + ...;
+                    // the form only selects how the kernel evaluates the
+                    // same controls
+                    const int slot = tile_code_slot(op.code);
                     const bool mat = op.code < 12;
-                    const int form = (op.code % 12) / 4;
+                    const bool real = op.code >= kTileCodeMatR;
+                    const int form = tile_code_form(op.code);
                     if (form == 0 && (op.c_reg || op.c_thr || any)) throw 1;
                     if (form == 1 && (op.c_thr || any || !op.c_reg)) throw 1;
                     for (int j = 0; j < 16; ++j) {
@@ -198,6 +200,11 @@ void tile_apply(std::vector<cplx>& v, int n, const TilePass& tp) {
                             const cplx p = x[j], q = x[k];
                             x[j] = cplx(m[0], m[1]) * p + cplx(m[2], m[3]) * q;
                             x[k] = cplx(m[4], m[5]) * p + cplx(m[6], m[7]) * q;
+                        } else if (real) {
+                            const double* m = &tp.m128[8 * (size_t)(S.op_begin + o)];
+                            const cplx p = x[j], q = x[k];
+                            x[j] = m[0] * p + m[1] * q;
+                            x[k] = m[2] * p + m[3] * q;
                         } else {
                             std::swap(x[j], x[k]);
                         }

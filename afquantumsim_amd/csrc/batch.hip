@@ -5,13 +5,16 @@
 // The reference simulates one state at a time with a 2^n x 2^n operator per
 // gate (src/quantum.cpp:277-291).  Here every workgroup of a persistent grid
 // takes the next state from an atomic queue (states sorted by descending
-// cost), builds |0..0> in LDS when the state fits the LDS tile (2^lds_qubits
-// amplitudes) or in its HBM slot otherwise (then it stays L2-resident), runs
-// all of that state's gate passes with a workgroup barrier between passes,
-// reduces <sum_q Z_q> in FP64 and writes the final amplitudes back to HBM.
-// The launch (queue reset + persistent kernel) is captured once into a
-// hipGraph and replayed.  An autotuner times (workgroup size, LDS tile)
-// candidates and keeps the fastest.
+// cost).  A state that fits the LDS tile (2^lds_qubits amplitudes) is built
+// as |0..0> in LDS, runs all its gate passes there with a workgroup barrier
+// between passes, and is written to HBM once.  A larger state runs as a few
+// phases (batch_phases): per phase every tile of 2^lds_qubits amplitudes is
+// loaded into LDS, runs all of the phase's gates and is stored, so the state
+// crosses the memory hierarchy once per phase instead of once per gate.
+// <sum_q Z_q> is reduced in FP64 from the final tiles.  The launch (queue
+// reset + persistent kernel) is captured once into a hipGraph and replayed.
+// An autotuner times (workgroup size, LDS tile) candidates and keeps the
+// fastest.
 //
 // Per-state host preprocessing absorbs runs of uncontrolled one-qubit gates
 // on a qubit into one 2x2 matrix (FP64 products) until a multi-qubit gate
@@ -20,29 +23,21 @@
 
 #include <algorithm>
 #include <complex>
+#include <map>
 #include <numeric>
 #include <vector>
 
-#include "internal.h"
+#include "batch.h"
 
 using namespace aqs_hip;
 
 namespace {
 
-struct BOp {
-    int32_t kind;    // AQS_OP_MAT1 / DIAG1 / PHASE / X / SWAP
-    int32_t t0, t1;  // target bit positions (t1 for SWAP)
-    int32_t any;     // AQS_CTRL_ANY
-    uint32_t cmask;  // control bits (state index space)
-    uint32_t pmask;  // PHASE: all bits that must be set
-    float m[8];
-};
-
 struct Desc {
     uint64_t amp_offset;  // into the HBM pool (amplitudes)
-    uint32_t op_begin, op_count;
+    uint32_t phase_begin, phase_count;
     int32_t n;
-    int32_t index;        // position in the caller's order
+    int32_t index;  // position in the caller's order
 };
 
 __device__ __forceinline__ uint32_t ins0(uint32_t p, int t) { return ((p >> t) << (t + 1)) | (p & ((1u << t) - 1u)); }
@@ -52,8 +47,35 @@ __device__ __forceinline__ bool ctrl_ok(uint32_t i, uint32_t cmask, int any) {
     return any ? (i & cmask) != 0 : (i & cmask) == cmask;
 }
 
+// Tile-local index j -> state index: zeros inserted at the (ascending) bit
+// positions of gbits, then the tile's own bits g or'ed in.
+__device__ __forceinline__ uint32_t spread(uint32_t j, uint32_t gbits) {
+    for (uint32_t g = gbits; g; g &= g - 1u) j = ins0(j, __ffs(g) - 1);
+    return j;
+}
+
+// Tile number t -> its bits at the positions of gbits.
+__device__ __forceinline__ uint32_t deposit(uint32_t t, uint32_t gbits) {
+    uint32_t r = 0;
+    for (uint32_t g = gbits; g; g &= g - 1u, t >>= 1)
+        if (t & 1u) r |= g & (0u - g);
+    return r;
+}
+
+// Controls outside the tile, given the tile's bits g: false when the op does
+// nothing on this tile; an any-of control met outside drops the local ones.
+__device__ __forceinline__ bool resolve(BatchOp& e, uint32_t g) {
+    if (!e.gmask) return true;
+    if (e.kind == AQS_OP_PHASE || !e.any) return (g & e.gmask) == e.gmask;
+    if (g & e.gmask) {
+        e.cmask = 0;
+        return true;
+    }
+    return e.cmask != 0;
+}
+
 template <int BLOCK>
-__device__ void apply_block(float2* sv, uint32_t N, const BOp& op) {
+__device__ void apply_block(float2* sv, uint32_t N, const BatchOp& op) {
     const int tid = threadIdx.x;
     switch (op.kind) {
         case AQS_OP_MAT1: {
@@ -116,9 +138,22 @@ __device__ void apply_block(float2* sv, uint32_t N, const BOp& op) {
 }
 
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_batch(const Desc* __restrict__ desc, int count, const BOp* __restrict__ ops,
-                                                  float2* __restrict__ pool, double* __restrict__ out,
-                                                  unsigned* __restrict__ queue, int lds_qubits) {
+__device__ double block_sum(double acc, double* s_red) {
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < BLOCK / 64; ++w) s += s_red[w];
+    return s;
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_batch(const Desc* __restrict__ desc, int count,
+                                                  const BatchPhase* __restrict__ phases,
+                                                  const BatchOp* __restrict__ ops, float2* __restrict__ pool,
+                                                  double* __restrict__ out, unsigned* __restrict__ queue,
+                                                  int lds_qubits) {
     extern __shared__ float2 lds[];
     __shared__ int s_idx;
     __shared__ double s_red[BLOCK / 64];
@@ -130,31 +165,55 @@ __global__ __launch_bounds__(BLOCK) void k_batch(const Desc* __restrict__ desc, 
         __syncthreads();
         if (k >= count) break;  // every wave of the workgroup leaves together
         const Desc d = desc[k];
-        const uint32_t N = 1u << d.n;
         float2* hbm = pool + d.amp_offset;
-        float2* sv = (d.n <= lds_qubits) ? lds : hbm;
-        for (uint32_t i = tid; i < N; i += BLOCK) sv[i] = make_float2(i == 0 ? 1.f : 0.f, 0.f);
-        __syncthreads();
-        for (uint32_t o = 0; o < d.op_count; ++o) {
-            apply_block<BLOCK>(sv, N, ops[d.op_begin + o]);
-            __syncthreads();
-        }
-        // <sum Z> in FP64, and the final amplitudes to HBM
         double acc = 0.0;
-        for (uint32_t i = tid; i < N; i += BLOCK) {
-            const float2 v = sv[i];
-            const double p = (double)v.x * v.x + (double)v.y * v.y;
-            acc += p * (double)(d.n - 2 * (int)__popc(i));
-            if (sv != hbm) hbm[i] = v;
+        if (d.n <= lds_qubits) {
+            // the whole state in LDS: one phase
+            const uint32_t N = 1u << d.n;
+            const BatchPhase ph = phases[d.phase_begin];
+            for (uint32_t i = tid; i < N; i += BLOCK) lds[i] = make_float2(i == 0 ? 1.f : 0.f, 0.f);
+            __syncthreads();
+            for (uint32_t o = 0; o < ph.op_count; ++o) {
+                apply_block<BLOCK>(lds, N, ops[ph.op_begin + o]);
+                __syncthreads();
+            }
+            for (uint32_t i = tid; i < N; i += BLOCK) {
+                const float2 v = lds[i];
+                acc += ((double)v.x * v.x + (double)v.y * v.y) * (double)(d.n - 2 * (int)__popc(i));
+                hbm[i] = v;
+            }
+        } else {
+            const uint32_t T = 1u << lds_qubits, tiles = 1u << (d.n - lds_qubits);
+            for (uint32_t p = 0; p < d.phase_count; ++p) {
+                const BatchPhase ph = phases[d.phase_begin + p];
+                const bool last = p + 1 == d.phase_count;
+                for (uint32_t t = 0; t < tiles; ++t) {
+                    const uint32_t g = deposit(t, ph.gbits);
+                    if (p == 0) {
+                        for (uint32_t j = tid; j < T; j += BLOCK)
+                            lds[j] = make_float2(t == 0 && j == 0 ? 1.f : 0.f, 0.f);
+                    } else {
+                        for (uint32_t j = tid; j < T; j += BLOCK) lds[j] = hbm[spread(j, ph.gbits) | g];
+                    }
+                    __syncthreads();
+                    for (uint32_t o = 0; o < ph.op_count; ++o) {
+                        BatchOp e = ops[ph.op_begin + o];
+                        if (!resolve(e, g)) continue;  // uniform over the workgroup
+                        apply_block<BLOCK>(lds, T, e);
+                        __syncthreads();
+                    }
+                    for (uint32_t j = tid; j < T; j += BLOCK) {
+                        const float2 v = lds[j];
+                        const uint32_t i = spread(j, ph.gbits) | g;
+                        if (last) acc += ((double)v.x * v.x + (double)v.y * v.y) * (double)(d.n - 2 * (int)__popc(i));
+                        hbm[i] = v;
+                    }
+                    __syncthreads();  // the next tile's load reuses the LDS
+                }
+            }
         }
-        for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-        if ((tid & 63) == 0) s_red[tid >> 6] = acc;
-        __syncthreads();
-        if (tid == 0) {
-            double s = 0.0;
-            for (int w = 0; w < BLOCK / 64; ++w) s += s_red[w];
-            out[d.index] = s;
-        }
+        const double s = block_sum<BLOCK>(acc, s_red);
+        if (tid == 0) out[d.index] = s;
         __syncthreads();
     }
 }
@@ -191,17 +250,171 @@ M2 mul(const M2& x, const M2& y) {  // x * y
     return r;
 }
 
+// Bits an op must find inside the tile, bits it changes (non-diagonal
+// use) and bits it only reads or multiplies on (diagonal use).
+struct Uses {
+    uint32_t need, nd, d;
+};
+
+Uses uses_of(const BatchOp& o) {
+    const uint32_t t0 = 1u << o.t0;
+    switch (o.kind) {
+        case AQS_OP_MAT1:
+        case AQS_OP_X:
+            return {t0, t0, o.cmask};
+        case AQS_OP_SWAP:
+            return {t0 | (1u << o.t1), t0 | (1u << o.t1), o.cmask};
+        case AQS_OP_DIAG1:
+            return {t0, 0u, t0 | o.cmask};
+        default:  // PHASE
+            return {0u, 0u, o.pmask};
+    }
+}
+
+// Positions of the bits of `mask` inside the tile whose bits are `local`.
+uint32_t compress(uint32_t mask, uint32_t local) {
+    uint32_t r = 0;
+    int k = 0;
+    for (int b = 0; b < 32; ++b) {
+        if (!((local >> b) & 1u)) continue;
+        if ((mask >> b) & 1u) r |= 1u << k;
+        ++k;
+    }
+    return r;
+}
+
+int local_pos(int b, uint32_t local) { return __builtin_popcount(local & ((1u << b) - 1u)); }
+
 }  // namespace
 
+namespace aqs_hip {
+
+std::vector<BatchOp> batch_absorb(const aqs_op* ops, size_t count, int n) {
+    std::vector<BatchOp> out;
+    std::vector<M2> pending(n);
+    std::vector<char> has(n, 0);
+    auto flush = [&](int q) {
+        if (!has[q]) return;
+        BatchOp o{};
+        o.kind = AQS_OP_MAT1;
+        o.t0 = n - 1 - q;
+        for (int i = 0; i < 4; ++i) {
+            o.m[2 * i] = (float)pending[q].a[i].real();
+            o.m[2 * i + 1] = (float)pending[q].a[i].imag();
+        }
+        out.push_back(o);
+        has[q] = 0;
+    };
+    for (size_t i = 0; i < count; ++i) {
+        const aqs_op& op = ops[i];
+        validate_op(op, n);
+        if (op.kind == AQS_OP_MATK) fail(AQS_ERR_INVALID_ARGUMENT, "batched states take no MATK ops");
+        const bool one_q =
+            (op.kind == AQS_OP_MAT1 || op.kind == AQS_OP_DIAG1) && op.ncontrols == 0 && !op.more_controls;
+        if (one_q) {
+            const int q = op.targets[0];
+            M2 m = mat_of(op);
+            if (has[q]) {
+                pending[q] = mul(m, pending[q]);
+            } else {
+                pending[q] = m;
+                has[q] = 1;
+            }
+            continue;
+        }
+        for (int j = 0; j < op.ntargets; ++j) flush(op.targets[j]);
+        for (int j = 0; j < op.ncontrols; ++j) flush(op.controls[j]);
+        for (int q = 0; q < n; ++q)
+            if ((op.more_controls >> q) & 1ull) flush(q);
+        BatchOp o{};
+        o.kind = op.kind;
+        o.t0 = n - 1 - op.targets[0];
+        o.t1 = op.ntargets > 1 ? n - 1 - op.targets[1] : 0;
+        o.any = (op.flags & AQS_CTRL_ANY) ? 1 : 0;
+        for (int j = 0; j < op.ncontrols; ++j) o.cmask |= 1u << (n - 1 - op.controls[j]);
+        for (int q = 0; q < n; ++q)
+            if ((op.more_controls >> q) & 1ull) o.cmask |= 1u << (n - 1 - q);
+        if (op.kind == AQS_OP_PHASE) {
+            o.pmask = o.cmask;
+            for (int j = 0; j < op.ntargets; ++j) o.pmask |= 1u << (n - 1 - op.targets[j]);
+        }
+        for (int j = 0; j < 8; ++j) o.m[j] = (float)op.m[j];
+        out.push_back(o);
+    }
+    for (int q = 0; q < n; ++q) flush(q);
+    return out;
+}
+
+void batch_phases(const std::vector<BatchOp>& flat, int n, int m, std::vector<BatchOp>& out,
+                  std::vector<BatchPhase>& phases) {
+    if (n <= m) {
+        phases.push_back({(uint32_t)out.size(), (uint32_t)flat.size(), 0u, 0u});
+        out.insert(out.end(), flat.begin(), flat.end());
+        return;
+    }
+    const uint32_t all = (n >= 32) ? ~0u : (1u << n) - 1u;
+    std::vector<Uses> use(flat.size());
+    for (size_t i = 0; i < flat.size(); ++i) use[i] = uses_of(flat[i]);
+    std::vector<uint32_t> rem(flat.size()), next;
+    std::iota(rem.begin(), rem.end(), 0u);
+    while (!rem.empty()) {
+        uint32_t local = (1u << std::min(5, m)) - 1u;
+        for (uint32_t i : rem) {
+            for (uint32_t b = use[i].need; b && __builtin_popcount(local) < m; b &= b - 1u) local |= b & (0u - b);
+            if (__builtin_popcount(local) == m) break;
+        }
+        for (int b = 0; __builtin_popcount(local) < m; ++b) local |= 1u << b;
+        BatchPhase ph{(uint32_t)out.size(), 0u, all & ~local, 0u};
+        uint32_t blocked_nd = 0, blocked_d = 0;
+        next.clear();
+        for (uint32_t i : rem) {
+            const Uses& u = use[i];
+            const bool ok = !(u.need & ~local) && !(u.nd & (blocked_nd | blocked_d)) && !(u.d & blocked_nd);
+            if (!ok) {
+                next.push_back(i);
+                blocked_nd |= u.nd;
+                blocked_d |= u.d;
+                continue;
+            }
+            BatchOp o = flat[i];
+            o.t0 = local_pos(o.t0, local);
+            if (o.kind == AQS_OP_SWAP) o.t1 = local_pos(o.t1, local);
+            if (o.kind == AQS_OP_PHASE) {
+                o.gmask = o.pmask & ~local;
+                o.pmask = compress(o.pmask, local);
+                o.t0 = 0;
+            } else {
+                o.gmask = o.cmask & ~local;
+            }
+            o.cmask = compress(o.cmask, local);
+            out.push_back(o);
+            ++ph.op_count;
+        }
+        phases.push_back(ph);
+        rem.swap(next);
+    }
+}
+
+}  // namespace aqs_hip
+
 struct aqs_batch_s {
+    // The program for one LDS tile size.
+    struct Prog {
+        std::vector<Desc> desc;  // sorted by descending cost
+        std::vector<BatchOp> ops;
+        std::vector<BatchPhase> phases;
+        void* d_desc = nullptr;
+        void* d_ops = nullptr;
+        void* d_phases = nullptr;
+    };
     int count = 0;
     std::vector<int> nq;
-    std::vector<Desc> desc;  // sorted by descending cost
-    std::vector<BOp> ops;
+    std::vector<std::vector<BatchOp>> flat;  // per state, absorbed, state-index positions
+    std::vector<int> order;                  // states by descending cost
+    std::vector<uint64_t> offset;            // amplitude offset of each state in the pool
+    std::map<int, Prog> progs;               // by LDS tile qubits
     uint64_t total_amps = 0;
     size_t source_ops = 0;
-    void* d_desc = nullptr;
-    void* d_ops = nullptr;
     float2* d_pool = nullptr;
     double* d_out = nullptr;
     unsigned* d_queue = nullptr;
@@ -218,6 +431,47 @@ struct aqs_batch_s {
 
 namespace {
 
+void free_prog(aqs_batch_s::Prog& p) {
+    if (p.d_desc) (void)hipFree(p.d_desc);
+    if (p.d_ops) (void)hipFree(p.d_ops);
+    if (p.d_phases) (void)hipFree(p.d_phases);
+    p.d_desc = p.d_ops = p.d_phases = nullptr;
+}
+
+// The program for LDS tiles of 2^m amplitudes, built and uploaded on first use.
+aqs_batch_s::Prog& prog_for(aqs_batch b, int m) {
+    auto it = b->progs.find(m);
+    if (it != b->progs.end()) return it->second;
+    aqs_batch_s::Prog& p = b->progs[m];
+    try {
+        for (int k = 0; k < b->count; ++k) {
+            const int s = b->order[k];
+            Desc d{};
+            d.amp_offset = b->offset[s];
+            d.phase_begin = (uint32_t)p.phases.size();
+            batch_phases(b->flat[s], b->nq[s], m, p.ops, p.phases);
+            d.phase_count = (uint32_t)(p.phases.size() - d.phase_begin);
+            d.n = b->nq[s];
+            d.index = s;
+            p.desc.push_back(d);
+        }
+        if (p.ops.size() >= (1ull << 32) || p.phases.size() >= (1ull << 32))
+            fail(AQS_ERR_INVALID_ARGUMENT, "batch program exceeds 2^32 ops");
+        AQS_HIP(hipMalloc(&p.d_desc, sizeof(Desc) * p.desc.size()));
+        AQS_HIP(hipMalloc(&p.d_ops, sizeof(BatchOp) * std::max<size_t>(1, p.ops.size())));
+        AQS_HIP(hipMalloc(&p.d_phases, sizeof(BatchPhase) * p.phases.size()));
+        AQS_HIP(hipMemcpy(p.d_desc, p.desc.data(), sizeof(Desc) * p.desc.size(), hipMemcpyHostToDevice));
+        if (!p.ops.empty())
+            AQS_HIP(hipMemcpy(p.d_ops, p.ops.data(), sizeof(BatchOp) * p.ops.size(), hipMemcpyHostToDevice));
+        AQS_HIP(hipMemcpy(p.d_phases, p.phases.data(), sizeof(BatchPhase) * p.phases.size(), hipMemcpyHostToDevice));
+    } catch (...) {
+        free_prog(p);
+        b->progs.erase(m);
+        throw;
+    }
+    return p;
+}
+
 void destroy_graph(aqs_batch b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
@@ -226,20 +480,24 @@ void destroy_graph(aqs_batch b) {
 }
 
 void launch_kernel(aqs_batch b, int block, int lds_qubits, int grid) {
+    aqs_batch_s::Prog& p = prog_for(b, lds_qubits);
     const size_t lds_bytes = sizeof(float2) << lds_qubits;
+    const auto* desc = (const Desc*)p.d_desc;
+    const auto* phases = (const BatchPhase*)p.d_phases;
+    const auto* ops = (const BatchOp*)p.d_ops;
     hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, b->stream, b->d_queue);
     switch (block) {
         case 256:
-            hipLaunchKernelGGL(k_batch<256>, dim3(grid), dim3(256), lds_bytes, b->stream, (const Desc*)b->d_desc, b->count,
-                               (const BOp*)b->d_ops, b->d_pool, b->d_out, b->d_queue, lds_qubits);
+            hipLaunchKernelGGL(k_batch<256>, dim3(grid), dim3(256), lds_bytes, b->stream, desc, b->count, phases, ops,
+                               b->d_pool, b->d_out, b->d_queue, lds_qubits);
             break;
         case 512:
-            hipLaunchKernelGGL(k_batch<512>, dim3(grid), dim3(512), lds_bytes, b->stream, (const Desc*)b->d_desc, b->count,
-                               (const BOp*)b->d_ops, b->d_pool, b->d_out, b->d_queue, lds_qubits);
+            hipLaunchKernelGGL(k_batch<512>, dim3(grid), dim3(512), lds_bytes, b->stream, desc, b->count, phases, ops,
+                               b->d_pool, b->d_out, b->d_queue, lds_qubits);
             break;
         default:
-            hipLaunchKernelGGL(k_batch<1024>, dim3(grid), dim3(1024), lds_bytes, b->stream, (const Desc*)b->d_desc,
-                               b->count, (const BOp*)b->d_ops, b->d_pool, b->d_out, b->d_queue, lds_qubits);
+            hipLaunchKernelGGL(k_batch<1024>, dim3(grid), dim3(1024), lds_bytes, b->stream, desc, b->count, phases, ops,
+                               b->d_pool, b->d_out, b->d_queue, lds_qubits);
             break;
     }
     AQS_HIP(hipGetLastError());
@@ -266,6 +524,7 @@ void set_lds_limit() {
 
 void build_graph(aqs_batch b) {
     destroy_graph(b);
+    prog_for(b, b->lds_qubits);  // allocates and uploads: not allowed while capturing
     AQS_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     try {
         launch_kernel(b, b->block, b->lds_qubits, b->grid);
@@ -279,16 +538,25 @@ void build_graph(aqs_batch b) {
     AQS_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
 }
 
+// Median of three timed replays of the configuration's captured graph (as
+// aqs_batch_run launches it) after one warm-up replay.  Leaves the batch
+// configured as (block, lds_qubits).
 double time_config(aqs_batch b, int block, int lds_qubits) {
-    const int grid = grid_for(block, lds_qubits);
-    launch_kernel(b, block, lds_qubits, grid);  // warm
-    AQS_HIP(hipEventRecord(b->e0, b->stream));
-    launch_kernel(b, block, lds_qubits, grid);
-    AQS_HIP(hipEventRecord(b->e1, b->stream));
-    AQS_HIP(hipEventSynchronize(b->e1));
-    float ms = 0.f;
-    AQS_HIP(hipEventElapsedTime(&ms, b->e0, b->e1));
-    return ms;
+    b->block = block;
+    b->lds_qubits = lds_qubits;
+    b->grid = grid_for(block, lds_qubits);
+    build_graph(b);
+    AQS_HIP(hipGraphLaunch(b->exec, b->stream));
+    float t[3];
+    for (float& ms : t) {
+        AQS_HIP(hipEventRecord(b->e0, b->stream));
+        AQS_HIP(hipGraphLaunch(b->exec, b->stream));
+        AQS_HIP(hipEventRecord(b->e1, b->stream));
+        AQS_HIP(hipEventSynchronize(b->e1));
+        AQS_HIP(hipEventElapsedTime(&ms, b->e0, b->e1));
+    }
+    std::sort(t, t + 3);
+    return t[1];
 }
 
 aqs_batch check_batch(aqs_batch b) {
@@ -312,97 +580,32 @@ aqs_status aqs_batch_create(int count, const int* nqubits, const aqs_op* ops, co
             b->nq.assign(nqubits, nqubits + count);
             b->source_ops = op_offsets[count];
             std::vector<uint64_t> cost(count);
-            std::vector<std::vector<BOp>> per(count);
+            b->flat.resize(count);
             for (int s = 0; s < count; ++s) {
                 const int n = nqubits[s];
                 if (n < 1 || n > 20) fail(AQS_ERR_INVALID_ARGUMENT, "batched states take 1..20 qubits");
-                std::vector<M2> pending(n);
-                std::vector<char> has(n, 0);
-                auto flush = [&](int q) {
-                    if (!has[q]) return;
-                    BOp o{};
-                    o.kind = AQS_OP_MAT1;
-                    o.t0 = n - 1 - q;
-                    for (int i = 0; i < 4; ++i) {
-                        o.m[2 * i] = (float)pending[q].a[i].real();
-                        o.m[2 * i + 1] = (float)pending[q].a[i].imag();
-                    }
-                    per[s].push_back(o);
-                    has[q] = 0;
-                };
-                for (size_t i = op_offsets[s]; i < op_offsets[s + 1]; ++i) {
-                    const aqs_op& op = ops[i];
-                    validate_op(op, n);
-                    if (op.kind == AQS_OP_MATK) fail(AQS_ERR_INVALID_ARGUMENT, "batched states take no MATK ops");
-                    const bool one_q =
-                        (op.kind == AQS_OP_MAT1 || op.kind == AQS_OP_DIAG1) && op.ncontrols == 0 && !op.more_controls;
-                    if (one_q) {
-                        const int q = op.targets[0];
-                        M2 m = mat_of(op);
-                        if (has[q]) {
-                            pending[q] = mul(m, pending[q]);
-                        } else {
-                            pending[q] = m;
-                            has[q] = 1;
-                        }
-                        continue;
-                    }
-                    for (int j = 0; j < op.ntargets; ++j) flush(op.targets[j]);
-                    for (int j = 0; j < op.ncontrols; ++j) flush(op.controls[j]);
-                    for (int q = 0; q < n; ++q)
-                        if ((op.more_controls >> q) & 1ull) flush(q);
-                    BOp o{};
-                    o.kind = op.kind;
-                    o.t0 = n - 1 - op.targets[0];
-                    o.t1 = op.ntargets > 1 ? n - 1 - op.targets[1] : 0;
-                    o.any = (op.flags & AQS_CTRL_ANY) ? 1 : 0;
-                    for (int j = 0; j < op.ncontrols; ++j) o.cmask |= 1u << (n - 1 - op.controls[j]);
-                    for (int q = 0; q < n; ++q)
-                        if ((op.more_controls >> q) & 1ull) o.cmask |= 1u << (n - 1 - q);
-                    if (op.kind == AQS_OP_PHASE) {
-                        o.pmask = o.cmask;
-                        for (int j = 0; j < op.ntargets; ++j) o.pmask |= 1u << (n - 1 - op.targets[j]);
-                    }
-                    for (int j = 0; j < 8; ++j) o.m[j] = (float)op.m[j];
-                    per[s].push_back(o);
-                }
-                for (int q = 0; q < n; ++q) flush(q);
-                cost[s] = (uint64_t)per[s].size() << n;
+                b->flat[s] = batch_absorb(ops + op_offsets[s], op_offsets[s + 1] - op_offsets[s], n);
+                cost[s] = (uint64_t)b->flat[s].size() << n;
             }
-            std::vector<int> order(count);
-            std::iota(order.begin(), order.end(), 0);
-            std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return cost[a] > cost[c]; });
-            std::vector<uint64_t> offset(count);
+            b->order.resize(count);
+            std::iota(b->order.begin(), b->order.end(), 0);
+            std::stable_sort(b->order.begin(), b->order.end(), [&](int a, int c) { return cost[a] > cost[c]; });
+            b->offset.resize(count);
             uint64_t amps = 0;
             for (int s = 0; s < count; ++s) {
-                offset[s] = amps;
+                b->offset[s] = amps;
                 amps += 1ull << nqubits[s];
             }
             b->total_amps = amps;
-            for (int k = 0; k < count; ++k) {
-                const int s = order[k];
-                Desc d{};
-                d.amp_offset = offset[s];
-                d.op_begin = (uint32_t)b->ops.size();
-                d.op_count = (uint32_t)per[s].size();
-                d.n = nqubits[s];
-                d.index = s;
-                b->desc.push_back(d);
-                b->ops.insert(b->ops.end(), per[s].begin(), per[s].end());
-            }
             current_device();
             AQS_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
             AQS_HIP(hipEventCreate(&b->e0));
             AQS_HIP(hipEventCreate(&b->e1));
-            AQS_HIP(hipMalloc(&b->d_desc, sizeof(Desc) * b->desc.size()));
-            AQS_HIP(hipMalloc(&b->d_ops, sizeof(BOp) * std::max<size_t>(1, b->ops.size())));
             AQS_HIP(hipMalloc(&b->d_pool, sizeof(float2) * amps));
             AQS_HIP(hipMalloc(&b->d_out, sizeof(double) * count));
             AQS_HIP(hipMalloc(&b->d_queue, 256));
-            AQS_HIP(hipMemcpy(b->d_desc, b->desc.data(), sizeof(Desc) * b->desc.size(), hipMemcpyHostToDevice));
-            if (!b->ops.empty())
-                AQS_HIP(hipMemcpy(b->d_ops, b->ops.data(), sizeof(BOp) * b->ops.size(), hipMemcpyHostToDevice));
             set_lds_limit();
+            prog_for(b, b->lds_qubits);
             b->grid = grid_for(b->block, b->lds_qubits);
         } catch (...) {
             aqs_batch_destroy(b);
@@ -431,9 +634,31 @@ aqs_status aqs_batch_autotune(aqs_batch b, int* block, int* lds_qubits, double* 
         b->grid = grid_for(bb, bl);
         b->tuned = true;
         destroy_graph(b);
+        for (auto it = b->progs.begin(); it != b->progs.end();) {
+            if (it->first == bl) {
+                ++it;
+                continue;
+            }
+            free_prog(it->second);
+            it = b->progs.erase(it);
+        }
         if (block) *block = bb;
         if (lds_qubits) *lds_qubits = bl;
         if (best_ms) *best_ms = best;
+    });
+}
+
+aqs_status aqs_batch_configure(aqs_batch b, int block, int lds_qubits) {
+    return guard([&] {
+        check_batch(b);
+        if (block != 256 && block != 512 && block != 1024)
+            fail(AQS_ERR_INVALID_ARGUMENT, "batch workgroup size must be 256, 512 or 1024");
+        if (lds_qubits < 12 || lds_qubits > 14) fail(AQS_ERR_INVALID_ARGUMENT, "batch LDS tile must be 12..14 qubits");
+        prog_for(b, lds_qubits);
+        b->block = block;
+        b->lds_qubits = lds_qubits;
+        b->grid = grid_for(block, lds_qubits);
+        destroy_graph(b);
     });
 }
 
@@ -469,7 +694,7 @@ aqs_status aqs_batch_state(aqs_batch b, int index, void* host) {
 aqs_status aqs_batch_info(aqs_batch b, size_t* passes, size_t* source_ops, int* block, int* lds_qubits, int* grid) {
     return guard([&] {
         check_batch(b);
-        if (passes) *passes = b->ops.size();
+        if (passes) *passes = prog_for(b, b->lds_qubits).ops.size();
         if (source_ops) *source_ops = b->source_ops;
         if (block) *block = b->block;
         if (lds_qubits) *lds_qubits = b->lds_qubits;
@@ -482,8 +707,7 @@ aqs_status aqs_batch_destroy(aqs_batch b) {
         if (!b) return;
         if (b->stream) (void)hipStreamSynchronize(b->stream);
         destroy_graph(b);
-        if (b->d_desc) (void)hipFree(b->d_desc);
-        if (b->d_ops) (void)hipFree(b->d_ops);
+        for (auto& kv : b->progs) free_prog(kv.second);
         if (b->d_pool) (void)hipFree(b->d_pool);
         if (b->d_out) (void)hipFree(b->d_out);
         if (b->d_queue) (void)hipFree(b->d_queue);

@@ -83,7 +83,7 @@ EXPORTS = [
     "aqs_norm2", "aqs_normalize", "aqs_prob1", "aqs_prob1_all", "aqs_expect_sum_z",
     "aqs_probabilities", "aqs_amplitude", "aqs_sample", "aqs_measure_qubit", "aqs_collapse_to",
     "aqs_inner", "aqs_profile_enable", "aqs_profile_read", "aqs_timer_start", "aqs_timer_stop",
-    "aqs_batch_create", "aqs_batch_autotune", "aqs_batch_run", "aqs_batch_state", "aqs_batch_info",
+    "aqs_batch_create", "aqs_batch_autotune", "aqs_batch_configure", "aqs_batch_run", "aqs_batch_state", "aqs_batch_info",
     "aqs_batch_destroy", "aqs_autotune", "aqs_tune_set", "aqs_server_create", "aqs_server_set_starvation_limit",
     "aqs_server_submit", "aqs_server_wait", "aqs_server_stats", "aqs_server_destroy",
 ]
@@ -157,6 +157,7 @@ def load(path: str = LIB_PATH):
         "aqs_timer_stop": [vp, dp],
         "aqs_batch_create": [i32, ctypes.POINTER(i32), ctypes.POINTER(Op), ctypes.POINTER(sz), ctypes.POINTER(vp)],
         "aqs_batch_autotune": [vp, ctypes.POINTER(i32), ctypes.POINTER(i32), dp],
+        "aqs_batch_configure": [vp, i32, i32],
         "aqs_batch_run": [vp, dp, dp],
         "aqs_batch_state": [vp, i32, vp],
         "aqs_batch_info": [vp, ctypes.POINTER(sz), ctypes.POINTER(sz), ctypes.POINTER(i32), ctypes.POINTER(i32),
@@ -440,6 +441,10 @@ class Batch:
         b, lq, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
         check(_lib.aqs_batch_autotune(self.h, ctypes.byref(b), ctypes.byref(lq), ctypes.byref(ms)))
         return {"block": b.value, "lds_qubits": lq.value, "ms": ms.value}
+
+    def configure(self, block, lds_qubits):
+        """Pins the workgroup size and the LDS tile (12..14 qubits)."""
+        check(_lib.aqs_batch_configure(self.h, int(block), int(lds_qubits)))
 
     def run(self):
         out = np.zeros(len(self.n), dtype=np.float64)

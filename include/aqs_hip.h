@@ -238,6 +238,9 @@ aqs_status aqs_batch_create(int count, const int* nqubits, const aqs_op* ops, co
                             aqs_batch* out);
 /* Times the (workgroup size, LDS tile) candidates and keeps the fastest. */
 aqs_status aqs_batch_autotune(aqs_batch b, int* block, int* lds_qubits, double* best_ms);
+/* Pins the workgroup size (256, 512 or 1024) and the LDS tile (12..14
+ * qubits).  States of more qubits than the tile run as phases of LDS tiles. */
+aqs_status aqs_batch_configure(aqs_batch b, int block, int lds_qubits);
 /* One persistent launch (replayed from a hipGraph) over all states;
  * expect_sum_z (host, count doubles, may be NULL) receives <sum_q Z_q> of
  * every final state in FP64; ms the device time of the launch. */

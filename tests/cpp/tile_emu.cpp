@@ -7,7 +7,9 @@
 // segment, per thread the 16 register amplitudes, with the op encodings
 // (register slots, thread-fixed positions, global bits) interpreted as the
 // kernel interprets them.  Plain passes and bit permutations are applied
-// directly.  block_emulate() does the same for the dense-block schedule of
+// directly.  batch_emulate() runs the program of one batched state
+// (afquantumsim_amd/csrc/batch.hip) phase by phase and tile by tile.
+// block_emulate() does the same for the dense-block schedule of
 // aqs_fuse, diagonal tables included (afquantumsim_amd/csrc/diag.hip:
 // factor = product over chunks of table[anchor value][chunk][chunk value]
 // on amplitudes whose required bits are all set).  The results are compared
@@ -16,6 +18,7 @@
 #include <cstring>
 #include <vector>
 
+#include "batch.h"
 #include "schedule.h"
 
 using namespace aqs_hip;
@@ -217,6 +220,42 @@ void tile_apply(std::vector<cplx>& v, int n, const TilePass& tp) {
     }
 }
 
+// One op of a batched state's program on a tile held as `tile` (local
+// positions), read as the batch kernel reads it.
+void batch_op_apply(std::vector<cplx>& tile, const BatchOp& op) {
+    const uint32_t T = (uint32_t)tile.size();
+    auto ok = [&](uint32_t i) { return !op.cmask || (op.any ? (i & op.cmask) != 0 : (i & op.cmask) == op.cmask); };
+    const cplx m00(op.m[0], op.m[1]), m01(op.m[2], op.m[3]), m10(op.m[4], op.m[5]), m11(op.m[6], op.m[7]);
+    const uint32_t t = 1u << op.t0;
+    for (uint32_t i = 0; i < T; ++i) {
+        switch (op.kind) {
+            case AQS_OP_MAT1:
+                if (!(i & t) && ok(i)) {
+                    const cplx a = tile[i], b = tile[i | t];
+                    tile[i] = m00 * a + m01 * b;
+                    tile[i | t] = m10 * a + m11 * b;
+                }
+                break;
+            case AQS_OP_X:
+                if (!(i & t) && ok(i)) std::swap(tile[i], tile[i | t]);
+                break;
+            case AQS_OP_DIAG1:
+                if (ok(i)) tile[i] *= (i & t) ? m11 : m00;
+                break;
+            case AQS_OP_PHASE:
+                if ((i & op.pmask) == op.pmask) tile[i] *= m00;
+                break;
+            case AQS_OP_SWAP: {
+                const uint32_t a = 1u << op.t0, b = 1u << op.t1;
+                if ((i & a) && !(i & b) && ok(i)) std::swap(tile[i], tile[i ^ a ^ b]);
+                break;
+            }
+            default:
+                throw 1;
+        }
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -293,6 +332,74 @@ int block_emulate(const aqs_op* ops, size_t count, int n, int kmax, double* stat
     counts[2] = tables;
     counts[3] = perms;
     aqs_schedule_destroy(sch);
+    return AQS_OK;
+}
+
+// Lowers the ops of one batched state for LDS tiles of 2^m amplitudes
+// (batch_absorb + batch_phases) and applies the program to `state` (2^n
+// complex FP64) on the host.  Each phase is read as the batch kernel reads
+// it: per tile t, local index j -> state index with zeros inserted at the
+// phase's outside bits (ascending) and t's bits placed there; controls
+// outside the tile resolved once per tile (all-of: every bit set, else the
+// op is skipped; any-of: one set bit drops the local controls, none and no
+// local controls skips the op).  The ops' float matrices are applied in FP64.
+// counts[0..2] = ops, phases, tile loads.
+int batch_emulate(const aqs_op* ops, size_t count, int n, int m, double* state, size_t* counts) {
+    std::vector<BatchOp> prog;
+    std::vector<BatchPhase> phases;
+    try {
+        batch_phases(batch_absorb(ops, count, n), n, m, prog, phases);
+    } catch (...) {
+        return -1;
+    }
+    std::vector<cplx> v((size_t)1 << n);
+    std::memcpy(v.data(), state, v.size() * sizeof(cplx));
+    const int local = n < m ? n : m;
+    std::vector<cplx> tile((size_t)1 << local);
+    size_t loads = 0;
+    for (const BatchPhase& ph : phases) {
+        std::vector<int> gpos;
+        for (int b = 0; b < 32; ++b)
+            if ((ph.gbits >> b) & 1u) gpos.push_back(b);
+        if ((int)gpos.size() != n - local) return -3;
+        auto state_index = [&](uint32_t j, uint32_t t) {
+            uint32_t i = 0;
+            int jb = 0, tb = 0;
+            for (int b = 0; b < n; ++b) {
+                const bool outside = (ph.gbits >> b) & 1u;
+                const uint32_t bit = outside ? (t >> tb++) & 1u : (j >> jb++) & 1u;
+                i |= bit << b;
+            }
+            return i;
+        };
+        for (uint32_t t = 0; t < (1u << gpos.size()); ++t) {
+            const uint32_t g = state_index(0, t);
+            for (uint32_t j = 0; j < tile.size(); ++j) tile[j] = v[state_index(j, t)];
+            for (uint32_t o = 0; o < ph.op_count; ++o) {
+                BatchOp e = prog[ph.op_begin + o];
+                if (e.gmask) {
+                    if (e.kind == AQS_OP_PHASE || !e.any) {
+                        if ((g & e.gmask) != e.gmask) continue;
+                    } else if (g & e.gmask) {
+                        e.cmask = 0;
+                    } else if (!e.cmask) {
+                        continue;
+                    }
+                }
+                try {
+                    batch_op_apply(tile, e);
+                } catch (int) {
+                    return -2;
+                }
+            }
+            for (uint32_t j = 0; j < tile.size(); ++j) v[state_index(j, t)] = tile[j];
+            ++loads;
+        }
+    }
+    std::memcpy(state, v.data(), v.size() * sizeof(cplx));
+    counts[0] = prog.size();
+    counts[1] = phases.size();
+    counts[2] = loads;
     return AQS_OK;
 }
 

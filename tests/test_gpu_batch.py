@@ -62,3 +62,39 @@ def test_batch_matches_single_state_path():
         st.apply(ops.lower_all(g))
         assert abs(st.expect_sum_z() - expect[i]) < 1e-5
         st.close()
+
+
+@pytest.mark.parametrize("block,lds_qubits", [(256, 12), (512, 13), (1024, 14), (1024, 12)])
+def test_batch_states_larger_than_the_tile(block, lds_qubits):
+    """States of up to 20 qubits run as phases of LDS tiles: ansatz circuits
+    and every op kind (controls inside and outside the tile, any-of controls,
+    many-controlled gates) against the FP64 oracle and the numpy restatement."""
+    from tests.test_batch_cpu import batch_ops, np_reference
+    spec = [(n, circuits.ry_rz_ladder(n, L, np.random.default_rng(n + L))) for n, L in
+            ((11, 5), (14, 6), (15, 8), (16, 12), (17, 3), (20, 2))]
+    b = hip.Batch([(n, ops.lower_all(g)) for n, g in spec])
+    b.configure(block, lds_qubits)
+    assert b.info()["lds_qubits"] == lds_qubits
+    expect, _ = b.run()
+    _check(spec, b, expect)
+    b.close()
+    mixed = [(n, batch_ops(n, 60, 100 + n)) for n in (12, 14, 15, 16, 18)]
+    b = hip.Batch(mixed)
+    b.configure(block, lds_qubits)
+    expect, _ = b.run()
+    for i, (n, op_list) in enumerate(mixed):
+        want = np_reference(op_list, n, oracle.basis_state(n, 0))
+        np.testing.assert_allclose(b.state(i), want, atol=2e-6, err_msg=f"state {i} n={n}")
+        assert abs(expect[i] - oracle.expect_sum_z(want, n)) < 1e-5
+    again, _ = b.run()
+    assert np.array_equal(expect, again)
+    b.close()
+
+
+def test_batch_configure_rejects_bad_shapes():
+    spec = circuits.variational_batch(2, nmin=4, nmax=5, lmin=1, lmax=2, seed=1)
+    b = hip.Batch([(n, ops.lower_all(g)) for n, g in spec])
+    for blk, lq in ((128, 13), (512, 11), (512, 15)):
+        with pytest.raises(RuntimeError):
+            b.configure(blk, lq)
+    b.close()

@@ -10,29 +10,45 @@
 
 namespace aqs_hip {
 
-// One pass of the batch kernel over an LDS-resident tile of a state.
-// Positions t0, t1, cmask and pmask are tile-local (they equal the
-// state-index positions when the whole state is one tile).  gmask holds the
-// controls (for a PHASE: the required bits) that lie outside the tile, in
-// state-index positions; they are resolved once per tile.
+// One gate of a batched state's program.  In a register segment t0 is a
+// register slot (0..3) and creg / preg are masks over slots; in a plain
+// segment t0, t1 are tile-local positions and creg = preg = 0.  cmask and
+// pmask are tile-local positions outside the segment's register bits;
+// gmask holds the controls (for a PHASE: the required bits) outside the tile
+// in state-index positions, resolved once per tile.
 struct BatchOp {
     int32_t kind;    // AQS_OP_MAT1 / DIAG1 / PHASE / X / SWAP
-    int32_t t0, t1;  // target positions (t1 for SWAP)
+    int32_t t0, t1;  // targets (t1 for SWAP, plain segments only)
     int32_t any;     // AQS_CTRL_ANY: at least one control bit must be set
-    uint32_t cmask;  // control bits inside the tile
-    uint32_t pmask;  // PHASE: bits inside the tile that must be set
+    uint32_t cmask;  // control bits inside the tile, outside the register bits
+    uint32_t pmask;  // PHASE: required bits inside the tile, outside the register bits
     uint32_t gmask;  // control / PHASE bits outside the tile
-    uint32_t pad;
+    uint16_t creg;   // control register slots
+    uint16_t preg;   // PHASE: required register slots
     float m[8];
 };
 
-// One sweep over a state: every tile of 2^m amplitudes that agree on the
-// state-index bits `gbits` is loaded into LDS, runs ops [op_begin,
-// op_begin + op_count) and is stored back.  The tile's local index lists the
-// other bits in ascending order.  A state of at most m qubits is one phase
-// with gbits = 0.
-struct BatchPhase {
+// Register segments: every thread holds 2^kBatchRegBits amplitudes of a
+// tile, the combinations of 4 tile-local bits (the slots), and runs the
+// segment's gates on them in registers between one LDS load and one store.
+constexpr int kBatchRegBits = 4;
+
+// A run of gates whose changed bits lie within 4 tile-local positions
+// (rmask, ascending = slots 0..3), or, with rmask = 0, gates applied one LDS
+// pass each.
+struct BatchSeg {
     uint32_t op_begin, op_count;
+    uint32_t rmask;
+    uint32_t pad;
+};
+
+// One sweep over a state: every tile of 2^m amplitudes that agree on the
+// state-index bits `gbits` is loaded into LDS, runs segments [seg_begin,
+// seg_begin + seg_count) and is stored back.  The tile's local index lists
+// the other bits in ascending order.  A state of at most m qubits is one
+// phase with gbits = 0.
+struct BatchPhase {
+    uint32_t seg_begin, seg_count;
     uint32_t gbits;
     uint32_t pad;
 };
@@ -46,11 +62,12 @@ std::vector<BatchOp> batch_absorb(const aqs_op* ops, size_t count, int n);
 // otherwise one phase).  Each phase keeps bits 0..4 in the tile (runs of at
 // least 32 amplitudes per load), adds the qubits the pending ops need in
 // order of first use, and takes every op whose non-diagonal bits are in the
-// tile and that commutes with each op it passes over.  Ops are appended to
-// `out` (tile-local positions), phases to `phases` (op_begin relative to
-// the start of `out`).
-void batch_phases(const std::vector<BatchOp>& flat, int n, int m, std::vector<BatchOp>& out,
-                  std::vector<BatchPhase>& phases);
+// tile and that commutes with each op it passes over.  A phase's ops are
+// then cut, in order, into register segments (swaps as three X gates) and,
+// for tiles of fewer than 4 bits or any-of controlled swaps, plain segments.
+// Ops, segments and phases are appended to the vectors (indices absolute).
+void batch_phases(const std::vector<BatchOp>& flat, int n, int m, std::vector<BatchOp>& ops,
+                  std::vector<BatchSeg>& segs, std::vector<BatchPhase>& phases);
 
 }  // namespace aqs_hip
 

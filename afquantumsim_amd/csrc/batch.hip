@@ -42,6 +42,11 @@ struct Desc {
 
 __device__ __forceinline__ uint32_t ins0(uint32_t p, int t) { return ((p >> t) << (t + 1)) | (p & ((1u << t) - 1u)); }
 
+// LDS slot of tile index i: one padding slot per 32 amplitudes, so threads
+// whose indices differ by 2^k (k >= 4) fall on different banks.  For i and
+// j with disjoint bits, lds_slot(i | j) = lds_slot(i) + lds_slot(j).
+__device__ __forceinline__ uint32_t lds_slot(uint32_t i) { return i + (i >> 5); }
+
 __device__ __forceinline__ bool ctrl_ok(uint32_t i, uint32_t cmask, int any) {
     if (!cmask) return true;
     return any ? (i & cmask) != 0 : (i & cmask) == cmask;
@@ -63,33 +68,36 @@ __device__ __forceinline__ uint32_t deposit(uint32_t t, uint32_t gbits) {
 }
 
 // Controls outside the tile, given the tile's bits g: false when the op does
-// nothing on this tile; an any-of control met outside drops the local ones.
+// nothing on this tile; an any-of control met outside makes it unconditional.
 __device__ __forceinline__ bool resolve(BatchOp& e, uint32_t g) {
     if (!e.gmask) return true;
     if (e.kind == AQS_OP_PHASE || !e.any) return (g & e.gmask) == e.gmask;
     if (g & e.gmask) {
+        e.any = 0;
         e.cmask = 0;
+        e.creg = 0;
         return true;
     }
-    return e.cmask != 0;
+    return (e.cmask | e.creg) != 0;
 }
 
+// One gate as one pass over the tile in LDS (plain segments).
 template <int BLOCK>
-__device__ void apply_block(float2* sv, uint32_t N, const BatchOp& op) {
+__device__ void apply_block(float2* lds, uint32_t N, const BatchOp& op) {
     const int tid = threadIdx.x;
     switch (op.kind) {
         case AQS_OP_MAT1: {
             for (uint32_t p = tid; p < N / 2; p += BLOCK) {
                 const uint32_t i0 = ins0(p, op.t0), i1 = i0 | (1u << op.t0);
                 if (!ctrl_ok(i0, op.cmask, op.any)) continue;
-                const float2 x = sv[i0], y = sv[i1];
+                const float2 x = lds[lds_slot(i0)], y = lds[lds_slot(i1)];
                 float2 a, b;
                 a.x = op.m[0] * x.x - op.m[1] * x.y + op.m[2] * y.x - op.m[3] * y.y;
                 a.y = op.m[0] * x.y + op.m[1] * x.x + op.m[2] * y.y + op.m[3] * y.x;
                 b.x = op.m[4] * x.x - op.m[5] * x.y + op.m[6] * y.x - op.m[7] * y.y;
                 b.y = op.m[4] * x.y + op.m[5] * x.x + op.m[6] * y.y + op.m[7] * y.x;
-                sv[i0] = a;
-                sv[i1] = b;
+                lds[lds_slot(i0)] = a;
+                lds[lds_slot(i1)] = b;
             }
             break;
         }
@@ -98,16 +106,16 @@ __device__ void apply_block(float2* sv, uint32_t N, const BatchOp& op) {
                 if (!ctrl_ok(i, op.cmask, op.any)) continue;
                 const bool one = (i >> op.t0) & 1u;
                 const float dr = one ? op.m[6] : op.m[0], di = one ? op.m[7] : op.m[1];
-                const float2 x = sv[i];
-                sv[i] = make_float2(dr * x.x - di * x.y, dr * x.y + di * x.x);
+                const float2 x = lds[lds_slot(i)];
+                lds[lds_slot(i)] = make_float2(dr * x.x - di * x.y, dr * x.y + di * x.x);
             }
             break;
         }
         case AQS_OP_PHASE: {
             for (uint32_t i = tid; i < N; i += BLOCK) {
                 if ((i & op.pmask) != op.pmask) continue;
-                const float2 x = sv[i];
-                sv[i] = make_float2(op.m[0] * x.x - op.m[1] * x.y, op.m[0] * x.y + op.m[1] * x.x);
+                const float2 x = lds[lds_slot(i)];
+                lds[lds_slot(i)] = make_float2(op.m[0] * x.x - op.m[1] * x.y, op.m[0] * x.y + op.m[1] * x.x);
             }
             break;
         }
@@ -115,9 +123,9 @@ __device__ void apply_block(float2* sv, uint32_t N, const BatchOp& op) {
             for (uint32_t p = tid; p < N / 2; p += BLOCK) {
                 const uint32_t i0 = ins0(p, op.t0), i1 = i0 | (1u << op.t0);
                 if (!ctrl_ok(i0, op.cmask, op.any)) continue;
-                const float2 x = sv[i0];
-                sv[i0] = sv[i1];
-                sv[i1] = x;
+                const float2 x = lds[lds_slot(i0)];
+                lds[lds_slot(i0)] = lds[lds_slot(i1)];
+                lds[lds_slot(i1)] = x;
             }
             break;
         }
@@ -126,15 +134,157 @@ __device__ void apply_block(float2* sv, uint32_t N, const BatchOp& op) {
             for (uint32_t i = tid; i < N; i += BLOCK) {
                 if (!(i & ma) || (i & mb) || !ctrl_ok(i, op.cmask, op.any)) continue;
                 const uint32_t j = i ^ ma ^ mb;
-                const float2 x = sv[i];
-                sv[i] = sv[j];
-                sv[j] = x;
+                const float2 x = lds[lds_slot(i)];
+                lds[lds_slot(i)] = lds[lds_slot(j)];
+                lds[lds_slot(j)] = x;
             }
             break;
         }
         default:
             break;
     }
+}
+
+constexpr int kRegs = 1 << kBatchRegBits;
+
+// Register ops.  Every variant assigns all 16 amplitudes in straight-line
+// code: a register whose controls fail gets identity coefficients (or keeps
+// its operand in a select) instead of a branch.  Partial updates under
+// branches make the compiler copy the whole array at every join, which
+// costs more than the selects.  Slot controls `creg` are uniform over the
+// workgroup; a lane's controls outside the registers (`p`) only select.
+
+// 2x2 [[m0 m1] [m2 m3]] (complex, interleaved in m) on slot S, no slot
+// controls.
+template <int S>
+__device__ __forceinline__ void reg_mat_u(float2 (&a)[kRegs], const float (&m)[8]) {
+#pragma unroll
+    for (int k = 0; k < kRegs; ++k) {
+        if ((k >> S) & 1) continue;
+        const float2 x = a[k], y = a[k | (1 << S)];
+        a[k] = make_float2(m[0] * x.x - m[1] * x.y + m[2] * y.x - m[3] * y.y,
+                           m[0] * x.y + m[1] * x.x + m[2] * y.y + m[3] * y.x);
+        a[k | (1 << S)] = make_float2(m[4] * x.x - m[5] * x.y + m[6] * y.x - m[7] * y.y,
+                                      m[4] * x.y + m[5] * x.x + m[6] * y.y + m[7] * y.x);
+    }
+}
+
+// The same for registers whose slots `creg` are all set (identity on the
+// others).
+template <int S>
+__device__ __forceinline__ void reg_mat_c(float2 (&a)[kRegs], const float (&m)[8], uint32_t creg) {
+#pragma unroll
+    for (int k = 0; k < kRegs; ++k) {
+        if ((k >> S) & 1) continue;
+        const bool ok = ((uint32_t)k & creg) == creg;
+        const float c0 = ok ? m[0] : 1.f, c1 = ok ? m[1] : 0.f, c2 = ok ? m[2] : 0.f, c3 = ok ? m[3] : 0.f;
+        const float c4 = ok ? m[4] : 0.f, c5 = ok ? m[5] : 0.f, c6 = ok ? m[6] : 1.f, c7 = ok ? m[7] : 0.f;
+        const float2 x = a[k], y = a[k | (1 << S)];
+        a[k] = make_float2(c0 * x.x - c1 * x.y + c2 * y.x - c3 * y.y, c0 * x.y + c1 * x.x + c2 * y.y + c3 * y.x);
+        a[k | (1 << S)] =
+            make_float2(c4 * x.x - c5 * x.y + c6 * y.x - c7 * y.y, c4 * x.y + c5 * x.x + c6 * y.y + c7 * y.x);
+    }
+}
+
+// Exchange across slot S on registers whose slots `creg` are all set, on
+// lanes with p.
+template <int S>
+__device__ __forceinline__ void reg_x(float2 (&a)[kRegs], uint32_t creg, bool p) {
+#pragma unroll
+    for (int k = 0; k < kRegs; ++k) {
+        if ((k >> S) & 1) continue;
+        const bool ok = p && ((uint32_t)k & creg) == creg;
+        const float2 x = a[k], y = a[k | (1 << S)];
+        a[k] = make_float2(ok ? y.x : x.x, ok ? y.y : x.y);
+        a[k | (1 << S)] = make_float2(ok ? x.x : y.x, ok ? x.y : y.y);
+    }
+}
+
+// One gate on a thread's registers; `base` is the tile index of register 0
+// (zero at the register bits).  (Register segments hold no any-of
+// controlled gates.)
+__device__ __forceinline__ void apply_regs(float2 (&a)[kRegs], const BatchOp& e, uint32_t base) {
+    const uint32_t outside = e.cmask | e.pmask;
+    const bool p = (base & outside) == outside;
+    if (e.kind == AQS_OP_X) {
+        switch (e.t0) {
+            case 0: reg_x<0>(a, e.creg, p); break;
+            case 1: reg_x<1>(a, e.creg, p); break;
+            case 2: reg_x<2>(a, e.creg, p); break;
+            default: reg_x<3>(a, e.creg, p); break;
+        }
+        return;
+    }
+    if (e.kind == AQS_OP_PHASE) {
+#pragma unroll
+        for (int k = 0; k < kRegs; ++k) {
+            const bool ok = p && ((uint32_t)k & e.preg) == e.preg;
+            const float zr = ok ? e.m[0] : 1.f, zi = ok ? e.m[1] : 0.f;
+            const float2 x = a[k];
+            a[k] = make_float2(zr * x.x - zi * x.y, zr * x.y + zi * x.x);
+        }
+        return;
+    }
+    // MAT1, or DIAG1 as a 2x2 with zero off-diagonal entries
+    const bool off = p && e.kind != AQS_OP_DIAG1;
+    const float m[8] = {p ? e.m[0] : 1.f, p ? e.m[1] : 0.f, off ? e.m[2] : 0.f, off ? e.m[3] : 0.f,
+                        off ? e.m[4] : 0.f, off ? e.m[5] : 0.f, p ? e.m[6] : 1.f, p ? e.m[7] : 0.f};
+    if (e.creg == 0) {
+        switch (e.t0) {
+            case 0: reg_mat_u<0>(a, m); break;
+            case 1: reg_mat_u<1>(a, m); break;
+            case 2: reg_mat_u<2>(a, m); break;
+            default: reg_mat_u<3>(a, m); break;
+        }
+    } else {
+        switch (e.t0) {
+            case 0: reg_mat_c<0>(a, m, e.creg); break;
+            case 1: reg_mat_c<1>(a, m, e.creg); break;
+            case 2: reg_mat_c<2>(a, m, e.creg); break;
+            default: reg_mat_c<3>(a, m, e.creg); break;
+        }
+    }
+}
+
+// A register segment over a tile of T amplitudes in LDS: every thread takes
+// groups of 16 amplitudes (the register bits' combinations), runs the
+// segment's gates on them and stores them back.
+template <int BLOCK>
+__device__ void run_reg_segment(float2* lds, uint32_t T, const BatchSeg& sg, const BatchOp* __restrict__ ops,
+                                uint32_t g) {
+    uint32_t pos[kBatchRegBits];
+    uint32_t r = sg.rmask;
+#pragma unroll
+    for (int s = 0; s < kBatchRegBits; ++s) {
+        pos[s] = __ffs(r) - 1;
+        r &= r - 1u;
+    }
+    uint32_t off[kRegs];
+#pragma unroll
+    for (int k = 0; k < kRegs; ++k) {
+        uint32_t o = 0;
+#pragma unroll
+        for (int s = 0; s < kBatchRegBits; ++s)
+            if ((k >> s) & 1) o |= 1u << pos[s];
+        off[k] = lds_slot(o);
+    }
+    for (uint32_t grp = threadIdx.x; grp < (T >> kBatchRegBits); grp += BLOCK) {
+        uint32_t base = grp;
+#pragma unroll
+        for (int s = 0; s < kBatchRegBits; ++s) base = ins0(base, pos[s]);
+        const uint32_t pb = lds_slot(base);
+        float2 a[kRegs];
+#pragma unroll
+        for (int k = 0; k < kRegs; ++k) a[k] = lds[pb + off[k]];
+        for (uint32_t o = 0; o < sg.op_count; ++o) {
+            BatchOp e = ops[sg.op_begin + o];
+            if (!resolve(e, g)) continue;
+            apply_regs(a, e, base);
+        }
+#pragma unroll
+        for (int k = 0; k < kRegs; ++k) lds[pb + off[k]] = a[k];
+    }
+    __syncthreads();
 }
 
 template <int BLOCK>
@@ -148,12 +298,16 @@ __device__ double block_sum(double acc, double* s_red) {
     return s;
 }
 
+// Persistent: each workgroup takes states from the queue until none is left.
+// A state of n qubits is swept in tiles of 2^min(n, lds_qubits) amplitudes;
+// its first phase starts from |0..0> in LDS, its last phase reduces
+// <sum Z> while storing.
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_batch(const Desc* __restrict__ desc, int count,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_batch(const Desc* __restrict__ desc, int count,
                                                   const BatchPhase* __restrict__ phases,
-                                                  const BatchOp* __restrict__ ops, float2* __restrict__ pool,
-                                                  double* __restrict__ out, unsigned* __restrict__ queue,
-                                                  int lds_qubits) {
+                                                  const BatchSeg* __restrict__ segs, const BatchOp* __restrict__ ops,
+                                                  float2* __restrict__ pool, double* __restrict__ out,
+                                                  unsigned* __restrict__ queue, int lds_qubits) {
     extern __shared__ float2 lds[];
     __shared__ int s_idx;
     __shared__ double s_red[BLOCK / 64];
@@ -161,55 +315,46 @@ __global__ __launch_bounds__(BLOCK) void k_batch(const Desc* __restrict__ desc, 
     while (true) {
         if (tid == 0) s_idx = (int)atomicAdd(queue, 1u);
         __syncthreads();
-        const int k = s_idx;
+        const int k = __builtin_amdgcn_readfirstlane(s_idx);
         __syncthreads();
         if (k >= count) break;  // every wave of the workgroup leaves together
         const Desc d = desc[k];
         float2* hbm = pool + d.amp_offset;
+        const int m = d.n < lds_qubits ? d.n : lds_qubits;
+        const uint32_t T = 1u << m, tiles = 1u << (d.n - m);
         double acc = 0.0;
-        if (d.n <= lds_qubits) {
-            // the whole state in LDS: one phase
-            const uint32_t N = 1u << d.n;
-            const BatchPhase ph = phases[d.phase_begin];
-            for (uint32_t i = tid; i < N; i += BLOCK) lds[i] = make_float2(i == 0 ? 1.f : 0.f, 0.f);
-            __syncthreads();
-            for (uint32_t o = 0; o < ph.op_count; ++o) {
-                apply_block<BLOCK>(lds, N, ops[ph.op_begin + o]);
+        for (uint32_t p = 0; p < d.phase_count; ++p) {
+            const BatchPhase ph = phases[d.phase_begin + p];
+            const bool last = p + 1 == d.phase_count;
+            for (uint32_t t = 0; t < tiles; ++t) {
+                const uint32_t g = deposit(t, ph.gbits);
+                if (p == 0) {
+                    for (uint32_t j = tid; j < T; j += BLOCK)
+                        lds[lds_slot(j)] = make_float2(t == 0 && j == 0 ? 1.f : 0.f, 0.f);
+                } else {
+                    for (uint32_t j = tid; j < T; j += BLOCK) lds[lds_slot(j)] = hbm[spread(j, ph.gbits) | g];
+                }
                 __syncthreads();
-            }
-            for (uint32_t i = tid; i < N; i += BLOCK) {
-                const float2 v = lds[i];
-                acc += ((double)v.x * v.x + (double)v.y * v.y) * (double)(d.n - 2 * (int)__popc(i));
-                hbm[i] = v;
-            }
-        } else {
-            const uint32_t T = 1u << lds_qubits, tiles = 1u << (d.n - lds_qubits);
-            for (uint32_t p = 0; p < d.phase_count; ++p) {
-                const BatchPhase ph = phases[d.phase_begin + p];
-                const bool last = p + 1 == d.phase_count;
-                for (uint32_t t = 0; t < tiles; ++t) {
-                    const uint32_t g = deposit(t, ph.gbits);
-                    if (p == 0) {
-                        for (uint32_t j = tid; j < T; j += BLOCK)
-                            lds[j] = make_float2(t == 0 && j == 0 ? 1.f : 0.f, 0.f);
-                    } else {
-                        for (uint32_t j = tid; j < T; j += BLOCK) lds[j] = hbm[spread(j, ph.gbits) | g];
+                for (uint32_t s = 0; s < ph.seg_count; ++s) {
+                    const BatchSeg sg = segs[ph.seg_begin + s];
+                    if (sg.rmask) {
+                        run_reg_segment<BLOCK>(lds, T, sg, ops, g);
+                        continue;
                     }
-                    __syncthreads();
-                    for (uint32_t o = 0; o < ph.op_count; ++o) {
-                        BatchOp e = ops[ph.op_begin + o];
+                    for (uint32_t o = 0; o < sg.op_count; ++o) {
+                        BatchOp e = ops[sg.op_begin + o];
                         if (!resolve(e, g)) continue;  // uniform over the workgroup
                         apply_block<BLOCK>(lds, T, e);
                         __syncthreads();
                     }
-                    for (uint32_t j = tid; j < T; j += BLOCK) {
-                        const float2 v = lds[j];
-                        const uint32_t i = spread(j, ph.gbits) | g;
-                        if (last) acc += ((double)v.x * v.x + (double)v.y * v.y) * (double)(d.n - 2 * (int)__popc(i));
-                        hbm[i] = v;
-                    }
-                    __syncthreads();  // the next tile's load reuses the LDS
                 }
+                for (uint32_t j = tid; j < T; j += BLOCK) {
+                    const float2 v = lds[lds_slot(j)];
+                    const uint32_t i = spread(j, ph.gbits) | g;
+                    if (last) acc += ((double)v.x * v.x + (double)v.y * v.y) * (double)(d.n - 2 * (int)__popc(i));
+                    hbm[i] = v;
+                }
+                __syncthreads();  // the next tile's load reuses the LDS
             }
         }
         const double s = block_sum<BLOCK>(acc, s_red);
@@ -285,6 +430,71 @@ uint32_t compress(uint32_t mask, uint32_t local) {
 
 int local_pos(int b, uint32_t local) { return __builtin_popcount(local & ((1u << b) - 1u)); }
 
+// Appends a phase's ops (tile-local positions, tile of `bits` bits) as
+// segments, in order: each register segment takes ops while the bits they
+// change fit in 4 positions (padded with the lowest free ones).  Gates move
+// to slot form; a swap of a and b under controls C becomes CX(a -> b),
+// (C and b) -> X on a, CX(a -> b).  Tiles of fewer than 4 bits and any-of
+// controlled gates go to plain segments.
+void cut_segments(const std::vector<BatchOp>& in, int bits, std::vector<BatchOp>& ops, std::vector<BatchSeg>& segs) {
+    auto plain = [&](const BatchOp& o) { return bits < kBatchRegBits || o.any; };
+    size_t i = 0;
+    while (i < in.size()) {
+        BatchSeg sg{(uint32_t)ops.size(), 0u, 0u, 0u};
+        if (plain(in[i])) {
+            for (; i < in.size() && plain(in[i]); ++i) {
+                ops.push_back(in[i]);
+                ++sg.op_count;
+            }
+            segs.push_back(sg);
+            continue;
+        }
+        uint32_t R = 0;
+        size_t j = i;
+        for (; j < in.size() && !plain(in[j]); ++j) {
+            const uint32_t need = uses_of(in[j]).need;
+            if (__builtin_popcount(R | need) > kBatchRegBits) break;
+            R |= need;
+        }
+        for (int b = 0; __builtin_popcount(R) < kBatchRegBits; ++b) R |= 1u << b;
+        sg.rmask = R;
+        auto slot = [&](int pos) { return __builtin_popcount(R & ((1u << pos) - 1u)); };
+        auto slots = [&](uint32_t mask) { return (uint16_t)compress(mask & R, R); };
+        for (; i < j; ++i) {
+            BatchOp o = in[i];
+            if (o.kind == AQS_OP_PHASE) {
+                o.preg = slots(o.pmask);
+                o.pmask &= ~R;
+                o.cmask = 0;
+                o.creg = 0;
+            } else if (o.kind == AQS_OP_SWAP) {
+                BatchOp cx{};
+                cx.kind = AQS_OP_X;
+                cx.t0 = slot(o.t1);
+                cx.creg = (uint16_t)(1u << slot(o.t0));
+                BatchOp mid = o;
+                mid.kind = AQS_OP_X;
+                mid.t0 = slot(o.t0);
+                mid.t1 = 0;
+                mid.creg = (uint16_t)(slots(o.cmask) | (1u << slot(o.t1)));
+                mid.cmask = o.cmask & ~R;
+                ops.push_back(cx);
+                ops.push_back(mid);
+                ops.push_back(cx);
+                sg.op_count += 3;
+                continue;
+            } else {
+                o.t0 = slot(o.t0);
+                o.creg = slots(o.cmask);
+                o.cmask &= ~R;
+            }
+            ops.push_back(o);
+            ++sg.op_count;
+        }
+        segs.push_back(sg);
+    }
+}
+
 }  // namespace
 
 namespace aqs_hip {
@@ -330,7 +540,7 @@ std::vector<BatchOp> batch_absorb(const aqs_op* ops, size_t count, int n) {
         o.kind = op.kind;
         o.t0 = n - 1 - op.targets[0];
         o.t1 = op.ntargets > 1 ? n - 1 - op.targets[1] : 0;
-        o.any = (op.flags & AQS_CTRL_ANY) ? 1 : 0;
+        o.any = (op.flags & AQS_CTRL_ANY) && op.ncontrols > 0 ? 1 : 0;
         for (int j = 0; j < op.ncontrols; ++j) o.cmask |= 1u << (n - 1 - op.controls[j]);
         for (int q = 0; q < n; ++q)
             if ((op.more_controls >> q) & 1ull) o.cmask |= 1u << (n - 1 - q);
@@ -345,11 +555,12 @@ std::vector<BatchOp> batch_absorb(const aqs_op* ops, size_t count, int n) {
     return out;
 }
 
-void batch_phases(const std::vector<BatchOp>& flat, int n, int m, std::vector<BatchOp>& out,
-                  std::vector<BatchPhase>& phases) {
+void batch_phases(const std::vector<BatchOp>& flat, int n, int m, std::vector<BatchOp>& ops,
+                  std::vector<BatchSeg>& segs, std::vector<BatchPhase>& phases) {
     if (n <= m) {
-        phases.push_back({(uint32_t)out.size(), (uint32_t)flat.size(), 0u, 0u});
-        out.insert(out.end(), flat.begin(), flat.end());
+        const uint32_t s0 = (uint32_t)segs.size();
+        cut_segments(flat, n, ops, segs);
+        phases.push_back({s0, (uint32_t)segs.size() - s0, 0u, 0u});
         return;
     }
     const uint32_t all = (n >= 32) ? ~0u : (1u << n) - 1u;
@@ -357,6 +568,7 @@ void batch_phases(const std::vector<BatchOp>& flat, int n, int m, std::vector<Ba
     for (size_t i = 0; i < flat.size(); ++i) use[i] = uses_of(flat[i]);
     std::vector<uint32_t> rem(flat.size()), next;
     std::iota(rem.begin(), rem.end(), 0u);
+    std::vector<BatchOp> taken;
     while (!rem.empty()) {
         uint32_t local = (1u << std::min(5, m)) - 1u;
         for (uint32_t i : rem) {
@@ -364,9 +576,9 @@ void batch_phases(const std::vector<BatchOp>& flat, int n, int m, std::vector<Ba
             if (__builtin_popcount(local) == m) break;
         }
         for (int b = 0; __builtin_popcount(local) < m; ++b) local |= 1u << b;
-        BatchPhase ph{(uint32_t)out.size(), 0u, all & ~local, 0u};
         uint32_t blocked_nd = 0, blocked_d = 0;
         next.clear();
+        taken.clear();
         for (uint32_t i : rem) {
             const Uses& u = use[i];
             const bool ok = !(u.need & ~local) && !(u.nd & (blocked_nd | blocked_d)) && !(u.d & blocked_nd);
@@ -387,10 +599,11 @@ void batch_phases(const std::vector<BatchOp>& flat, int n, int m, std::vector<Ba
                 o.gmask = o.cmask & ~local;
             }
             o.cmask = compress(o.cmask, local);
-            out.push_back(o);
-            ++ph.op_count;
+            taken.push_back(o);
         }
-        phases.push_back(ph);
+        const uint32_t s0 = (uint32_t)segs.size();
+        cut_segments(taken, m, ops, segs);
+        phases.push_back({s0, (uint32_t)segs.size() - s0, all & ~local, 0u});
         rem.swap(next);
     }
 }
@@ -402,9 +615,11 @@ struct aqs_batch_s {
     struct Prog {
         std::vector<Desc> desc;  // sorted by descending cost
         std::vector<BatchOp> ops;
+        std::vector<BatchSeg> segs;
         std::vector<BatchPhase> phases;
         void* d_desc = nullptr;
         void* d_ops = nullptr;
+        void* d_segs = nullptr;
         void* d_phases = nullptr;
     };
     int count = 0;
@@ -434,8 +649,9 @@ namespace {
 void free_prog(aqs_batch_s::Prog& p) {
     if (p.d_desc) (void)hipFree(p.d_desc);
     if (p.d_ops) (void)hipFree(p.d_ops);
+    if (p.d_segs) (void)hipFree(p.d_segs);
     if (p.d_phases) (void)hipFree(p.d_phases);
-    p.d_desc = p.d_ops = p.d_phases = nullptr;
+    p.d_desc = p.d_ops = p.d_segs = p.d_phases = nullptr;
 }
 
 // The program for LDS tiles of 2^m amplitudes, built and uploaded on first use.
@@ -449,20 +665,23 @@ aqs_batch_s::Prog& prog_for(aqs_batch b, int m) {
             Desc d{};
             d.amp_offset = b->offset[s];
             d.phase_begin = (uint32_t)p.phases.size();
-            batch_phases(b->flat[s], b->nq[s], m, p.ops, p.phases);
+            batch_phases(b->flat[s], b->nq[s], m, p.ops, p.segs, p.phases);
             d.phase_count = (uint32_t)(p.phases.size() - d.phase_begin);
             d.n = b->nq[s];
             d.index = s;
             p.desc.push_back(d);
         }
-        if (p.ops.size() >= (1ull << 32) || p.phases.size() >= (1ull << 32))
+        if (p.ops.size() >= (1ull << 32) || p.segs.size() >= (1ull << 32))
             fail(AQS_ERR_INVALID_ARGUMENT, "batch program exceeds 2^32 ops");
         AQS_HIP(hipMalloc(&p.d_desc, sizeof(Desc) * p.desc.size()));
         AQS_HIP(hipMalloc(&p.d_ops, sizeof(BatchOp) * std::max<size_t>(1, p.ops.size())));
+        AQS_HIP(hipMalloc(&p.d_segs, sizeof(BatchSeg) * std::max<size_t>(1, p.segs.size())));
         AQS_HIP(hipMalloc(&p.d_phases, sizeof(BatchPhase) * p.phases.size()));
         AQS_HIP(hipMemcpy(p.d_desc, p.desc.data(), sizeof(Desc) * p.desc.size(), hipMemcpyHostToDevice));
         if (!p.ops.empty())
             AQS_HIP(hipMemcpy(p.d_ops, p.ops.data(), sizeof(BatchOp) * p.ops.size(), hipMemcpyHostToDevice));
+        if (!p.segs.empty())
+            AQS_HIP(hipMemcpy(p.d_segs, p.segs.data(), sizeof(BatchSeg) * p.segs.size(), hipMemcpyHostToDevice));
         AQS_HIP(hipMemcpy(p.d_phases, p.phases.data(), sizeof(BatchPhase) * p.phases.size(), hipMemcpyHostToDevice));
     } catch (...) {
         free_prog(p);
@@ -479,24 +698,32 @@ void destroy_graph(aqs_batch b) {
     b->graph = nullptr;
 }
 
+// Dynamic LDS of a tile of 2^lds_qubits amplitudes with one padding slot
+// per 32 (lds_slot).
+size_t lds_bytes_for(int lds_qubits) {
+    const size_t T = 1ull << lds_qubits;
+    return sizeof(float2) * (T + (T >> 5));
+}
+
 void launch_kernel(aqs_batch b, int block, int lds_qubits, int grid) {
     aqs_batch_s::Prog& p = prog_for(b, lds_qubits);
-    const size_t lds_bytes = sizeof(float2) << lds_qubits;
+    const size_t lds_bytes = lds_bytes_for(lds_qubits);
     const auto* desc = (const Desc*)p.d_desc;
     const auto* phases = (const BatchPhase*)p.d_phases;
+    const auto* segs = (const BatchSeg*)p.d_segs;
     const auto* ops = (const BatchOp*)p.d_ops;
     hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, b->stream, b->d_queue);
     switch (block) {
         case 256:
-            hipLaunchKernelGGL(k_batch<256>, dim3(grid), dim3(256), lds_bytes, b->stream, desc, b->count, phases, ops,
+            hipLaunchKernelGGL(k_batch<256>, dim3(grid), dim3(256), lds_bytes, b->stream, desc, b->count, phases, segs, ops,
                                b->d_pool, b->d_out, b->d_queue, lds_qubits);
             break;
         case 512:
-            hipLaunchKernelGGL(k_batch<512>, dim3(grid), dim3(512), lds_bytes, b->stream, desc, b->count, phases, ops,
+            hipLaunchKernelGGL(k_batch<512>, dim3(grid), dim3(512), lds_bytes, b->stream, desc, b->count, phases, segs, ops,
                                b->d_pool, b->d_out, b->d_queue, lds_qubits);
             break;
         default:
-            hipLaunchKernelGGL(k_batch<1024>, dim3(grid), dim3(1024), lds_bytes, b->stream, desc, b->count, phases, ops,
+            hipLaunchKernelGGL(k_batch<1024>, dim3(grid), dim3(1024), lds_bytes, b->stream, desc, b->count, phases, segs, ops,
                                b->d_pool, b->d_out, b->d_queue, lds_qubits);
             break;
     }
@@ -507,7 +734,7 @@ int grid_for(int block, int lds_qubits) {
     int dev = current_device();
     hipDeviceProp_t prop;
     AQS_HIP(hipGetDeviceProperties(&prop, dev));
-    const size_t lds_bytes = sizeof(float2) << lds_qubits;
+    const size_t lds_bytes = lds_bytes_for(lds_qubits);
     const size_t lds_per_cu = 160 * 1024;
     int per_cu_lds = (int)std::max<size_t>(1, lds_per_cu / (lds_bytes + 1024));
     int per_cu_threads = std::max(1, 2048 / block);

@@ -220,8 +220,8 @@ void tile_apply(std::vector<cplx>& v, int n, const TilePass& tp) {
     }
 }
 
-// One op of a batched state's program on a tile held as `tile` (local
-// positions), read as the batch kernel reads it.
+// One op of a batched state's plain segment on a tile held as `tile`
+// (tile-local positions), read as the batch kernel reads it.
 void batch_op_apply(std::vector<cplx>& tile, const BatchOp& op) {
     const uint32_t T = (uint32_t)tile.size();
     auto ok = [&](uint32_t i) { return !op.cmask || (op.any ? (i & op.cmask) != 0 : (i & op.cmask) == op.cmask); };
@@ -254,6 +254,59 @@ void batch_op_apply(std::vector<cplx>& tile, const BatchOp& op) {
                 throw 1;
         }
     }
+}
+
+// One op of a register segment on the 16 amplitudes a thread holds (register
+// k = the combination k of the segment's 4 slots; `base` = tile index of
+// register 0), read as the batch kernel reads it: controls outside the
+// registers test `base`, slot controls test k (all-of; register segments
+// hold no any-of controlled gates, so one there is an error).
+void batch_reg_apply(cplx (&a)[16], const BatchOp& e, uint32_t base) {
+    const cplx m00(e.m[0], e.m[1]), m01(e.m[2], e.m[3]), m10(e.m[4], e.m[5]), m11(e.m[6], e.m[7]);
+    if (e.kind == AQS_OP_PHASE) {
+        if ((base & e.pmask) != e.pmask) return;
+        for (uint32_t k = 0; k < 16; ++k)
+            if ((k & e.preg) == e.preg) a[k] *= m00;
+        return;
+    }
+    if (e.any) throw 1;
+    if ((base & e.cmask) != e.cmask) return;
+    if (e.t0 < 0 || e.t0 > 3) throw 1;
+    const uint32_t t = 1u << e.t0;
+    for (uint32_t k = 0; k < 16; ++k) {
+        if ((k & e.creg) != e.creg) continue;
+        switch (e.kind) {
+            case AQS_OP_MAT1:
+                if (!(k & t)) {
+                    const cplx x = a[k], y = a[k | t];
+                    a[k] = m00 * x + m01 * y;
+                    a[k | t] = m10 * x + m11 * y;
+                }
+                break;
+            case AQS_OP_X:
+                if (!(k & t)) std::swap(a[k], a[k | t]);
+                break;
+            case AQS_OP_DIAG1:
+                a[k] *= (k & t) ? m11 : m00;
+                break;
+            default:
+                throw 1;
+        }
+    }
+}
+
+// Controls outside the tile, given the tile's bits g (as the kernel's
+// resolve): false when the op does nothing on this tile.
+bool batch_resolve(BatchOp& e, uint32_t g) {
+    if (!e.gmask) return true;
+    if (e.kind == AQS_OP_PHASE || !e.any) return (g & e.gmask) == e.gmask;
+    if (g & e.gmask) {
+        e.any = 0;
+        e.cmask = 0;
+        e.creg = 0;
+        return true;
+    }
+    return (e.cmask | e.creg) != 0;
 }
 
 }  // namespace
@@ -341,14 +394,19 @@ int block_emulate(const aqs_op* ops, size_t count, int n, int kmax, double* stat
 // it: per tile t, local index j -> state index with zeros inserted at the
 // phase's outside bits (ascending) and t's bits placed there; controls
 // outside the tile resolved once per tile (all-of: every bit set, else the
-// op is skipped; any-of: one set bit drops the local controls, none and no
-// local controls skips the op).  The ops' float matrices are applied in FP64.
-// counts[0..2] = ops, phases, tile loads.
+// op is skipped; any-of: one set bit makes the op unconditional, none and no
+// local controls skips it).  A plain segment applies its ops over the tile
+// one by one; a register segment gives each group of 16 amplitudes (the
+// combinations of its 4 slot bits) every op in turn.  A register-segment
+// op with any-of controls is an error (the kernel does not read them there).  The ops' float
+// matrices are applied in FP64.  counts[0..3] = ops, phases, tile loads,
+// register segments.
 int batch_emulate(const aqs_op* ops, size_t count, int n, int m, double* state, size_t* counts) {
     std::vector<BatchOp> prog;
+    std::vector<BatchSeg> segs;
     std::vector<BatchPhase> phases;
     try {
-        batch_phases(batch_absorb(ops, count, n), n, m, prog, phases);
+        batch_phases(batch_absorb(ops, count, n), n, m, prog, segs, phases);
     } catch (...) {
         return -1;
     }
@@ -356,12 +414,9 @@ int batch_emulate(const aqs_op* ops, size_t count, int n, int m, double* state, 
     std::memcpy(v.data(), state, v.size() * sizeof(cplx));
     const int local = n < m ? n : m;
     std::vector<cplx> tile((size_t)1 << local);
-    size_t loads = 0;
+    size_t loads = 0, reg_segs = 0;
     for (const BatchPhase& ph : phases) {
-        std::vector<int> gpos;
-        for (int b = 0; b < 32; ++b)
-            if ((ph.gbits >> b) & 1u) gpos.push_back(b);
-        if ((int)gpos.size() != n - local) return -3;
+        if (__builtin_popcount(ph.gbits) != n - local || (n < 32 && (ph.gbits >> n))) return -3;
         auto state_index = [&](uint32_t j, uint32_t t) {
             uint32_t i = 0;
             int jb = 0, tb = 0;
@@ -372,22 +427,42 @@ int batch_emulate(const aqs_op* ops, size_t count, int n, int m, double* state, 
             }
             return i;
         };
-        for (uint32_t t = 0; t < (1u << gpos.size()); ++t) {
+        for (uint32_t s = ph.seg_begin; s < ph.seg_begin + ph.seg_count; ++s) reg_segs += segs[s].rmask != 0;
+        for (uint32_t t = 0; t < (1u << (n - local)); ++t) {
             const uint32_t g = state_index(0, t);
             for (uint32_t j = 0; j < tile.size(); ++j) tile[j] = v[state_index(j, t)];
-            for (uint32_t o = 0; o < ph.op_count; ++o) {
-                BatchOp e = prog[ph.op_begin + o];
-                if (e.gmask) {
-                    if (e.kind == AQS_OP_PHASE || !e.any) {
-                        if ((g & e.gmask) != e.gmask) continue;
-                    } else if (g & e.gmask) {
-                        e.cmask = 0;
-                    } else if (!e.cmask) {
+            for (uint32_t s = ph.seg_begin; s < ph.seg_begin + ph.seg_count; ++s) {
+                const BatchSeg& sg = segs[s];
+                try {
+                    if (!sg.rmask) {
+                        for (uint32_t o = 0; o < sg.op_count; ++o) {
+                            BatchOp e = prog[sg.op_begin + o];
+                            if (batch_resolve(e, g)) batch_op_apply(tile, e);
+                        }
                         continue;
                     }
-                }
-                try {
-                    batch_op_apply(tile, e);
+                    if (__builtin_popcount(sg.rmask) != 4 || (sg.rmask >> local)) return -4;
+                    int pos[4];
+                    for (int q = 0, r = 0; r < 32; ++r)
+                        if ((sg.rmask >> r) & 1u) pos[q++] = r;
+                    for (uint32_t grp = 0; grp < (tile.size() >> 4); ++grp) {
+                        uint32_t base = 0;  // grp's bits at the non-slot positions, ascending
+                        for (int r = 0, gb = 0; r < local; ++r)
+                            if (!((sg.rmask >> r) & 1u)) base |= ((grp >> gb++) & 1u) << r;
+                        cplx a[16];
+                        auto at = [&](uint32_t k) {
+                            uint32_t i = base;
+                            for (int q = 0; q < 4; ++q)
+                                if ((k >> q) & 1u) i |= 1u << pos[q];
+                            return i;
+                        };
+                        for (uint32_t k = 0; k < 16; ++k) a[k] = tile[at(k)];
+                        for (uint32_t o = 0; o < sg.op_count; ++o) {
+                            BatchOp e = prog[sg.op_begin + o];
+                            if (batch_resolve(e, g)) batch_reg_apply(a, e, base);
+                        }
+                        for (uint32_t k = 0; k < 16; ++k) tile[at(k)] = a[k];
+                    }
                 } catch (int) {
                     return -2;
                 }
@@ -400,6 +475,34 @@ int batch_emulate(const aqs_op* ops, size_t count, int n, int m, double* state, 
     counts[0] = prog.size();
     counts[1] = phases.size();
     counts[2] = loads;
+    counts[3] = reg_segs;
+    return AQS_OK;
+}
+
+// The shape of one batched state's program, without running it:
+// out[0..5] = ops, phases, segments, register segments, register-segment
+// ops, tile loads.
+int batch_program_shape(const aqs_op* ops, size_t count, int n, int m, size_t* out) {
+    std::vector<BatchOp> prog;
+    std::vector<BatchSeg> segs;
+    std::vector<BatchPhase> phases;
+    try {
+        batch_phases(batch_absorb(ops, count, n), n, m, prog, segs, phases);
+    } catch (...) {
+        return -1;
+    }
+    size_t reg = 0, reg_ops = 0;
+    for (const BatchSeg& s : segs)
+        if (s.rmask) {
+            ++reg;
+            reg_ops += s.op_count;
+        }
+    out[0] = prog.size();
+    out[1] = phases.size();
+    out[2] = segs.size();
+    out[3] = reg;
+    out[4] = reg_ops;
+    out[5] = phases.size() << (n > m ? n - m : 0);
     return AQS_OK;
 }
 

@@ -4,7 +4,9 @@ tile run as phases of LDS tiles (afquantumsim_amd/csrc/batch.hip).
 tests/cpp/tile_emu.cpp's batch_emulate builds a state's program with the
 library's host code (batch_absorb + batch_phases) and reads every phase the
 way the batch kernel does: tile by tile, local index -> state index, controls
-outside the tile resolved per tile.  Expected states come from the numpy
+outside the tile resolved per tile, register segments group by group (16
+amplitudes over 4 slot bits, swaps as three X gates), plain segments op by
+op.  Expected states come from the numpy
 restatement of tests/test_many_controls.py (the state as a (2,)*n tensor) and
 from the FP64 oracle; the program rounds its matrices to FP32 once, so
 amplitudes agree to 1e-5, and permutation circuits bit for bit.
@@ -37,11 +39,11 @@ def emu():
 def emulate(emu, op_list, n, m, sv):
     arr, keep = hip.pack_ops(op_list)
     v = np.ascontiguousarray(sv, dtype=np.complex128).copy()
-    counts = (ctypes.c_size_t * 3)()
+    counts = (ctypes.c_size_t * 4)()
     rc = emu.batch_emulate(arr, len(op_list), n, m, v.ctypes.data, counts)
     del keep
     assert rc == 0, (rc, hip.load().aqs_last_error())
-    return v, {"ops": counts[0], "phases": counts[1], "tile_loads": counts[2]}
+    return v, {"ops": counts[0], "phases": counts[1], "tile_loads": counts[2], "register_segments": counts[3]}
 
 
 def batch_ops(n, count, seed, perm_only=False):
@@ -52,17 +54,21 @@ def batch_ops(n, count, seed, perm_only=False):
     rng = np.random.default_rng(seed)
     H = np.array([[1, 1], [1, -1]]) / np.sqrt(2)
     out = [] if perm_only else [{"kind": hip.OP_MAT1, "targets": (q,), "m": H.reshape(-1)} for q in range(n)]
-    kinds = ["x", "swap", "ccx", "or", "cswap"] + ([] if perm_only else ["u", "u", "cu", "cdiag", "phase", "diag"])
+    kinds = ["x", "swap", "ccx", "or", "cswap", "orswap"] + ([] if perm_only else
+                                                             ["u", "u", "cu", "cdiag", "phase", "diag"])
     for _ in range(count):
         k = str(rng.choice(kinds))
-        qs = [int(x) for x in rng.choice(n, 6, replace=False)]
+        qs = [int(x) for x in rng.choice(n, min(n, 6), replace=False)]
+        need = {"cu": 2, "cdiag": 2, "swap": 2, "ccx": 3, "or": 3, "cswap": 3, "orswap": 4}.get(k, 1)
+        if n < need:
+            k = "u"
         if k == "u":
             out.append({"kind": hip.OP_MAT1, "targets": (qs[0],), "m": unitary(2, rng).reshape(-1)})
         elif k == "diag":
             d = np.exp(1j * rng.uniform(-3, 3, size=2))
             out.append({"kind": hip.OP_DIAG1, "targets": (qs[0],), "m": np.array([d[0], 0, 0, d[1]])})
         elif k == "cu":
-            nc = int(rng.integers(1, 5))
+            nc = int(rng.integers(1, min(5, n)))
             out.append({"kind": hip.OP_MAT1, "targets": (qs[0],), "controls": tuple(qs[1:1 + nc]),
                         "m": unitary(2, rng).reshape(-1)})
         elif k == "cdiag":
@@ -70,11 +76,12 @@ def batch_ops(n, count, seed, perm_only=False):
             out.append({"kind": hip.OP_DIAG1, "targets": (qs[0],), "controls": (qs[1],),
                         "m": np.array([d[0], 0, 0, d[1]])})
         elif k == "phase":
-            nt, nc = int(rng.integers(1, 3)), int(rng.integers(0, 4))
+            nt = int(rng.integers(1, min(3, n + 1)))
+            nc = int(rng.integers(0, min(4, n - nt + 1)))
             out.append({"kind": hip.OP_PHASE, "targets": tuple(qs[:nt]), "controls": tuple(qs[nt:nt + nc]),
                         "m": np.array([np.exp(1j * rng.uniform(-3, 3)), 0, 0, 0])})
         elif k == "x":
-            nc = int(rng.integers(0, 6))
+            nc = int(rng.integers(0, len(qs)))
             out.append({"kind": hip.OP_X, "targets": (qs[0],), "controls": tuple(qs[1:1 + nc])})
         elif k == "ccx":
             out.append({"kind": hip.OP_X, "targets": (qs[0],), "controls": (qs[1], qs[2])})
@@ -82,20 +89,24 @@ def batch_ops(n, count, seed, perm_only=False):
             out.append({"kind": hip.OP_X, "targets": (qs[0],), "controls": (qs[1], qs[2]), "flags": hip.CTRL_ANY})
         elif k == "swap":
             out.append({"kind": hip.OP_SWAP, "targets": (qs[0], qs[1])})
-        else:
+        elif k == "orswap":
+            out.append({"kind": hip.OP_SWAP, "targets": (qs[0], qs[1]), "controls": (qs[2], qs[3]),
+                        "flags": hip.CTRL_ANY})
+        elif k == "cswap":
             out.append({"kind": hip.OP_SWAP, "targets": (qs[0], qs[1]), "controls": (qs[2],)})
     return out
 
 
 def np_reference(op_list, n, sv):
-    """numpy restatement; an any-of X (the reference's Or) as the sum of its
-    all-of cases: X if a, X if b, X if a and b (three flips = one)."""
+    """numpy restatement; an any-of controlled X or swap (the reference's Or)
+    as its all-of cases in turn: if a, if b, if a and b (an involution applied
+    three times is applied once)."""
     want = np.asarray(sv, dtype=complex).copy()
     for op in op_list:
         if op.get("flags", 0) & hip.CTRL_ANY:
             a, b = op["controls"]
             for c in ((a,), (b,), (a, b)):
-                want = np_run(want, n, [{"kind": hip.OP_X, "targets": op["targets"], "controls": c}])
+                want = np_run(want, n, [{"kind": op["kind"], "targets": op["targets"], "controls": c}])
         else:
             want = np_run(want, n, [op])
     return want
@@ -137,11 +148,24 @@ def test_ansatz_phases_per_layer(emu):
         np.testing.assert_allclose(got, want, atol=1e-5, rtol=0)
         assert cnt["ops"] <= L * (2 * n) + n
         assert cnt["phases"] <= 2 * L + 2, cnt
+        # the CX ladder moves 3 new qubits into the registers per segment
+        assert cnt["register_segments"] <= cnt["phases"] + L * ((n + 2) // 3 + 1), cnt
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6])
+def test_tiny_states(emu, n):
+    """Fewer than 4 qubits: plain segments; 4 and more: register segments
+    that pad their slots with free bits."""
+    op_list = batch_ops(n, 30, 40 + n)
+    sv = random_state(n, n)
+    got, cnt = emulate(emu, op_list, n, 12, sv)
+    np.testing.assert_allclose(got, np_reference(op_list, n, sv), atol=1e-5, rtol=0)
+    assert (cnt["register_segments"] == 0) == (n < 4)
 
 
 def test_rejects_matk(emu):
     op = {"kind": hip.OP_MATK, "targets": (0, 1), "u": np.eye(4)}
     arr, keep = hip.pack_ops([op])
     v = np.zeros(1 << 13, complex)
-    counts = (ctypes.c_size_t * 3)()
+    counts = (ctypes.c_size_t * 4)()
     assert emu.batch_emulate(arr, 1, 13, 12, v.ctypes.data, counts) != 0

@@ -159,12 +159,18 @@ int local_index(const std::vector<int>& qs, int q) {
     return -1;
 }
 
-std::vector<cd> op_matrix(const aqs_op& op, const std::vector<int>& qs) {
-    const int k = (int)qs.size();
-    const uint64_t K = 1ull << k;
+// op with its qubits renumbered to their positions in qs.
+aqs_op localize(const aqs_op& op, const std::vector<int>& qs) {
     aqs_op loc = op;
     for (int j = 0; j < op.ntargets; ++j) loc.targets[j] = local_index(qs, op.targets[j]);
     for (int j = 0; j < op.ncontrols; ++j) loc.controls[j] = local_index(qs, op.controls[j]);
+    return loc;
+}
+
+std::vector<cd> op_matrix(const aqs_op& op, const std::vector<int>& qs) {
+    const int k = (int)qs.size();
+    const uint64_t K = 1ull << k;
+    const aqs_op loc = localize(op, qs);
     std::vector<cd> M(K * K);
     std::vector<cd> e(K);
     for (uint64_t c = 0; c < K; ++c) {
@@ -174,31 +180,6 @@ std::vector<cd> op_matrix(const aqs_op& op, const std::vector<int>& qs) {
         for (uint64_t r = 0; r < K; ++r) M[r * K + c] = e[r];
     }
     return M;
-}
-
-std::vector<cd> embed(const Block& b, const std::vector<int>& qs) {
-    aqs_op op{};
-    op.kind = AQS_OP_MATK;
-    op.ntargets = (int)b.qubits.size();
-    for (int j = 0; j < op.ntargets; ++j) op.targets[j] = b.qubits[j];
-    std::vector<double> u(2 * b.U.size());
-    for (size_t i = 0; i < b.U.size(); ++i) {
-        u[2 * i] = b.U[i].real();
-        u[2 * i + 1] = b.U[i].imag();
-    }
-    op.u = u.data();
-    return op_matrix(op, qs);
-}
-
-std::vector<cd> matmul(const std::vector<cd>& A, const std::vector<cd>& B, uint64_t K) {
-    std::vector<cd> C(K * K);
-    for (uint64_t i = 0; i < K; ++i)
-        for (uint64_t l = 0; l < K; ++l) {
-            const cd a = A[i * K + l];
-            if (a == cd(0)) continue;
-            for (uint64_t j = 0; j < K; ++j) C[i * K + j] += a * B[l * K + j];
-        }
-    return C;
 }
 
 bool diagonal_matrix(const std::vector<cd>& U, uint64_t K) {
@@ -342,12 +323,49 @@ class Fuser {
         return U;
     }
 
+    // The blocks `hit` and then op, as one block on U.  Column c of the
+    // product is the basis vector c with each block (as a MATK on its
+    // qubits' positions in U) and then op applied: O(K^2) per 1-2 qubit
+    // gate, O(K^2 Kb) per merged block, instead of dense K^3 products.
     void merge(const aqs_op& op, const std::vector<size_t>& hit, const std::vector<int>& U) {
-        const uint64_t K = 1ull << U.size();
-        std::vector<cd> M(K * K);
-        for (uint64_t d = 0; d < K; ++d) M[d * K + d] = 1.0;
-        for (size_t b : hit) M = matmul(embed(open_[b], U), M, K);
-        M = matmul(op_matrix(op, U), M, K);
+        const int k = (int)U.size();
+        const uint64_t K = 1ull << k;
+        const aqs_op loc = localize(op, U);
+        std::vector<cd> M(K * K), col(K);
+        if (hit.size() == 1 && open_[hit[0]].qubits == U) {
+            // the gate lands in an open block on exactly these qubits
+            const std::vector<cd>& B = open_[hit[0]].U;
+            for (uint64_t c = 0; c < K; ++c) {
+                for (uint64_t r = 0; r < K; ++r) col[r] = B[r * K + c];
+                host_apply(col, k, loc);
+                for (uint64_t r = 0; r < K; ++r) M[r * K + c] = col[r];
+            }
+        } else {
+            std::vector<aqs_op> blocks;
+            std::vector<std::vector<double>> mats;
+            for (size_t b : hit) {
+                const Block& ob = open_[b];
+                aqs_op bo{};
+                bo.kind = AQS_OP_MATK;
+                bo.ntargets = (int)ob.qubits.size();
+                for (int j = 0; j < bo.ntargets; ++j) bo.targets[j] = local_index(U, ob.qubits[j]);
+                std::vector<double> u(2 * ob.U.size());
+                for (size_t i = 0; i < ob.U.size(); ++i) {
+                    u[2 * i] = ob.U[i].real();
+                    u[2 * i + 1] = ob.U[i].imag();
+                }
+                mats.push_back(std::move(u));
+                blocks.push_back(bo);
+            }
+            for (size_t j = 0; j < blocks.size(); ++j) blocks[j].u = mats[j].data();
+            for (uint64_t c = 0; c < K; ++c) {
+                std::fill(col.begin(), col.end(), cd(0));
+                col[c] = 1.0;
+                for (const aqs_op& bo : blocks) host_apply(col, k, bo);
+                host_apply(col, k, loc);
+                for (uint64_t r = 0; r < K; ++r) M[r * K + c] = col[r];
+            }
+        }
         for (size_t j = hit.size(); j-- > 0;) open_.erase(open_.begin() + hit[j]);
         Block nb;
         nb.qubits = U;

@@ -27,8 +27,11 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 # Packed FP32 instructions stay off in every device pass.
+# -fcx-limited-range: host complex products by the textbook formula (every
+# operand is a gate or unitary entry, |x| <= 1) instead of a library call
+# that rescues overflow and NaN.
 HIP_FLAGS = [
-    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-Wall",
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-Wall", "-fcx-limited-range",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", INCLUDE, "-I", CSRC,
 ]
@@ -71,7 +74,7 @@ def build_cpp() -> str:
     if not srcs:
         return ""
     tmp = AQS_LIB + ".tmp"
-    _run(["g++", "-O2", "-fPIC", "-std=c++17", "-Wall", "-shared", "-I", INCLUDE, "-o", tmp, *srcs,
+    _run(["g++", "-O2", "-fPIC", "-std=c++17", "-Wall", "-fcx-limited-range", "-shared", "-I", INCLUDE, "-o", tmp, *srcs,
           "-L", PKG, "-laqs_hip", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, AQS_LIB)
     return AQS_LIB

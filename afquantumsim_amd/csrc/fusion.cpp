@@ -121,6 +121,94 @@ void host_apply(std::vector<cd>& v, int k, const aqs_op& op) {
     }
 }
 
+// Apply an op to every column of a row-major 2^k x 2^k matrix M (M <- G M):
+// the op's pairs and groups are rows, each updated along its whole length.
+void apply_rows(std::vector<cd>& M, int k, const aqs_op& op) {
+    const uint64_t N = 1ull << k;
+    auto bit = [&](int q) { return 1ull << (k - 1 - q); };
+    uint64_t cmask = 0;
+    for (int j = 0; j < op.ncontrols; ++j) cmask |= bit(op.controls[j]);
+    const bool any = (op.flags & AQS_CTRL_ANY) != 0;
+    auto ctrl_ok = [&](uint64_t i) {
+        if (!cmask) return true;
+        return any ? (i & cmask) != 0 : (i & cmask) == cmask;
+    };
+    auto row = [&](uint64_t i) { return M.data() + i * N; };
+    const cd m00(op.m[0], op.m[1]), m01(op.m[2], op.m[3]), m10(op.m[4], op.m[5]), m11(op.m[6], op.m[7]);
+    switch (op.kind) {
+        case AQS_OP_MAT1:
+        case AQS_OP_DIAG1:
+        case AQS_OP_X: {
+            const uint64_t t = bit(op.targets[0]);
+            for (uint64_t i = 0; i < N; ++i) {
+                if ((i & t) || !ctrl_ok(i)) continue;
+                cd* a = row(i);
+                cd* b = row(i | t);
+                if (op.kind == AQS_OP_MAT1) {
+                    for (uint64_t c = 0; c < N; ++c) {
+                        const cd x = a[c], y = b[c];
+                        a[c] = m00 * x + m01 * y;
+                        b[c] = m10 * x + m11 * y;
+                    }
+                } else if (op.kind == AQS_OP_DIAG1) {
+                    for (uint64_t c = 0; c < N; ++c) {
+                        a[c] *= m00;
+                        b[c] *= m11;
+                    }
+                } else {
+                    std::swap_ranges(a, a + N, b);
+                }
+            }
+            break;
+        }
+        case AQS_OP_SWAP: {
+            const uint64_t a = bit(op.targets[0]), b = bit(op.targets[1]);
+            for (uint64_t i = 0; i < N; ++i)
+                if ((i & a) && !(i & b) && ctrl_ok(i)) std::swap_ranges(row(i), row(i) + N, row(i ^ a ^ b));
+            break;
+        }
+        case AQS_OP_PHASE: {
+            uint64_t mask = cmask;
+            for (int j = 0; j < op.ntargets; ++j) mask |= bit(op.targets[j]);
+            for (uint64_t i = 0; i < N; ++i)
+                if ((i & mask) == mask)
+                    for (uint64_t c = 0; c < N; ++c) row(i)[c] *= m00;
+            break;
+        }
+        case AQS_OP_MATK: {
+            const int kk = op.ntargets;
+            const uint64_t K = 1ull << kk;
+            std::vector<uint64_t> off(K);
+            uint64_t tmask = 0;
+            for (uint64_t r = 0; r < K; ++r) {
+                uint64_t o = 0;
+                for (int j = 0; j < kk; ++j)
+                    if ((r >> (kk - 1 - j)) & 1) o |= bit(op.targets[j]);
+                off[r] = o;
+            }
+            for (int j = 0; j < kk; ++j) tmask |= bit(op.targets[j]);
+            std::vector<cd> x(K * N);
+            for (uint64_t i = 0; i < N; ++i) {
+                if ((i & tmask) || !ctrl_ok(i)) continue;
+                for (uint64_t c = 0; c < K; ++c) std::copy(row(i | off[c]), row(i | off[c]) + N, x.data() + c * N);
+                for (uint64_t r = 0; r < K; ++r) {
+                    cd* out = row(i | off[r]);
+                    std::fill(out, out + N, cd(0));
+                    for (uint64_t c = 0; c < K; ++c) {
+                        const cd u(op.u[2 * (r * K + c)], op.u[2 * (r * K + c) + 1]);
+                        if (u == cd(0)) continue;
+                        const cd* xc = x.data() + c * N;
+                        for (uint64_t j = 0; j < N; ++j) out[j] += u * xc[j];
+                    }
+                }
+            }
+            break;
+        }
+        default:
+            break;
+    }
+}
+
 std::vector<int> op_qubits(const aqs_op& op) {
     std::vector<int> q;
     for (int j = 0; j < op.ntargets; ++j) q.push_back(op.targets[j]);
@@ -323,49 +411,47 @@ class Fuser {
         return U;
     }
 
-    // The blocks `hit` and then op, as one block on U.  Column c of the
-    // product is the basis vector c with each block (as a MATK on its
-    // qubits' positions in U) and then op applied: O(K^2) per 1-2 qubit
-    // gate, O(K^2 Kb) per merged block, instead of dense K^3 products.
+    // The blocks `hit` and then op, as one block on U.  The open blocks act
+    // on disjoint qubits, so together they are the Kronecker product of their
+    // unitaries (identity on U's other qubits): entry (r, c) is the product
+    // of each block's entry at r's and c's bits on its qubits.  op then
+    // updates the rows, O(K^2) per 1-2 qubit gate, instead of dense K^3
+    // products.
     void merge(const aqs_op& op, const std::vector<size_t>& hit, const std::vector<int>& U) {
         const int k = (int)U.size();
         const uint64_t K = 1ull << k;
         const aqs_op loc = localize(op, U);
-        std::vector<cd> M(K * K), col(K);
+        std::vector<cd> M;
         if (hit.size() == 1 && open_[hit[0]].qubits == U) {
-            // the gate lands in an open block on exactly these qubits
-            const std::vector<cd>& B = open_[hit[0]].U;
-            for (uint64_t c = 0; c < K; ++c) {
-                for (uint64_t r = 0; r < K; ++r) col[r] = B[r * K + c];
-                host_apply(col, k, loc);
-                for (uint64_t r = 0; r < K; ++r) M[r * K + c] = col[r];
-            }
+            M = std::move(open_[hit[0]].U);
         } else {
-            std::vector<aqs_op> blocks;
-            std::vector<std::vector<double>> mats;
-            for (size_t b : hit) {
-                const Block& ob = open_[b];
-                aqs_op bo{};
-                bo.kind = AQS_OP_MATK;
-                bo.ntargets = (int)ob.qubits.size();
-                for (int j = 0; j < bo.ntargets; ++j) bo.targets[j] = local_index(U, ob.qubits[j]);
-                std::vector<double> u(2 * ob.U.size());
-                for (size_t i = 0; i < ob.U.size(); ++i) {
-                    u[2 * i] = ob.U[i].real();
-                    u[2 * i + 1] = ob.U[i].imag();
+            // sub-index of each U-local index on each block's qubits
+            std::vector<std::vector<uint32_t>> sub(hit.size(), std::vector<uint32_t>(K));
+            uint64_t covered = 0;
+            for (size_t h = 0; h < hit.size(); ++h) {
+                const Block& ob = open_[hit[h]];
+                const int kb = (int)ob.qubits.size();
+                for (int j = 0; j < kb; ++j) covered |= 1ull << (k - 1 - local_index(U, ob.qubits[j]));
+                for (uint64_t r = 0; r < K; ++r) {
+                    uint32_t v = 0;
+                    for (int j = 0; j < kb; ++j)
+                        if ((r >> (k - 1 - local_index(U, ob.qubits[j]))) & 1ull) v |= 1u << (kb - 1 - j);
+                    sub[h][r] = v;
                 }
-                mats.push_back(std::move(u));
-                blocks.push_back(bo);
             }
-            for (size_t j = 0; j < blocks.size(); ++j) blocks[j].u = mats[j].data();
-            for (uint64_t c = 0; c < K; ++c) {
-                std::fill(col.begin(), col.end(), cd(0));
-                col[c] = 1.0;
-                for (const aqs_op& bo : blocks) host_apply(col, k, bo);
-                host_apply(col, k, loc);
-                for (uint64_t r = 0; r < K; ++r) M[r * K + c] = col[r];
-            }
+            M.assign(K * K, cd(0));
+            for (uint64_t r = 0; r < K; ++r)
+                for (uint64_t c = 0; c < K; ++c) {
+                    if ((r ^ c) & ~covered) continue;  // identity on qubits no block holds
+                    cd v = 1.0;
+                    for (size_t h = 0; h < hit.size(); ++h) {
+                        const Block& ob = open_[hit[h]];
+                        v *= ob.U[((uint64_t)sub[h][r] << ob.qubits.size()) + sub[h][c]];
+                    }
+                    M[r * K + c] = v;
+                }
         }
+        apply_rows(M, k, loc);
         for (size_t j = hit.size(); j-- > 0;) open_.erase(open_.begin() + hit[j]);
         Block nb;
         nb.qubits = U;

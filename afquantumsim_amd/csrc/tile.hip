@@ -78,9 +78,7 @@ __device__ __forceinline__ void butterfly(T2& p, T2& q, const R (&m)[8]) {
 }
 
 // Real 2x2 (m[0..3] row-major) on complex amplitudes: 8 multiplies per pair
-// instead of 16.  A general unitary reaches the tile as phase terms around
-// such a real rotation (its ZYZ form), the phases merged into the runs of
-// diagonal gates around it.
+// instead of 16, for one-qubit gates whose matrix is real (H, RotY).
 template <typename T2, typename R>
 __device__ __forceinline__ void butterfly_real(T2& p, T2& q, const R (&m)[8]) {
     T2 u, v;

@@ -64,7 +64,7 @@ std::vector<BatchOp> batch_absorb(const aqs_op* ops, size_t count, int n);
 // order of first use, and takes every op whose non-diagonal bits are in the
 // tile and that commutes with each op it passes over.  A phase's ops are
 // then cut, in order, into register segments (swaps as three X gates) and,
-// for tiles of fewer than 4 bits or any-of controlled swaps, plain segments.
+// for tiles of fewer than 4 bits or any-of controlled gates, plain segments.
 // Ops, segments and phases are appended to the vectors (indices absolute).
 void batch_phases(const std::vector<BatchOp>& flat, int n, int m, std::vector<BatchOp>& ops,
                   std::vector<BatchSeg>& segs, std::vector<BatchPhase>& phases);

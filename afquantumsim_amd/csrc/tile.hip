@@ -170,9 +170,24 @@ __device__ __forceinline__ void phases_direct(T2 (&x)[16], const TileTermDev* __
         R pr, pi;
         group_product<T2, R>(tt, z128, fold_val, k, g, lthr, base, pr, pi);
         k += g;
-#pragma unroll
-        for (int j = 0; j < 16; ++j)
-            if (((uint32_t)j & mr) == mr) x[j] = cmul<T2, R>(pr, pi, x[j]);
+        // one case per register mask: each multiplies a fixed set of
+        // registers in place (a runtime test per register made the compiler
+        // carry extra copies of the register array through the op loop)
+        switch (mr) {
+#define AQS_MASK_CASE(M)                                                      \
+    case M:                                                                   \
+        _Pragma("unroll") for (int j = 0; j < 16; ++j) if ((j & (M)) == (M)) \
+            x[j] = cmul<T2, R>(pr, pi, x[j]);                                 \
+        break;
+            AQS_MASK_CASE(0) AQS_MASK_CASE(1) AQS_MASK_CASE(2) AQS_MASK_CASE(3)
+            AQS_MASK_CASE(4) AQS_MASK_CASE(5) AQS_MASK_CASE(6) AQS_MASK_CASE(7)
+            AQS_MASK_CASE(8) AQS_MASK_CASE(9) AQS_MASK_CASE(10) AQS_MASK_CASE(11)
+            AQS_MASK_CASE(12) AQS_MASK_CASE(13) AQS_MASK_CASE(14)
+            default:  // mask 15
+                x[15] = cmul<T2, R>(pr, pi, x[15]);
+                break;
+#undef AQS_MASK_CASE
+        }
     }
 }
 

@@ -40,7 +40,7 @@ using namespace aqs_hip;
 
 namespace {
 
-constexpr int kTileAutoMinQubits = 22;
+constexpr int kTileAutoMinQubits = 20;
 
 // A phase term: multiply by z where every listed index bit is 1.
 struct Term {
@@ -654,8 +654,10 @@ aqs_status aqs_fuse_auto(const aqs_op* ops, size_t count, int nqubits, aqs_sched
     aqs_schedule blocks = nullptr, tiled = nullptr;
     aqs_status st = aqs_fuse(ops, count, nqubits, 5, &blocks);
     if (st != AQS_OK) return st;
-    // below 2^22 amplitudes the state stays in the caches and launches of
-    // the block schedule are cheaper than in-tile work
+    // below 2^20 amplitudes the state stays in the caches and launches of
+    // the block schedule are cheaper than in-tile work (measured: at 19
+    // qubits blocks win, at 20-21 tiles by 5-40% on brickwork and random
+    // layers)
     if (nqubits < kTileAutoMinQubits) {
         *out = blocks;
         return AQS_OK;

@@ -360,7 +360,7 @@ class Schedule:
     mode "blocks": dense blocks of <= max_qubits qubits (aqs_fuse);
     mode "tiled":  cache-blocked tile passes of 2^tile_qubits amplitudes
                    (aqs_fuse_tiled, tile_qubits 0 = 13);
-    mode "auto":   aqs_fuse_auto (blocks below 22 qubits, else the lower
+    mode "auto":   aqs_fuse_auto (blocks below 20 qubits, else the lower
                    modelled cost of the two).
     """
 

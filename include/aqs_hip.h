@@ -184,7 +184,7 @@ aqs_status aqs_fuse_tiled(const aqs_op* ops, size_t count, int nqubits, int tile
  * in-tile gate work of tile passes and the MFMA work of dense blocks in
  * sweep-time units.  Used to choose between aqs_fuse and aqs_fuse_tiled. */
 aqs_status aqs_schedule_cost(aqs_schedule sch, double* sweeps);
-/* compile()'s default: the aqs_fuse(.., 5, ..) schedule, or from 22 qubits
+/* compile()'s default: the aqs_fuse(.., 5, ..) schedule, or from 20 qubits
  * up the aqs_fuse_tiled(.., 0, ..) one when its modelled cost is lower. */
 aqs_status aqs_fuse_auto(const aqs_op* ops, size_t count, int nqubits, aqs_schedule* out);
 aqs_status aqs_schedule_destroy(aqs_schedule sch);

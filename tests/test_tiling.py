@@ -163,9 +163,9 @@ def test_tiled_cancelling_phases(emu):
 
 
 def test_auto_mode():
-    """aqs_fuse_auto: blocks below 22 qubits, else the lower modelled cost."""
-    small = ops.lower_all(circuits.brickwork(20, 6, seed=1))
-    assert hip.Schedule(small, 20, mode="auto").mode == "blocks"
+    """aqs_fuse_auto: blocks below 20 qubits, else the lower modelled cost."""
+    small = ops.lower_all(circuits.brickwork(19, 6, seed=1))
+    assert hip.Schedule(small, 19, mode="auto").mode == "blocks"
     for n, gates in ((24, circuits.brickwork(24, 8, seed=1)), (24, circuits.fourier_transform(24, True))):
         lowered = ops.lower_all(gates)
         a = hip.Schedule(lowered, n, mode="blocks")

@@ -117,6 +117,36 @@ __device__ __forceinline__ float2 cmul2(float2 f, float xr, float xi) {
 
 __device__ __forceinline__ uint32_t padded(uint32_t t) { return t + ((t >> 5) << 1); }
 
+// The parts of an amplitude's table lookup that depend only on its offset in
+// the tile (the anchor lies in the tile): anchor value g (bits 0-4), the
+// inside chunk's bits that lie in the tile (bits 5-9), and whether the
+// required bits that lie in the tile are set (bit 10).
+__device__ __forceinline__ uint32_t epi_lane_entry(const MfmaEpilogue& e, uint64_t off, uint64_t tilemask) {
+    const uint64_t rin = e.reqmask & tilemask;
+    uint32_t v = bits_of(off, e.gstart, e.gpos, e.ng);
+    if (e.inside >= 0) {
+        const int c = e.inside;
+        v |= bits_of(off, e.cstart[c], e.cpos[c], e.clen[c]) << 5;
+    }
+    if ((off & rin) == rin) v |= 1u << 10;
+    return v;
+}
+
+// Factor of an amplitude from its lane entry and the tile's part: the
+// required bits outside the tile set (req_out) and the inside chunk's bits
+// outside the tile (vout).
+__device__ __forceinline__ float2 epi_lane_factor(const MfmaEpilogue& e, const float2* pg, const float2* tin,
+                                                  uint32_t ent, bool req_out, uint32_t vout) {
+    if (!req_out || !(ent & (1u << 10))) return make_float2(1.f, 0.f);
+    const uint32_t g = ent & 31u;
+    float2 f = pg[g];
+    if (e.inside >= 0) {
+        const float2 t = tin[g * 32 + (((ent >> 5) & 31u) | vout)];
+        f = make_float2(f.x * t.x - f.y * t.y, f.x * t.y + f.y * t.x);
+    }
+    return f;
+}
+
 __device__ __forceinline__ uint64_t tile_base(uint64_t t, const MfmaArgs& a) {
 #pragma unroll
     for (int j = 0; j < 10; ++j) {
@@ -219,6 +249,19 @@ __global__ __launch_bounds__(64) void k_matk_mfma(MfmaArgs a) {
     const uint32_t rd_lane = a.tr[half] | a.tc[col];        // B rows 2s + half
     const uint32_t wr_lane = a.tr[4 * half] | a.tc[col];    // D rows 8(r/4) + 4 half + r%4
 
+    // table lookups of this lane's amplitudes: the in-tile parts, once
+    uint64_t tilemask = 0;
+    uint32_t ent[EPI ? CHUNKS : 1];
+    if (EPI) {
+        for (int j = 0; j < a.ntile_bits; ++j) tilemask |= 1ull << a.tpos[j];
+#pragma unroll
+        for (int j = 0; j < CHUNKS; ++j) {
+            const uint64_t off = (uint64_t)(a.gj[j] + glane) << 1;
+            ent[j] = epi_lane_entry(a.epi, off, tilemask) | (epi_lane_entry(a.epi, off | 1ull, tilemask) << 16);
+        }
+    }
+    const uint64_t rout = a.epi.reqmask & ~tilemask;
+
     uint64_t t = blockIdx.x;
     const uint64_t stride = gridDim.x;
     float4 pf[CHUNKS];
@@ -229,10 +272,25 @@ __global__ __launch_bounds__(64) void k_matk_mfma(MfmaArgs a) {
     }
     while (t < a.tiles) {
         const uint64_t b4 = tile_base(t, a) >> 1;
-        if (EPI) outside_factors(a, pg, b4 << 1, lane);
+        bool req_out = true;
+        uint32_t vout = 0;
+        if (EPI) {
+            outside_factors(a, pg, b4 << 1, lane);
+            req_out = ((b4 << 1) & rout) == rout;
+            if (a.epi.inside >= 0) {
+                const int c = a.epi.inside;
+                vout = bits_of(b4 << 1, a.epi.cstart[c], a.epi.cpos[c], a.epi.clen[c]);
+            }
+        }
         if (DIAG == 2) {
             __syncthreads();  // pg of this tile before the prologue reads it
-            prologue(a, pg, tin, pf, b4, glane);
+#pragma unroll
+            for (int j = 0; j < CHUNKS; ++j) {
+                const float2 f0 = epi_lane_factor(a.epi, pg, tin, ent[j] & 0xffffu, req_out, vout);
+                const float2 f1 = epi_lane_factor(a.epi, pg, tin, ent[j] >> 16, req_out, vout);
+                const float2 y0 = cmul2(f0, pf[j].x, pf[j].y), y1 = cmul2(f1, pf[j].z, pf[j].w);
+                pf[j] = make_float4(y0.x, y0.y, y1.x, y1.y);
+            }
         }
         // tile -> LDS (memory order)
 #pragma unroll
@@ -256,8 +314,8 @@ __global__ __launch_bounds__(64) void k_matk_mfma(MfmaArgs a) {
             const uint32_t tau = 2u * (uint32_t)(j * 64 + lane);
             float4 v = *reinterpret_cast<const float4*>(&lds[padded(tau)]);
             if (DIAG == 1) {
-                const uint64_t i0 = (b4 + a.gj[j] + glane) << 1;
-                const float2 f0 = epi_factor(a.epi, pg, tin, i0), f1 = epi_factor(a.epi, pg, tin, i0 | 1ull);
+                const float2 f0 = epi_lane_factor(a.epi, pg, tin, ent[j] & 0xffffu, req_out, vout);
+                const float2 f1 = epi_lane_factor(a.epi, pg, tin, ent[j] >> 16, req_out, vout);
                 const float2 y0 = cmul2(f0, v.x, v.y), y1 = cmul2(f1, v.z, v.w);
                 v = make_float4(y0.x, y0.y, y1.x, y1.y);
             }

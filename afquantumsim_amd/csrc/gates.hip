@@ -389,8 +389,11 @@ __global__ __launch_bounds__(BS) void k_phase_v4(PhaseArgs<float4, float> a) {
 
 // Phase whose required bits include some of bits 0..3: the amplitudes it
 // changes do not fill whole 128-byte lines, so it sweeps every line that
-// holds one (bits >= 4 fixed by insertion), writes whole lines back and
-// tests the low required bits per amplitude.
+// holds one (bits >= 4 fixed by insertion) and tests the low required bits
+// per amplitude.  With bit 3 required, the 64-byte half line without it is
+// neither read nor written (CZ on bits 2,3 at 28 qubits: 645 -> 486 us; on
+// bits 3,4: 344 -> 291 us).  Skipping 32- or 16-byte pieces instead was
+// slower (960 us for bits 0,2): HBM moves 64-byte granules.
 struct PhaseLineArgs {
     float4* sv;
     uint64_t units;
@@ -406,17 +409,19 @@ __global__ __launch_bounds__(BS) void k_phase_lines(PhaseLineArgs a) {
     const IT base = (IT)blockIdx.x * (IT)(BS * U) + (IT)threadIdx.x;
     float4 v[U];
     IT idx[U];
-    bool ok[U];
+    bool need[U];
+    const uint32_t gmask = a.lowmask & 8u;  // amplitude bit 3 selects a 64-byte half line
 #pragma unroll
     for (int j = 0; j < U; ++j) {
         const IT u = base + (IT)j * BS;
-        ok[j] = u < units;
         idx[j] = insert_zeros_t<IT>(u, a.fixed) | setmask;
-        if (ok[j]) v[j] = ld_nt(&a.sv[idx[j]]);
+        const uint32_t i0 = 2u * (uint32_t)(idx[j] & 7u);
+        need[j] = u < units && ((i0 | 1u) & gmask) == gmask;
+        if (need[j]) v[j] = ld_nt(&a.sv[idx[j]]);
     }
 #pragma unroll
     for (int j = 0; j < U; ++j) {
-        if (!ok[j]) continue;
+        if (!need[j]) continue;
         const uint32_t i0 = 2u * (uint32_t)(idx[j] & 7u);
         const bool c0 = (i0 & a.lowmask) == a.lowmask, c1 = ((i0 | 1u) & a.lowmask) == a.lowmask;
         float4 x = v[j], y = x;

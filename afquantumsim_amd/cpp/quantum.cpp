@@ -459,10 +459,37 @@ void QSimulator::simulate(const QCircuit& circuit) {
     circuit.run_compiled(statevector_.handle());
     QCircuit temp{qubit_count()};
     detail::Sink sink;
-    sink.state = statevector_.handle();
+    if (qubit_count() > 12) {
+        // one aqs_apply per op, as the gates emit them
+        sink.state = statevector_.handle();
+        temp.sink_ = &sink;
+        for (size_t i = circuit.cached_index_; i < circuit.gate_list().size(); ++i)
+            circuit.gate_list()[i]->operator()(temp);
+        temp.sink_ = nullptr;
+        return;
+    }
+    // Small states: the ops are collected and applied by one aqs_apply, which
+    // runs them in LDS-resident batches instead of one launch each.
+    std::vector<detail::OpRecord> rec;
+    sink.record = &rec;
     temp.sink_ = &sink;
-    for (size_t i = circuit.cached_index_; i < circuit.gate_list().size(); ++i) circuit.gate_list()[i]->operator()(temp);
+    auto flush = [&] {
+        std::vector<aqs_op> v;
+        v.reserve(rec.size());
+        for (const auto& r : rec) v.push_back(r.op);
+        check(aqs_apply(statevector_.handle(), v.data(), v.size()));
+    };
+    try {
+        for (size_t i = circuit.cached_index_; i < circuit.gate_list().size(); ++i)
+            circuit.gate_list()[i]->operator()(temp);
+    } catch (...) {
+        // the ops emitted before the failing gate still apply
+        temp.sink_ = nullptr;
+        flush();
+        throw;
+    }
     temp.sink_ = nullptr;
+    flush();
 }
 
 bool QSimulator::peek_measure(uint32_t qubit) const {
@@ -1183,6 +1210,18 @@ std::vector<double> unitary64(const QCircuit& internal) {
     return u;
 }
 
+// True when the K x K unitary (row-major, interleaved) is exactly the
+// identity except for its last diagonal entry.
+bool phase_on_all_ones(const std::vector<double>& u, size_t K) {
+    for (size_t r = 0; r < K; ++r)
+        for (size_t c = 0; c < K; ++c) {
+            if (r == K - 1 && c == K - 1) continue;
+            const double re = u[2 * (r * K + c)], im = u[2 * (r * K + c) + 1];
+            if (re != (r == c ? 1.0 : 0.0) || im != 0.0) return false;
+        }
+    return true;
+}
+
 void emit_subcircuit(QCircuit& qc, QCircuit& internal, uint32_t begin, int control) {
     internal.compile();
     const uint32_t k = internal.qubit_count();
@@ -1191,11 +1230,21 @@ void emit_subcircuit(QCircuit& qc, QCircuit& internal, uint32_t begin, int contr
         if (!cached) cached = std::make_shared<const std::vector<double>>(unitary64(internal));
         const std::vector<double>& u = *cached;
         aqs_op op{};
-        op.kind = AQS_OP_MATK;
         op.ntargets = (int)k;
         for (uint32_t j = 0; j < k; ++j) op.targets[j] = (int)(begin + j);
         if (control >= 0) op.controls[op.ncontrols++] = control;
-        op.u = u.data();
+        if (phase_on_all_ones(u, (size_t)1 << k)) {
+            // diag(1, ..., 1, z) exactly (a multi-controlled Z or phase, as in
+            // NControl_Gate): one PHASE op on the all-ones index of its
+            // qubits, a plain phase pass instead of a dense 2^k block
+            op.kind = AQS_OP_PHASE;
+            const size_t last = ((size_t)1 << k) - 1;
+            op.m[0] = u[2 * (last * ((size_t)1 << k) + last)];
+            op.m[1] = u[2 * (last * ((size_t)1 << k) + last) + 1];
+        } else {
+            op.kind = AQS_OP_MATK;
+            op.u = u.data();
+        }
         qc.emit(op);
     } else {
         emit_flattened(qc, internal, begin, control);

@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <exception>
 
 #include "internal.h"
 
@@ -981,6 +982,154 @@ void apply_masked_t(aqs_state s, const aqs_op& op) {
     AQS_HIP(hipGetLastError());
 }
 
+// ---------------------------------------------------------------------------
+// Small states (at most 12 qubits in complex64, 11 in complex128): one
+// workgroup loads the whole state into LDS, applies a run of up to CH ops
+// with a barrier between ops, and stores it back.  A 10-qubit Grover search
+// emits about 1560 ops; one launch per op made it launch-bound (about 3 us
+// per op), one launch per CH ops is not.
+constexpr int kSmallQubitsC64 = 12;
+constexpr int kSmallQubitsC128 = 11;
+constexpr int kSmallBlock = 512;
+
+// One op in amplitude-index bit masks (n <= 12).  Units are the N/2
+// indices with a zero inserted at bit `lo`; i0 = that index, i1 = i0 ^ flip.
+//   MAT1 / X / DIAG1: lo = flip = target; the pair (i0, i1) is updated.
+//   SWAP: lo = bit b, `allmask` also holds bit a; (i0, i1) are exchanged.
+//   PHASE: lo = flip = one required bit; i1 is multiplied when every bit of
+//   `allmask` (targets, controls, further controls) is set in it.
+// Controls: every bit of `allmask` set in i0 (PHASE: in i1) and, when
+// `anymask` is nonzero (AQS_CTRL_ANY), at least one of its bits.
+template <typename R>
+struct SmallOp {
+    uint32_t code;  // 0 MAT1, 1 X, 2 SWAP, 3 DIAG1, 4 PHASE
+    uint32_t lo, flip, allmask, anymask, pad;
+    R m[8];
+};
+
+template <typename R, int CH>
+struct SmallRunArgs {
+    void* sv;
+    uint32_t n, count;
+    SmallOp<R> ops[CH];
+};
+
+// 64 complex64 ops (3.6 KiB) or 40 complex128 ops (3.5 KiB) per launch keep
+// the arguments under 4 KiB.
+template <typename R>
+constexpr int small_chunk() { return sizeof(R) == 4 ? 64 : 40; }
+
+template <typename T2, typename R, int CH>
+__global__ __launch_bounds__(kSmallBlock) void k_small_run(SmallRunArgs<R, CH> a) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    T2* S = reinterpret_cast<T2*>(smem);
+    T2* sv = reinterpret_cast<T2*>(a.sv);
+    const uint32_t N = 1u << a.n, half = N >> 1;
+    for (uint32_t i = threadIdx.x; i < N; i += kSmallBlock) S[i] = sv[i];
+    __syncthreads();
+    for (uint32_t k = 0; k < a.count; ++k) {
+        const SmallOp<R>& op = a.ops[k];
+        const uint32_t lo = op.lo, low = (1u << lo) - 1u;
+        for (uint32_t u = threadIdx.x; u < half; u += kSmallBlock) {
+            const uint32_t i0 = ((u & ~low) << 1) | (u & low);
+            const uint32_t i1 = i0 ^ op.flip;
+            const uint32_t c = op.code == 4 ? i1 : i0;
+            if ((c & op.allmask) != op.allmask) continue;
+            if (op.anymask && !(c & op.anymask)) continue;
+            if (op.code == 4) {
+                const T2 y = S[i1];
+                T2 ny;
+                ny.x = op.m[0] * y.x - op.m[1] * y.y;
+                ny.y = op.m[0] * y.y + op.m[1] * y.x;
+                S[i1] = ny;
+                continue;
+            }
+            const T2 x = S[i0], y = S[i1];
+            if (op.code == 1 || op.code == 2) {
+                S[i0] = y;
+                S[i1] = x;
+            } else if (op.code == 0) {
+                T2 nx, ny;
+                nx.x = op.m[0] * x.x - op.m[1] * x.y + op.m[2] * y.x - op.m[3] * y.y;
+                nx.y = op.m[0] * x.y + op.m[1] * x.x + op.m[2] * y.y + op.m[3] * y.x;
+                ny.x = op.m[4] * x.x - op.m[5] * x.y + op.m[6] * y.x - op.m[7] * y.y;
+                ny.y = op.m[4] * x.y + op.m[5] * x.x + op.m[6] * y.y + op.m[7] * y.x;
+                S[i0] = nx;
+                S[i1] = ny;
+            } else {
+                T2 nx, ny;
+                nx.x = op.m[0] * x.x - op.m[1] * x.y;
+                nx.y = op.m[0] * x.y + op.m[1] * x.x;
+                ny.x = op.m[6] * y.x - op.m[7] * y.y;
+                ny.y = op.m[6] * y.y + op.m[7] * y.x;
+                S[i0] = nx;
+                S[i1] = ny;
+            }
+        }
+        __syncthreads();
+    }
+    for (uint32_t i = threadIdx.x; i < N; i += kSmallBlock) sv[i] = S[i];
+}
+
+// An op (validated, not MATK) in the small kernel's form.
+template <typename R>
+SmallOp<R> small_op(int n, const aqs_op& op) {
+    SmallOp<R> o{};
+    // PHASE takes its controls as required bits whatever the flag, as the
+    // per-op and batch kernels do
+    const bool any = (op.flags & AQS_CTRL_ANY) && op.ncontrols > 0 && op.kind != AQS_OP_PHASE;
+    uint32_t ctrl = 0;
+    for (int j = 0; j < op.ncontrols; ++j) ctrl |= (uint32_t)qbit(n, op.controls[j]);
+    const uint32_t req = (uint32_t)more_control_bits(n, op);
+    o.allmask = (any ? 0u : ctrl) | req;
+    o.anymask = any ? ctrl : 0u;
+    for (int i = 0; i < 8; ++i) o.m[i] = (R)op.m[i];
+    switch (op.kind) {
+        case AQS_OP_MAT1:
+        case AQS_OP_X:
+        case AQS_OP_DIAG1: {
+            const int t = qpos(n, op.targets[0]);
+            o.code = op.kind == AQS_OP_MAT1 ? 0u : (op.kind == AQS_OP_X ? 1u : 3u);
+            o.lo = (uint32_t)t;
+            o.flip = 1u << t;
+            break;
+        }
+        case AQS_OP_SWAP: {
+            const int pa = qpos(n, op.targets[0]), pb = qpos(n, op.targets[1]);
+            o.code = 2;
+            o.lo = (uint32_t)pb;
+            o.flip = (1u << pa) | (1u << pb);
+            o.allmask |= 1u << pa;
+            break;
+        }
+        default: {  // AQS_OP_PHASE
+            const int p = qpos(n, op.targets[0]);
+            o.code = 4;
+            o.lo = (uint32_t)p;
+            o.flip = 1u << p;
+            for (int j = 0; j < op.ntargets; ++j) o.allmask |= (uint32_t)qbit(n, op.targets[j]);
+            break;
+        }
+    }
+    return o;
+}
+
+template <typename T2, typename R>
+void launch_small_run(aqs_state s, const aqs_op* ops, size_t count) {
+    constexpr int CH = small_chunk<R>();
+    const size_t lds = s->N * sizeof(T2);
+    for (size_t b = 0; b < count; b += CH) {
+        SmallRunArgs<R, CH> a{};
+        a.sv = s->amps;
+        a.n = (uint32_t)s->n;
+        a.count = (uint32_t)std::min<size_t>(CH, count - b);
+        for (uint32_t k = 0; k < a.count; ++k) a.ops[k] = small_op<R>(s->n, ops[b + k]);
+        ProfScope ps(s, kSlotSmall);
+        hipLaunchKernelGGL((k_small_run<T2, R, CH>), dim3(1), dim3(kSmallBlock), lds, s->stream, a);
+        AQS_HIP(hipGetLastError());
+    }
+}
+
 }  // namespace
 
 namespace aqs_hip {
@@ -1044,11 +1193,37 @@ extern "C" aqs_status aqs_apply(aqs_state s, const aqs_op* ops, size_t count) {
     return guard([&] {
         if (!s || !s->amps) fail(AQS_ERR_INVALID_ARGUMENT, "null state handle");
         if (count && !ops) fail(AQS_ERR_INVALID_ARGUMENT, "null op array");
-        for (size_t i = 0; i < count; ++i) {
+        const bool small = s->n <= (s->precision == AQS_C64 ? kSmallQubitsC64 : kSmallQubitsC128);
+        for (size_t i = 0; i < count;) {
+            if (small && ops[i].kind != AQS_OP_MATK) {
+                // a run of two or more ops goes through the LDS-resident kernel
+                // (ops before an invalid one are applied, as on the per-op path)
+                size_t e = i;
+                std::exception_ptr err;
+                while (e < count && ops[e].kind != AQS_OP_MATK) {
+                    try {
+                        validate_op(ops[e], s->n);
+                    } catch (...) {
+                        err = std::current_exception();
+                        break;
+                    }
+                    ++e;
+                }
+                if (e - i >= 2) {
+                    if (s->precision == AQS_C64)
+                        launch_small_run<float2, float>(s, ops + i, e - i);
+                    else
+                        launch_small_run<double2, double>(s, ops + i, e - i);
+                    if (err) std::rethrow_exception(err);
+                    i = e;
+                    continue;
+                }
+            }
             // A staged MATK matrix must not be overwritten while a previous
             // kernel may still read it.
             if (ops[i].kind == AQS_OP_MATK && i > 0) AQS_HIP(hipStreamSynchronize(s->stream));
             apply_op(s, ops[i]);
+            ++i;
         }
     });
 }

@@ -64,6 +64,7 @@ constexpr int kSlotMfma = 7;
 constexpr int kSlotDiagTable = 8;
 constexpr int kSlotPermute = 9;
 constexpr int kSlotTile = 10;
+constexpr int kSlotSmall = 11;
 
 struct ProfileRec {
     hipEvent_t start, stop;

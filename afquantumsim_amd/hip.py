@@ -35,7 +35,7 @@ OP_PERMB = 8
 OP_TILE = 9
 CTRL_ANY = 1
 
-PROFILE_SLOTS = ["other", "MAT1", "DIAG1", "X", "SWAP", "MATK", "PHASE", "MATK_MFMA", "DIAGT", "PERMB", "TILE"]
+PROFILE_SLOTS = ["other", "MAT1", "DIAG1", "X", "SWAP", "MATK", "PHASE", "MATK_MFMA", "DIAGT", "PERMB", "TILE", "SMALL"]
 
 
 class AqsError(RuntimeError):

@@ -145,7 +145,9 @@ aqs_status aqs_state_download(aqs_state s, void* host);
 
 /* ---- gate application (QSimulator::simulate, src/quantum.cpp:277-291) --- */
 
-/* Apply ops in order, one HIP kernel pass each. */
+/* Apply ops in order, one HIP kernel pass each; on a state of at most 12
+ * qubits (complex64) or 11 (complex128), runs of two or more non-MATK ops
+ * are applied by one LDS-resident workgroup, up to 64 (40) ops per launch. */
 aqs_status aqs_apply(aqs_state s, const aqs_op* ops, size_t count);
 
 /* Launch-shape autotuner for the one-qubit pass at this state's size: times
@@ -303,8 +305,9 @@ aqs_status aqs_profile_enable(aqs_state s, int enable);
 /* Slots (arrays of AQS_PROFILE_SLOTS): 0 state preparation and measurement,
  * 1..6 the AQS_OP_* kind of a plain pass (MATK on the scalar kernel), 7 MATK
  * on MFMA, 8 diagonal table passes, 9 bit-permutation passes, 10 tile
- * passes.  Total ms and launch count since enable. */
-#define AQS_PROFILE_SLOTS 11
+ * passes, 11 runs of ops on a small state held in LDS (aqs_apply).  Total ms
+ * and launch count since enable. */
+#define AQS_PROFILE_SLOTS 12
 aqs_status aqs_profile_read(aqs_state s, double* ms_by_kind, uint64_t* launches_by_kind);
 /* Event pair on the state's stream around arbitrary work. */
 aqs_status aqs_timer_start(aqs_state s);

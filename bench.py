@@ -122,6 +122,8 @@ def batched(args):
         worst = max(worst, abs(st.expect_sum_z() - expect[i]))
         st.close()
     # plain per-gate path on a sample (one state at a time)
+    # the same states one at a time through aqs_apply (states of at most 12
+    # qubits run there as LDS-resident op runs, larger ones one launch per op)
     sample = lowered[:256]
     sg = sum(len(o) for _, o in sample)
     st_cache = {}

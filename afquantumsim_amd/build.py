@@ -37,7 +37,7 @@ HIP_FLAGS = [
 ]
 
 HIP_SOURCES = ["state.hip", "gates.hip", "mfma.hip", "diag.hip", "perm.hip", "tile.hip", "measure.hip", "batch.hip",
-               "fusion.cpp", "tiling.cpp", "serve.cpp"]
+               "fusion.cpp", "tiling.cpp", "serve_core.cpp", "serve.cpp"]
 CPP_SOURCES = ["quantum.cpp", "quantum_algo.cpp", "quantum_gates.cpp"]  # algo_bench.cpp is a program
 
 
@@ -80,12 +80,12 @@ def build_cpp() -> str:
     return AQS_LIB
 
 
-CPP_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "aqs_tests.cpp")
-CPP_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "aqs_tests")
+CPP_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "api_test.cpp")
+CPP_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "api_test")
 
 
 def build_cpp_tests() -> str:
-    """The C++ suite mirroring the reference's test/tests.cpp (runs on a GPU)."""
+    """The fixture-driven test program of the aqs:: C++ API (runs on a GPU)."""
     if not os.path.exists(CPP_TEST_SRC):
         return ""
     tmp = CPP_TEST_BIN + ".tmp"
@@ -125,6 +125,25 @@ def build_test_helpers() -> str:
     return TILE_EMU_LIB
 
 
+SERVE_CORE_SRC = os.path.join(CSRC, "serve_core.cpp")
+SERVE_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "serve_core_test.cpp")
+SERVE_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "serve_core_test")
+SERVE_TEST_TSAN_BIN = os.path.join(ROOT, "tests", "cpp", "serve_core_test_tsan")
+
+
+def build_serve_core_tests() -> str:
+    """Host tests of the server's scheduling core (a stand-in backend, no GPU),
+    plain and under ThreadSanitizer (host code only: g++, no device code)."""
+    if not os.path.exists(SERVE_TEST_SRC):
+        return ""
+    common = ["-std=c++17", "-Wall", "-I", INCLUDE, "-I", CSRC, SERVE_CORE_SRC, SERVE_TEST_SRC, "-lpthread"]
+    for out, flags in ((SERVE_TEST_BIN, ["-O2"]), (SERVE_TEST_TSAN_BIN, ["-O1", "-g", "-fsanitize=thread"])):
+        tmp = out + ".tmp"
+        _run(["g++", *flags, "-o", tmp, *common])
+        os.replace(tmp, out)
+    return SERVE_TEST_BIN
+
+
 def build_all() -> None:
     if not os.path.exists(HIPCC) and not shutil.which("hipcc"):
         raise RuntimeError("hipcc not found")
@@ -133,6 +152,7 @@ def build_all() -> None:
     build_cpp_tests()
     build_algo_bench()
     build_test_helpers()
+    build_serve_core_tests()
 
 
 if __name__ == "__main__":

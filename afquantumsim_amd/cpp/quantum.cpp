@@ -1286,7 +1286,7 @@ Gate::Gate(const QCircuit& circuit_, uint32_t target_qubit_begin_, std::string n
 bool Gate::check(const QCircuit& qc) const {
     const auto qubits = qc.qubit_count();
     if (target_qubit_begin >= qubits) throw std::out_of_range{"Cannot add gate at the given qubit position"};
-    if (target_qubit_begin + qubit_count > qubits) throw std::out_of_range{"Gate must fit inside the circuit qubit count"};
+    if (target_qubit_begin + qubit_count > qubits) throw std::out_of_range{"Cannot add gate at the given qubit position"};
     return true;
 }
 

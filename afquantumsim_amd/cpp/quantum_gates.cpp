@@ -23,6 +23,11 @@ QCircuit Group_Gate(uint32_t qubits, std::vector<uint32_t> target_qubits, QCircu
     return qc;
 }
 
+// Targets below the control form one controlled sub-circuit, targets above
+// it another.  Each sub-circuit holds one copy of `gate` per target in that
+// group, on consecutive qubits, and is placed at the group's first target
+// (src/quantum_gates.cpp:55-72): a group whose targets are not consecutive,
+// e.g. {0, 2} with the control at 3, acts on qubits 0 and 1.
 QCircuit Control_Group_Gate(uint32_t qubits, uint32_t control_qubit, std::vector<uint32_t> target_qubits,
                             const QCircuit& gate, bool compile) {
     if (control_qubit >= qubits) throw std::invalid_argument{"Invalid control qubit position"};
@@ -33,23 +38,23 @@ QCircuit Control_Group_Gate(uint32_t qubits, uint32_t control_qubit, std::vector
     if (pivot != target_qubits.cend() && *pivot == control_qubit)
         throw std::invalid_argument{"Cannot add control gate at the target qubit positions"};
     QCircuit qc(qubits);
-    // targets above the control form one controlled sub-circuit, targets below another
-    const auto top = (uint32_t)std::distance(target_qubits.cbegin(), pivot);
-    const auto bottom = (uint32_t)std::distance(pivot, target_qubits.cend());
-    if (top) {
-        QCircuit temp(target_qubits[top - 1] - target_qubits.front() + 1);
-        for (auto it = target_qubits.cbegin(); it != pivot; ++it) temp << Gate(gate, *it - target_qubits.front());
-        qc << ControlGate(temp, control_qubit, target_qubits.front());
-    }
-    if (bottom) {
-        QCircuit temp(target_qubits.back() - *pivot + 1);
-        for (auto it = pivot; it != target_qubits.cend(); ++it) temp << Gate(gate, *it - *pivot);
-        qc << ControlGate(temp, control_qubit, *pivot);
-    }
+    auto add_group = [&](std::vector<uint32_t>::const_iterator first, std::vector<uint32_t>::const_iterator last) {
+        const auto width = (uint32_t)std::distance(first, last);
+        if (width == 0) return;
+        QCircuit group(width);
+        for (uint32_t j = 0; j < width; ++j) group << Gate(gate, j);
+        qc << ControlGate(group, control_qubit, *first);
+    };
+    add_group(target_qubits.cbegin(), pivot);
+    add_group(pivot, target_qubits.cend());
     if (compile) qc.compile();
     return qc;
 }
 
+// The controls are consecutive, directly above the target range.  The
+// innermost ControlGate takes the last control; each further one wraps the
+// circuit built so far with a control on its new qubit 0, and the result is
+// placed at control_qubit_begin (src/quantum_gates.cpp:96-111).
 QCircuit NControl_Gate(uint32_t qubits, uint32_t control_qubit_begin, uint32_t control_qubit_count,
                        uint32_t target_qubit_begin, const QCircuit& gate, bool compile) {
     if (gate.qubit_count() >= qubits) throw std::invalid_argument{"Gate not supported"};
@@ -58,13 +63,37 @@ QCircuit NControl_Gate(uint32_t qubits, uint32_t control_qubit_begin, uint32_t c
         throw std::invalid_argument{"Invalid target qubit_begin position"};
     if (control_qubit_begin + control_qubit_count > target_qubit_begin)
         throw std::invalid_argument{"Invalid control_qubit position"};
-    std::vector<uint32_t> controls;
-    for (uint32_t i = 0; i < control_qubit_count; ++i) controls.push_back(control_qubit_begin + i);
-    return NControl_Gate(qubits, controls, target_qubit_begin, gate, compile);
+    QCircuit qc(qubits);
+    if (control_qubit_count == 1) {
+        qc << ControlGate(gate, control_qubit_begin, target_qubit_begin);
+    } else {
+        // qubit 0 of `inner` is the last control; the gap up to the targets stays idle
+        const uint32_t gap = target_qubit_begin - (control_qubit_begin + control_qubit_count - 1);
+        QCircuit inner(gate.qubit_count() + gap);
+        inner << ControlGate(gate, 0, gap);
+        for (uint32_t added = 1; added < control_qubit_count; ++added) {
+            QCircuit wider(inner.qubit_count() + 1);
+            wider << ControlGate(inner, 0, 1);
+            inner = std::move(wider);
+        }
+        qc << Gate(inner, control_qubit_begin);
+    }
+    if (compile) qc.compile();
+    return qc;
 }
 
-// Controls are added one at a time, each wrapping the circuit built so far in
-// a ControlGate, first those below the target range, then those above it.
+// The reference's construction, step for step (src/quantum_gates.cpp:116-178),
+// including the width and placement it gives:
+//   - controls below the targets are added innermost first; the last of them
+//     widens the circuit to qubits - target_qubit_begin qubits, counted from
+//     target_qubit_begin;
+//   - controls above the targets are then added from the nearest to the
+//     farthest; the farthest places the circuit built so far at
+//     qubits - (its width).
+// So with controls only below the targets the result has
+// qubits - target_qubit_begin qubits (its qubit 0 is target_qubit_begin), and
+// with one control above and none below the gate lands on the last
+// gate.qubit_count() qubits whatever target_qubit_begin says.
 QCircuit NControl_Gate(uint32_t qubits, std::vector<uint32_t> control_qubits, uint32_t target_qubit_begin,
                        const QCircuit& gate, bool compile) {
     if (control_qubits.empty()) throw std::invalid_argument{"Number of control qubits must be at least one"};
@@ -77,27 +106,33 @@ QCircuit NControl_Gate(uint32_t qubits, std::vector<uint32_t> control_qubits, ui
     auto pivot = std::lower_bound(control_qubits.begin(), control_qubits.end(), target_qubit_begin);
     if (pivot != control_qubits.end() && *pivot < target_qubit_begin + gate.qubit_count())
         throw std::invalid_argument{"Cannot add control gate at the target qubit positions"};
-    std::vector<uint32_t> above(control_qubits.begin(), pivot);
-    std::vector<uint32_t> below(pivot, control_qubits.end());
-    // `current` spans qubits [lo, lo + current.qubit_count()) of the result
+    const std::vector<uint32_t> above(control_qubits.begin(), pivot);
+    const std::vector<uint32_t> below(pivot, control_qubits.end());
+
     QCircuit current = gate;
-    uint32_t lo = target_qubit_begin;
-    for (uint32_t c : below) {
-        QCircuit wider(c - lo + 1);
-        wider << ControlGate(current, c - lo, 0);
+    for (size_t i = 0; i < below.size(); ++i) {
+        // relative to target_qubit_begin; the last control widens to the end
+        const uint32_t c = below[i] - target_qubit_begin;
+        const uint32_t width = (i + 1 < below.size()) ? c + 1 : qubits - target_qubit_begin;
+        QCircuit wider(width);
+        wider << ControlGate(current, c, 0);
         current = std::move(wider);
     }
-    for (auto it = above.rbegin(); it != above.rend(); ++it) {
-        const uint32_t c = *it;
-        QCircuit wider(lo + current.qubit_count() - c);
-        wider << ControlGate(current, 0, lo - c);
-        current = std::move(wider);
-        lo = c;
+    if (!above.empty()) {
+        uint32_t nearer = target_qubit_begin;
+        for (size_t i = above.size() - 1; i > 0; --i) {
+            const uint32_t c = above[i];
+            QCircuit wider(qubits - c);
+            wider << ControlGate{current, 0, nearer - c};
+            nearer = c;
+            current = std::move(wider);
+        }
+        QCircuit whole(qubits);
+        whole << ControlGate(current, above.front(), qubits - current.qubit_count());
+        current = std::move(whole);
     }
-    QCircuit qc(qubits);
-    qc << Gate(current, lo);
-    if (compile) qc.compile();
-    return qc;
+    if (compile) current.compile();
+    return current;
 }
 
 QCircuit Rewire_Gate(uint32_t qubits, const std::vector<uint32_t>& new_qubit_positions, const QCircuit& gate,
@@ -133,67 +168,68 @@ QCircuit Rewire_Gate(uint32_t qubits, const std::vector<uint32_t>& new_qubit_pos
 namespace {
 
 template <typename G>
-std::shared_ptr<QGate> copy_as(const QGate& g) {
-    return std::make_shared<G>(static_cast<const G&>(g));
+void negate_angle(QGate& g) {
+    auto& t = static_cast<G&>(g);
+    t.angle = -t.angle;
 }
 
-// A copy of the inverse of one gate.
-std::shared_ptr<QGate> adjoint_of(const QGate& g) {
+// Turns one gate into its inverse in place (src/quantum_gates.cpp:244-313).
+// Self-inverse gates stay; rotation and phase angles are negated; a
+// sub-circuit gate gets the adjoint of its circuit.  CH and any gate outside
+// this list throw, as in the reference.  U3 (no reference counterpart)
+// becomes U3(-theta, -lambda, -phi).
+void invert_in_place(QGate& g) {
     const uint32_t t = g.type();
-    if (t == Barrier::static_type()) return copy_as<Barrier>(g);
-    if (t == X::static_type()) return copy_as<X>(g);
-    if (t == Y::static_type()) return copy_as<Y>(g);
-    if (t == Z::static_type()) return copy_as<Z>(g);
-    if (t == H::static_type()) return copy_as<H>(g);
-    if (t == Swap::static_type()) return copy_as<Swap>(g);
-    if (t == CSwap::static_type()) return copy_as<CSwap>(g);
-    if (t == CX::static_type()) return copy_as<CX>(g);
-    if (t == CY::static_type()) return copy_as<CY>(g);
-    if (t == CZ::static_type()) return copy_as<CZ>(g);
-    if (t == CH::static_type()) return copy_as<CH>(g);
-    if (t == CCNot::static_type()) return copy_as<CCNot>(g);
-    if (t == Or::static_type()) return copy_as<Or>(g);
-    auto negate = [](auto p) {
-        p->angle = -p->angle;
-        return p;
-    };
-    if (t == RotX::static_type()) return negate(std::make_shared<RotX>(static_cast<const RotX&>(g)));
-    if (t == RotY::static_type()) return negate(std::make_shared<RotY>(static_cast<const RotY&>(g)));
-    if (t == RotZ::static_type()) return negate(std::make_shared<RotZ>(static_cast<const RotZ&>(g)));
-    if (t == Phase::static_type()) return negate(std::make_shared<Phase>(static_cast<const Phase&>(g)));
-    if (t == CRotX::static_type()) return negate(std::make_shared<CRotX>(static_cast<const CRotX&>(g)));
-    if (t == CRotY::static_type()) return negate(std::make_shared<CRotY>(static_cast<const CRotY&>(g)));
-    if (t == CRotZ::static_type()) return negate(std::make_shared<CRotZ>(static_cast<const CRotZ&>(g)));
-    if (t == CPhase::static_type()) return negate(std::make_shared<CPhase>(static_cast<const CPhase&>(g)));
+    if (t == Barrier::static_type() || t == X::static_type() || t == Y::static_type() || t == Z::static_type() ||
+        t == H::static_type() || t == Swap::static_type() || t == CSwap::static_type() || t == CX::static_type() ||
+        t == CY::static_type() || t == CZ::static_type() || t == CCNot::static_type() || t == Or::static_type())
+        return;
+    if (t == RotX::static_type()) return negate_angle<RotX>(g);
+    if (t == RotY::static_type()) return negate_angle<RotY>(g);
+    if (t == RotZ::static_type()) return negate_angle<RotZ>(g);
+    if (t == Phase::static_type()) return negate_angle<Phase>(g);
+    if (t == CRotX::static_type()) return negate_angle<CRotX>(g);
+    if (t == CRotY::static_type()) return negate_angle<CRotY>(g);
+    if (t == CRotZ::static_type()) return negate_angle<CRotZ>(g);
+    if (t == CPhase::static_type()) return negate_angle<CPhase>(g);
     if (t == U3::static_type()) {
-        auto p = std::make_shared<U3>(static_cast<const U3&>(g));
-        const float phi = p->phi;
-        p->theta = -p->theta;
-        p->phi = -p->lambda;
-        p->lambda = -phi;
-        return p;
+        auto& u = static_cast<U3&>(g);
+        const float phi = u.phi;
+        u.theta = -u.theta;
+        u.phi = -u.lambda;
+        u.lambda = -phi;
+        return;
     }
     if (t == Gate::static_type()) {
-        auto p = std::make_shared<Gate>(static_cast<const Gate&>(g));
-        p->internal_circuit = std::make_shared<QCircuit>(Adjoint_Gate(*p->internal_circuit));
-        return p;
+        auto& sub = static_cast<Gate&>(g);
+        sub.internal_circuit = std::make_shared<QCircuit>(Adjoint_Gate(*sub.internal_circuit));
+        return;
     }
     if (t == ControlGate::static_type()) {
-        auto p = std::make_shared<ControlGate>(static_cast<const ControlGate&>(g));
-        p->internal_circuit = std::make_shared<QCircuit>(Adjoint_Gate(*p->internal_circuit));
-        return p;
+        auto& sub = static_cast<ControlGate&>(g);
+        sub.internal_circuit = std::make_shared<QCircuit>(Adjoint_Gate(*sub.internal_circuit));
+        return;
     }
     throw std::runtime_error{"Unknown unsupported gate cannot be adjoint"};
 }
 
 }  // namespace
 
+// A copy of the circuit shares its gate objects (QCircuit's copy keeps the
+// shared pointers), and the reference inverts those objects in place
+// (src/quantum_gates.cpp:233-313): after the call the input circuit's
+// rotation angles are negated too, and a CH part way through leaves the gates
+// after it (in program order) already inverted when it throws.  The returned
+// circuit is compiled: it runs the conjugate transpose of the input's
+// operator (the reference transposes its compiled matrix, :335; compiling
+// the inverted gates in reverse order gives the same operator).
 QCircuit Adjoint_Gate(const QCircuit& gate) {
-    QCircuit qc{gate.qubit_count()};
-    for (auto it = gate.gate_list().rbegin(); it != gate.gate_list().rend(); ++it)
-        qc.gate_list().push_back(adjoint_of(**it));
+    QCircuit qc = gate;
+    auto& list = qc.gate_list();
+    std::reverse(list.begin(), list.end());
+    for (auto& g : list) invert_in_place(*g);
     // representation entries in reverse order (src/quantum_gates.cpp:322-331)
-    const std::string& rep = gate.representation();
+    const std::string& rep = qc.representation();
     std::string reversed;
     size_t prev = 0, mark = rep.find(';', prev);
     while (mark != std::string::npos) {
@@ -202,6 +238,7 @@ QCircuit Adjoint_Gate(const QCircuit& gate) {
         mark = rep.find(';', prev);
     }
     qc.representation() = reversed;
+    qc.clear_cache();
     qc.compile();
     return qc;
 }

@@ -279,6 +279,51 @@ unsigned red_grid(uint64_t N) {
     return (unsigned)std::min<uint64_t>(std::max<uint64_t>(b, 1), kRedGrid);
 }
 
+// Tables of the sampling pass over nb blocks, carved from one scratch area:
+// per-block sums, inclusive prefixes, last non-zero indices, then the total
+// and the global last non-zero index.
+struct SampleTables {
+    uint64_t* sums;
+    uint64_t* incl;
+    int64_t* lastnz;
+    uint64_t* misc;
+};
+
+constexpr size_t sample_table_bytes(uint64_t nb) { return (size_t)(3 * nb + 2) * sizeof(uint64_t); }
+
+SampleTables sample_tables(void* base, uint64_t nb) {
+    SampleTables t;
+    t.sums = (uint64_t*)base;
+    t.incl = t.sums + nb;
+    t.lastnz = (int64_t*)(t.incl + nb);
+    t.misc = (uint64_t*)(t.lastnz + nb);
+    return t;
+}
+
+// Block sums, the exact scan and one wavefront per shot, on `stream`.
+void launch_sampling(const void* amps, int precision, uint64_t N, const SampleTables& t, const double* du,
+                     uint32_t shots, uint64_t* dout, hipStream_t stream) {
+    const uint64_t nb = (N + kSampleBlock - 1) / kSampleBlock;
+    if (precision == AQS_C64)
+        hipLaunchKernelGGL(k_block_sums<float2>, dim3((unsigned)nb), dim3(kBlock), 0, stream, (const float2*)amps, N,
+                           t.sums, t.lastnz);
+    else
+        hipLaunchKernelGGL(k_block_sums<double2>, dim3((unsigned)nb), dim3(kBlock), 0, stream, (const double2*)amps,
+                           N, t.sums, t.lastnz);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, stream, (const uint64_t*)t.sums,
+                       (const int64_t*)t.lastnz, nb, t.incl, t.misc);
+    const unsigned g = (shots + (kBlock / 64) - 1) / (kBlock / 64);
+    if (precision == AQS_C64)
+        hipLaunchKernelGGL(k_sample<float2>, dim3(g), dim3(kBlock), 0, stream, (const float2*)amps, N,
+                           (const uint64_t*)t.sums, (const uint64_t*)t.incl, (const uint64_t*)t.misc, nb, du, shots,
+                           dout);
+    else
+        hipLaunchKernelGGL(k_sample<double2>, dim3(g), dim3(kBlock), 0, stream, (const double2*)amps, N,
+                           (const uint64_t*)t.sums, (const uint64_t*)t.incl, (const uint64_t*)t.misc, nb, du, shots,
+                           dout);
+    AQS_HIP(hipGetLastError());
+}
+
 double* partial_buf(aqs_state s) { return (double*)s->scratch; }
 double* result_buf(aqs_state s) { return (double*)s->scratch + 2 * kRedGrid; }
 
@@ -388,22 +433,24 @@ aqs_status aqs_probabilities(aqs_state s, float* out) {
     return guard([&] {
         check(s);
         if (!out) fail(AQS_ERR_INVALID_ARGUMENT, "null output");
-        float* d = nullptr;
-        AQS_HIP(hipMalloc(&d, s->N * sizeof(float)));
-        try {
-            const unsigned g = red_grid(s->N) * 4;
+        // chunks of at most 2^24 floats through the state's staging buffer:
+        // no allocation of N floats per call, and no hipFree stalling the device
+        const uint64_t chunk = std::min<uint64_t>(s->N, 1ull << 24);
+        ensure_staging(s, chunk * sizeof(float));
+        float* d = (float*)s->staging;
+        for (uint64_t off = 0; off < s->N; off += chunk) {
+            const uint64_t len = std::min<uint64_t>(chunk, s->N - off);
+            const unsigned g = red_grid(len) * 4;
             if (s->precision == AQS_C64)
-                hipLaunchKernelGGL(k_probs<float2>, dim3(g), dim3(kBlock), 0, s->stream, (const float2*)s->amps, s->N, d);
+                hipLaunchKernelGGL(k_probs<float2>, dim3(g), dim3(kBlock), 0, s->stream, (const float2*)s->amps + off,
+                                   len, d);
             else
-                hipLaunchKernelGGL(k_probs<double2>, dim3(g), dim3(kBlock), 0, s->stream, (const double2*)s->amps, s->N, d);
+                hipLaunchKernelGGL(k_probs<double2>, dim3(g), dim3(kBlock), 0, s->stream, (const double2*)s->amps + off,
+                                   len, d);
             AQS_HIP(hipGetLastError());
-            AQS_HIP(hipMemcpyAsync(out, d, s->N * sizeof(float), hipMemcpyDeviceToHost, s->stream));
-            AQS_HIP(hipStreamSynchronize(s->stream));
-        } catch (...) {
-            (void)hipFree(d);
-            throw;
+            AQS_HIP(hipMemcpyAsync(out + off, d, len * sizeof(float), hipMemcpyDeviceToHost, s->stream));
         }
-        AQS_HIP(hipFree(d));
+        AQS_HIP(hipStreamSynchronize(s->stream));
     });
 }
 
@@ -433,12 +480,7 @@ aqs_status aqs_sample(aqs_state s, const double* u, uint32_t shots, uint64_t* ou
         if (shots == 0) return;
         if (!u || !out_index) fail(AQS_ERR_INVALID_ARGUMENT, "null uniforms or output");
         const uint64_t nb = (s->N + kSampleBlock - 1) / kSampleBlock;
-        uint64_t* sums = (uint64_t*)s->scratch;
-        uint64_t* incl = sums + nb;
-        int64_t* lastnz = (int64_t*)(incl + nb);
-        uint64_t* misc = (uint64_t*)(lastnz + nb);
-        if ((size_t)((char*)(misc + 2) - (char*)s->scratch) > s->scratch_bytes)
-            fail(AQS_ERR_RUNTIME, "sampling scratch too small");
+        if (sample_table_bytes(nb) > s->scratch_bytes) fail(AQS_ERR_RUNTIME, "sampling scratch too small");
         // uniforms in, indices out through the staging buffer
         const size_t ubytes = sizeof(double) * shots, obytes = sizeof(uint64_t) * shots;
         ensure_staging(s, ubytes + obytes);
@@ -447,23 +489,7 @@ aqs_status aqs_sample(aqs_state s, const double* u, uint32_t shots, uint64_t* ou
         uint64_t* dout = (uint64_t*)((char*)s->staging + ubytes);
         {
             ProfScope ps(s, 0);
-            if (s->precision == AQS_C64) {
-                hipLaunchKernelGGL(k_block_sums<float2>, dim3((unsigned)nb), dim3(kBlock), 0, s->stream,
-                                   (const float2*)s->amps, s->N, sums, lastnz);
-            } else {
-                hipLaunchKernelGGL(k_block_sums<double2>, dim3((unsigned)nb), dim3(kBlock), 0, s->stream,
-                                   (const double2*)s->amps, s->N, sums, lastnz);
-            }
-            hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s->stream, (const uint64_t*)sums,
-                               (const int64_t*)lastnz, nb, incl, misc);
-            const unsigned g = (shots + (kBlock / 64) - 1) / (kBlock / 64);
-            if (s->precision == AQS_C64)
-                hipLaunchKernelGGL(k_sample<float2>, dim3(g), dim3(kBlock), 0, s->stream, (const float2*)s->amps, s->N,
-                                   (const uint64_t*)sums, (const uint64_t*)incl, (const uint64_t*)misc, nb, du, shots, dout);
-            else
-                hipLaunchKernelGGL(k_sample<double2>, dim3(g), dim3(kBlock), 0, s->stream, (const double2*)s->amps, s->N,
-                                   (const uint64_t*)sums, (const uint64_t*)incl, (const uint64_t*)misc, nb, du, shots, dout);
-            AQS_HIP(hipGetLastError());
+            launch_sampling(s->amps, s->precision, s->N, sample_tables(s->scratch, nb), du, shots, dout, s->stream);
         }
         AQS_HIP(hipMemcpyAsync(out_index, dout, obytes, hipMemcpyDeviceToHost, s->stream));
         AQS_HIP(hipStreamSynchronize(s->stream));

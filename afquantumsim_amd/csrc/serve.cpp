@@ -1,239 +1,127 @@
-// Continuous-batching request server (host side).
+// Request server: the C ABI and the HIP backend of the scheduling core
+// (serve_core.h).
 //
-// The reference has no serving layer: a caller builds a QSimulator per
-// circuit and runs it to completion (src/quantum.cpp:277-291, 467-501).
-// Here a server takes circuits of mixed size as they arrive and keeps the
-// device busy with several of them at once:
-//
-//   - each request is a gate list, a qubit count, caller-supplied uniforms
-//     and a shot count; the result is the sampled basis indices;
-//   - W worker threads each own a HIP stream (default priority); a worker
-//     runs one request at a time: compile() fusion on the host, then
-//     |0..0>, the fused passes and sampling on its stream;
-//   - admission is by HBM footprint: a request starts only when a worker is
-//     free and the bytes of the resident states plus its own fit the
-//     budget; idle states stay resident in a per-size pool and are reused,
-//     and freed only when an admission needs their memory;
-//   - requests are admitted in arrival order; a later request may start
-//     ahead of an earlier one that does not fit yet, unless the earlier one
-//     has waited longer than the starvation limit;
-//   - a request that can never fit the budget is rejected at submit.
-// Gate kernels are deterministic, so a request's result does not depend on
-// what else runs beside it.
+// A request is a gate list, a qubit count, caller-supplied uniforms and a
+// shot count; its result is the sampled basis indices.  Each worker thread
+// owns a HIP stream (default priority) and runs one request at a time:
+// compile() fusion on the host (aqs_fuse_auto, or aqs_fuse of a fixed
+// width), then |0..0>, the fused passes and sampling on its stream.  Gate
+// kernels are deterministic, so a request's result does not depend on what
+// else runs beside it.
 #include <hip/hip_runtime.h>
 
-#include <chrono>
-#include <condition_variable>
-#include <deque>
-#include <map>
+#include <algorithm>
 #include <memory>
-#include <thread>
-#include <unordered_map>
-#include <vector>
 
 #include "internal.h"
+#include "serve_core.h"
 
 using namespace aqs_hip;
+using aqs_serve::Request;
+using aqs_serve::ServeError;
 
 namespace {
 
-using Clock = std::chrono::steady_clock;
-
-double now_s() { return std::chrono::duration<double>(Clock::now().time_since_epoch()).count(); }
-
-struct Request {
-    uint64_t id = 0;
-    int n = 0;
-    std::vector<aqs_op> ops;
-    std::vector<std::vector<double>> mats;
-    std::vector<double> u;
-    uint32_t shots = 0;
-    int keep_state = 0;
-    size_t footprint = 0;
-    // results
-    int status = AQS_OK;
-    std::string error;
-    std::vector<uint64_t> indices;
-    std::vector<float> state;  // complex64 interleaved when keep_state
-    double t_submit = 0, t_admit = 0, t_done = 0;
-    double norm2 = 0;
-    size_t passes = 0;
-    bool done = false;
-};
-
+// Device bytes of a resident state: amplitudes, reduction / sampling scratch,
+// staging.
 size_t state_footprint(int n) {
     const uint64_t N = 1ull << n;
     const size_t blocks = (size_t)((N + 1023) / 1024);
     return (size_t)(N * 8) + std::max<size_t>(65536, (blocks + 1024) * 32) + 65536;
 }
 
+class HipBackend final : public aqs_serve::Backend {
+   public:
+    HipBackend(int device, int fuse_qubits, int workers) : device_(device), fuse_qubits_(fuse_qubits) {
+        AQS_HIP(hipSetDevice(device_));
+        for (int w = 0; w < workers; ++w) {
+            hipStream_t st;
+            hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+            if (e != hipSuccess) {
+                release_streams();
+                hip_check(e, "hipStreamCreateWithFlags");
+            }
+            streams_.push_back(st);
+        }
+    }
+    ~HipBackend() override { release_streams(); }
+
+    void worker_start(int) override { (void)hipSetDevice(device_); }
+    size_t footprint(int n) const override { return state_footprint(n); }
+
+    void* create_state(int n) override {
+        aqs_state st = nullptr;
+        const aqs_status rc = aqs_state_create(n, AQS_C64, &st);
+        if (rc != AQS_OK) throw ServeError{rc, aqs_last_error()};
+        return st;
+    }
+    void destroy_state(void* st) override { aqs_state_destroy((aqs_state)st); }
+
+    void run(Request& r, void* state, int worker) override {
+        aqs_state st = (aqs_state)state;
+        auto ck = [](aqs_status rc) {
+            if (rc != AQS_OK) throw ServeError{rc, aqs_last_error()};
+        };
+        auto stop_if_cancelled = [&] {
+            if (r.cancel.load()) throw ServeError{AQS_ERR_RUNTIME, "request cancelled"};
+        };
+        aqs_schedule sch = nullptr;
+        try {
+            if (fuse_qubits_ == 0)
+                ck(aqs_fuse_auto(r.ops.data(), r.ops.size(), r.n, &sch));
+            else
+                ck(aqs_fuse(r.ops.data(), r.ops.size(), r.n, fuse_qubits_, &sch));
+            ck(aqs_schedule_info(sch, &r.passes, nullptr));
+            stop_if_cancelled();
+            ck(aqs_state_set_stream(st, streams_[(size_t)worker]));
+            ck(aqs_state_set_basis(st, 0));
+            ck(aqs_apply_schedule(st, sch));
+            stop_if_cancelled();
+            r.indices.resize(r.shots);
+            if (r.shots) ck(aqs_sample(st, r.u.data(), r.shots, r.indices.data()));
+            ck(aqs_norm2(st, &r.norm2));
+            if (r.keep_state) {
+                r.state.resize(2ull << r.n);
+                ck(aqs_state_download(st, r.state.data()));
+            }
+        } catch (...) {
+            if (sch) aqs_schedule_destroy(sch);
+            throw;
+        }
+        aqs_schedule_destroy(sch);
+    }
+
+   private:
+    int device_;
+    int fuse_qubits_;
+    std::vector<hipStream_t> streams_;
+
+    void release_streams() {
+        for (auto s : streams_) (void)hipStreamDestroy(s);
+        streams_.clear();
+    }
+};
+
+// ServeError -> the library's Error, so guard() reports it.
+template <typename F>
+auto serve_call(F&& f) -> decltype(f()) {
+    try {
+        return f();
+    } catch (const ServeError& e) {
+        fail(e.code, e.msg);
+    }
+}
+
 }  // namespace
 
 struct aqs_server_s {
-    size_t budget = 0;
-    int fuse_qubits = 0;  // 0: aqs_fuse_auto
-    double starvation_s = 0.25;
-    int device = 0;
-
-    std::mutex mu;
-    std::condition_variable cv_work;   // workers wait for assignments
-    std::condition_variable cv_done;   // waiters for results
-    bool stopping = false;
-
-    std::deque<std::shared_ptr<Request>> pending;
-    std::unordered_map<uint64_t, std::shared_ptr<Request>> all;
-    std::vector<std::shared_ptr<Request>> slot;  // per worker: assigned request
-    std::vector<std::thread> workers;
-    std::vector<hipStream_t> streams;
-
-    // resident states: idle pool per size, bytes of everything resident
-    std::map<int, std::vector<aqs_state>> idle;
-    size_t resident = 0;
-    size_t in_use = 0;
-
-    // statistics
-    size_t peak_resident = 0;
-    int max_concurrent = 0;
-    int running = 0;
-    uint64_t admitted = 0, rejected = 0, completed = 0, evictions = 0;
+    std::unique_ptr<aqs_serve::Server> core;
 };
 
 namespace {
 
-// Called with s->mu held: assign pending requests to idle workers.
-void dispatch_locked(aqs_server s) {
-    const double t = now_s();
-    for (auto it = s->pending.begin(); it != s->pending.end();) {
-        int w = -1;
-        for (size_t i = 0; i < s->slot.size(); ++i)
-            if (!s->slot[i]) {
-                w = (int)i;
-                break;
-            }
-        if (w < 0) return;
-        auto& r = *it;
-        // memory: reuse an idle state of this size, else make room
-        bool reuse = !s->idle[r->n].empty();
-        size_t need = reuse ? 0 : r->footprint;
-        size_t idle_bytes = s->resident - s->in_use;
-        bool fits = s->in_use + r->footprint <= s->budget && (s->resident + need <= s->budget || idle_bytes >= need);
-        if (!fits) {
-            // head-of-line: an old request blocks later ones (no starvation)
-            if (t - r->t_submit > s->starvation_s) return;
-            ++it;
-            continue;
-        }
-        s->slot[w] = r;
-        s->in_use += r->footprint;
-        r->t_admit = t;
-        ++s->admitted;
-        ++s->running;
-        s->max_concurrent = std::max(s->max_concurrent, s->running);
-        it = s->pending.erase(it);
-        s->cv_work.notify_all();
-    }
-}
-
-// Take a state of n qubits for a worker: idle pool first, else evict idle
-// states until the new one fits, then allocate.
-aqs_state acquire_state(aqs_server s, int n) {
-    {
-        std::lock_guard<std::mutex> lock(s->mu);
-        auto& v = s->idle[n];
-        if (!v.empty()) {
-            aqs_state st = v.back();
-            v.pop_back();
-            return st;
-        }
-        const size_t need = state_footprint(n);
-        while (s->resident + need > s->budget) {
-            aqs_state victim = nullptr;
-            int vn = -1;
-            for (auto& kv : s->idle)
-                if (!kv.second.empty()) {  // evict the largest idle states first
-                    vn = kv.first;
-                }
-            if (vn < 0) break;
-            victim = s->idle[vn].back();
-            s->idle[vn].pop_back();
-            s->resident -= state_footprint(vn);
-            ++s->evictions;
-            aqs_state_destroy(victim);
-        }
-        s->resident += need;
-        s->peak_resident = std::max(s->peak_resident, s->resident);
-    }
-    aqs_state st = nullptr;
-    aqs_status rc = aqs_state_create(n, AQS_C64, &st);
-    if (rc != AQS_OK) {
-        std::lock_guard<std::mutex> lock(s->mu);
-        s->resident -= state_footprint(n);
-        fail(rc, aqs_last_error());
-    }
-    return st;
-}
-
-void run_request(aqs_server s, Request& r, hipStream_t stream) {
-    aqs_schedule sch = nullptr;
-    aqs_state st = nullptr;
-    auto ck = [&](aqs_status rc) {
-        if (rc != AQS_OK) fail(rc, aqs_last_error());
-    };
-    try {
-        if (s->fuse_qubits == 0)
-            ck(aqs_fuse_auto(r.ops.data(), r.ops.size(), r.n, &sch));
-        else
-            ck(aqs_fuse(r.ops.data(), r.ops.size(), r.n, s->fuse_qubits, &sch));
-        ck(aqs_schedule_info(sch, &r.passes, nullptr));
-        st = acquire_state(s, r.n);
-        ck(aqs_state_set_stream(st, stream));
-        ck(aqs_state_set_basis(st, 0));
-        ck(aqs_apply_schedule(st, sch));
-        r.indices.resize(r.shots);
-        if (r.shots) ck(aqs_sample(st, r.u.data(), r.shots, r.indices.data()));
-        ck(aqs_norm2(st, &r.norm2));
-        if (r.keep_state) {
-            r.state.resize(2ull << r.n);
-            ck(aqs_state_download(st, r.state.data()));
-        }
-    } catch (const Error& e) {
-        r.status = e.code;
-        r.error = e.msg;
-    }
-    if (sch) aqs_schedule_destroy(sch);
-    if (st) {
-        std::lock_guard<std::mutex> lock(s->mu);
-        s->idle[r.n].push_back(st);
-    }
-}
-
-void worker_main(aqs_server s, int w) {
-    (void)hipSetDevice(s->device);
-    for (;;) {
-        std::shared_ptr<Request> r;
-        {
-            std::unique_lock<std::mutex> lock(s->mu);
-            s->cv_work.wait(lock, [&] { return s->stopping || s->slot[w]; });
-            if (!s->slot[w]) return;  // stopping and nothing assigned
-            r = s->slot[w];
-        }
-        run_request(s, *r, s->streams[w]);
-        {
-            std::lock_guard<std::mutex> lock(s->mu);
-            r->t_done = now_s();
-            r->done = true;
-            s->slot[w] = nullptr;
-            s->in_use -= r->footprint;
-            --s->running;
-            ++s->completed;
-            dispatch_locked(s);
-        }
-        s->cv_done.notify_all();
-    }
-}
-
 aqs_server check_server(aqs_server s) {
-    if (!s) fail(AQS_ERR_INVALID_ARGUMENT, "null server handle");
+    if (!s || !s->core) fail(AQS_ERR_INVALID_ARGUMENT, "null server handle");
     return s;
 }
 
@@ -247,39 +135,15 @@ aqs_status aqs_server_create(size_t hbm_budget_bytes, int workers, int fuse_qubi
         *out = nullptr;
         if (workers < 1 || workers > 16) fail(AQS_ERR_INVALID_ARGUMENT, "workers must be 1..16");
         if (fuse_qubits < 0 || fuse_qubits > 5) fail(AQS_ERR_INVALID_ARGUMENT, "fuse_qubits must be 0..5");
-        auto* s = new aqs_server_s();
-        s->budget = hbm_budget_bytes;
-        s->fuse_qubits = fuse_qubits;
-        s->device = current_device();
-        s->slot.assign(workers, nullptr);
-        try {
-            for (int w = 0; w < workers; ++w) {
-                hipStream_t st;
-                AQS_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-                s->streams.push_back(st);
-            }
-            for (int w = 0; w < workers; ++w) s->workers.emplace_back(worker_main, s, w);
-        } catch (...) {
-            {
-                std::lock_guard<std::mutex> lock(s->mu);
-                s->stopping = true;
-            }
-            s->cv_work.notify_all();
-            for (auto& t : s->workers) t.join();
-            for (auto st : s->streams) (void)hipStreamDestroy(st);
-            delete s;
-            throw;
-        }
-        *out = s;
+        auto backend = std::make_unique<HipBackend>(current_device(), fuse_qubits, workers);
+        auto s = std::make_unique<aqs_server_s>();
+        serve_call([&] { s->core = std::make_unique<aqs_serve::Server>(std::move(backend), hbm_budget_bytes, workers); });
+        *out = s.release();
     });
 }
 
 aqs_status aqs_server_set_starvation_limit(aqs_server s, double seconds) {
-    return guard([&] {
-        check_server(s);
-        std::lock_guard<std::mutex> lock(s->mu);
-        s->starvation_s = seconds;
-    });
+    return guard([&] { check_server(s)->core->set_starvation_limit(seconds); });
 }
 
 aqs_status aqs_server_submit(aqs_server s, uint64_t id, int nqubits, const aqs_op* ops, size_t nops,
@@ -294,32 +158,20 @@ aqs_status aqs_server_submit(aqs_server s, uint64_t id, int nqubits, const aqs_o
         r->n = nqubits;
         r->shots = shots;
         r->keep_state = keep_state;
-        r->footprint = state_footprint(nqubits);
-        if (r->footprint > s->budget)
-            fail(AQS_ERR_INVALID_ARGUMENT, "request needs more device memory than the server budget");
         r->ops.assign(ops, ops + nops);
         for (auto& op : r->ops) {
             validate_op(op, nqubits);
             if (op.kind == AQS_OP_MATK) {
                 const size_t K = (size_t)1 << op.ntargets;
                 r->mats.emplace_back(op.u, op.u + 2 * K * K);
-                op.u = r->mats.back().data();
             }
         }
-        // matrices were moved into r->mats; re-point after the vector settled
+        // point MATK ops at the request's own copies (after mats settled)
         size_t mi = 0;
         for (auto& op : r->ops)
             if (op.kind == AQS_OP_MATK) op.u = r->mats[mi++].data();
         r->u.assign(uniforms, uniforms + shots);
-        r->t_submit = now_s();
-        std::lock_guard<std::mutex> lock(s->mu);
-        if (s->all.count(id)) {
-            ++s->rejected;
-            fail(AQS_ERR_INVALID_ARGUMENT, "duplicate request id");
-        }
-        s->all[id] = r;
-        s->pending.push_back(r);
-        dispatch_locked(s);
+        serve_call([&] { s->core->submit(std::move(r)); });
     });
 }
 
@@ -327,19 +179,7 @@ aqs_status aqs_server_wait(aqs_server s, uint64_t id, int timeout_ms, aqs_reques
                            void* state_host) {
     return guard([&] {
         check_server(s);
-        std::shared_ptr<Request> r;
-        {
-            std::unique_lock<std::mutex> lock(s->mu);
-            auto it = s->all.find(id);
-            if (it == s->all.end()) fail(AQS_ERR_OUT_OF_RANGE, "unknown request id");
-            r = it->second;
-            auto pred = [&] { return r->done; };
-            if (timeout_ms < 0)
-                s->cv_done.wait(lock, pred);
-            else if (!s->cv_done.wait_for(lock, std::chrono::milliseconds(timeout_ms), pred))
-                fail(AQS_ERR_RUNTIME, "request not finished within the timeout");
-            s->all.erase(it);
-        }
+        std::shared_ptr<Request> r = serve_call([&] { return s->core->wait(id, timeout_ms); });
         if (res) {
             res->id = r->id;
             res->status = r->status;
@@ -358,39 +198,38 @@ aqs_status aqs_server_wait(aqs_server s, uint64_t id, int timeout_ms, aqs_reques
     });
 }
 
+aqs_status aqs_server_cancel(aqs_server s, uint64_t id, int* cancelled) {
+    return guard([&] {
+        check_server(s);
+        const bool c = serve_call([&] { return s->core->cancel(id); });
+        if (cancelled) *cancelled = c ? 1 : 0;
+    });
+}
+
 aqs_status aqs_server_stats(aqs_server s, aqs_server_stats_t* out) {
     return guard([&] {
         check_server(s);
-        std::lock_guard<std::mutex> lock(s->mu);
         if (!out) return;
-        out->admitted = s->admitted;
-        out->completed = s->completed;
-        out->rejected = s->rejected;
-        out->evictions = s->evictions;
-        out->pending = s->pending.size();
-        out->running = s->running;
-        out->max_concurrent = s->max_concurrent;
-        out->resident_bytes = s->resident;
-        out->peak_resident_bytes = s->peak_resident;
-        out->budget_bytes = s->budget;
+        const aqs_serve::Stats st = s->core->stats();
+        out->admitted = st.admitted;
+        out->completed = st.completed;
+        out->rejected = st.rejected;
+        out->evictions = st.evictions;
+        out->pending = st.pending;
+        out->running = st.running;
+        out->max_concurrent = st.max_concurrent;
+        out->resident_bytes = st.resident_bytes;
+        out->peak_resident_bytes = st.peak_resident_bytes;
+        out->budget_bytes = st.budget_bytes;
+        out->cancelled = st.cancelled;
     });
 }
 
 aqs_status aqs_server_destroy(aqs_server s) {
     return guard([&] {
         if (!s) return;
-        {
-            std::unique_lock<std::mutex> lock(s->mu);
-            // finish admitted and pending work first
-            s->cv_done.wait(lock, [&] { return s->pending.empty() && s->running == 0; });
-            s->stopping = true;
-        }
-        s->cv_work.notify_all();
-        for (auto& t : s->workers) t.join();
-        for (auto& kv : s->idle)
-            for (aqs_state st : kv.second) aqs_state_destroy(st);
-        for (auto st : s->streams) (void)hipStreamDestroy(st);
-        delete s;
+        std::unique_ptr<aqs_server_s> owned(s);
+        owned->core.reset();  // finishes queued work, frees every resident state
     });
 }
 

@@ -85,7 +85,7 @@ EXPORTS = [
     "aqs_inner", "aqs_profile_enable", "aqs_profile_read", "aqs_timer_start", "aqs_timer_stop",
     "aqs_batch_create", "aqs_batch_autotune", "aqs_batch_configure", "aqs_batch_run", "aqs_batch_state", "aqs_batch_info",
     "aqs_batch_destroy", "aqs_autotune", "aqs_tune_set", "aqs_server_create", "aqs_server_set_starvation_limit",
-    "aqs_server_submit", "aqs_server_wait", "aqs_server_stats", "aqs_server_destroy",
+    "aqs_server_submit", "aqs_server_wait", "aqs_server_stats", "aqs_server_destroy", "aqs_server_cancel",
 ]
 
 
@@ -100,7 +100,8 @@ class ServerStats(ctypes.Structure):
     _fields_ = [("admitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("rejected", ctypes.c_uint64),
                 ("evictions", ctypes.c_uint64), ("pending", ctypes.c_uint64), ("running", ctypes.c_int32),
                 ("max_concurrent", ctypes.c_int32), ("resident_bytes", ctypes.c_uint64),
-                ("peak_resident_bytes", ctypes.c_uint64), ("budget_bytes", ctypes.c_uint64)]
+                ("peak_resident_bytes", ctypes.c_uint64), ("budget_bytes", ctypes.c_uint64),
+                ("cancelled", ctypes.c_uint64)]
 
 _lib = None
 
@@ -170,6 +171,7 @@ def load(path: str = LIB_PATH):
         "aqs_server_submit": [vp, u64, i32, ctypes.POINTER(Op), sz, dp, ctypes.c_uint32, i32],
         "aqs_server_wait": [vp, u64, i32, ctypes.POINTER(RequestResult), ctypes.POINTER(ctypes.c_uint64), vp],
         "aqs_server_stats": [vp, ctypes.POINTER(ServerStats)],
+        "aqs_server_cancel": [vp, u64, ctypes.POINTER(i32)],
         "aqs_server_destroy": [vp],
     }
     for name, args in sigs.items():
@@ -502,13 +504,24 @@ class Server:
 
     def wait(self, rid: int, timeout_ms: int = -1):
         res = RequestResult()
-        idx = np.zeros(max(1, self._shots[rid]), dtype=np.uint64)
-        state = np.zeros(1 << self._n[rid], dtype=np.complex64) if self._keep[rid] else None
-        check(_lib.aqs_server_wait(self.h, rid, timeout_ms, ctypes.byref(res),
-                                   idx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
-                                   state.ctypes.data if state is not None else None))
+        shots = self._shots.get(rid, 0)
+        idx = np.zeros(max(1, shots), dtype=np.uint64)
+        state = np.zeros(1 << self._n[rid], dtype=np.complex64) if self._keep.get(rid) else None
+        rc = _lib.aqs_server_wait(self.h, rid, timeout_ms, ctypes.byref(res),
+                                  idx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                  state.ctypes.data if state is not None else None)
+        if not (rc == AQS_ERR_RUNTIME and b"within the timeout" in _lib.aqs_last_error()):
+            # the result was handed over (or never will be): forget the request
+            for d in (self._shots, self._n, self._keep):
+                d.pop(rid, None)
+        check(rc)
         out = {f: getattr(res, f) for f, _ in RequestResult._fields_}
-        return out, idx[: self._shots.pop(rid)], state
+        return out, idx[:shots], state
+
+    def cancel(self, rid: int) -> bool:
+        c = ctypes.c_int()
+        check(_lib.aqs_server_cancel(self.h, rid, ctypes.byref(c)))
+        return bool(c.value)
 
     def stats(self):
         s = ServerStats()

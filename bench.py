@@ -219,6 +219,32 @@ def reference_algorithms(args):
     return res
 
 
+def fft_closed_form_infidelity(inp, out, n, inverse):
+    """Infidelity of a QFT result against its closed form, evaluated with an
+    FP64 FFT on the GPU (torch is plumbing here: an independent FFT):
+    fourier_transform(n) + final swaps = bitrev(ifft(bitrev(v))) sqrt(N);
+    inverse_fourier_transform(n) = bitrev(fft(v)) / sqrt(N)."""
+    import torch
+    dev = torch.device("cuda")
+
+    def bitrev(t):
+        return t.reshape([2] * n).permute(*reversed(range(n))).reshape(-1)
+
+    v = torch.from_numpy(inp).to(dev).to(torch.complex128)
+    if inverse:
+        ref = bitrev(torch.fft.fft(v, norm="ortho"))
+    else:
+        ref = bitrev(torch.fft.ifft(bitrev(v), norm="ortho"))
+    del v
+    got = torch.from_numpy(out).to(dev).to(torch.complex128)
+    ov = torch.vdot(ref, got)
+    infid = 1.0 - (abs(ov) ** 2 / (torch.vdot(got, got).real * torch.vdot(ref, ref).real)).item()
+    del ref, got
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    return max(0.0, infid)
+
+
 def qft30(args, n=30, seed=30, inverse=False):
     """30-qubit QFT (fourier_transform + final swaps, or
     inverse_fourier_transform) on a seeded Gaussian complex64 state
@@ -263,13 +289,27 @@ def qft30(args, n=30, seed=30, inverse=False):
         prof = st.profile_read()
         st.profile(False)
         mf_ms, mf_n = prof.get("MATK_MFMA", (0.0, 0))
+        inf_vs_c128 = infid()
+        closed_form = None
+        if kmax == 5:
+            # the fused result against the FFT closed form
+            st.sync()
+            res = st.download()
+            st.gaussian(seed)
+            st.sync()
+            closed_form = fft_closed_form_infidelity(st.download(), res, n, inverse)
+            del res
+            log(f"{'iqft' if inverse else 'qft'}{n} fused k=5 vs FFT closed form: infidelity {closed_form:.3e}")
+            if not closed_form < 1e-9:
+                raise RuntimeError(f"QFT result departs from the FFT closed form: infidelity {closed_form}")
         out["tiled" if kmax == "tiled" else f"fused_k{kmax}"] = {"passes": passes, "ms": round(1e3 * dt, 2),
-                                 "gate_apps_per_s": round(len(lowered) / dt, 1), "infidelity": infid(),
+                                 "gate_apps_per_s": round(len(lowered) / dt, 1), "infidelity": inf_vs_c128,
                                  "fuse_host_s": round(fuse_s, 3), "mfma_launches": mf_n,
                                  "mfma_avg_ms": round(mf_ms / max(1, mf_n), 4),
                                  "mfma_tflops": round(8 * 32 * (1 << n) / (mf_ms / max(1, mf_n) * 1e-3) / 1e12, 1)
                                  if mf_n else None,
                                  "speedup_vs_unfused": round(out["unfused"]["ms"] / (1e3 * dt), 2),
+                                 "infidelity_vs_fft_closed_form": closed_form,
                                  "profile_ms": {k: [round(v[0], 3), v[1]] for k, v in prof.items()}}
         sch.close()
     st.close()
@@ -376,32 +416,51 @@ def load_traffic():
 
 
 def probe(args):
-    """Short fixed sequence for counter collection on the 28-qubit state:
-    the one-qubit launch-shape tuning, 3 plain copies of the 2 GiB state
-    (calibration), 3 U3 passes on target bits 27, 2 and 0 (the three
-    one-qubit kernels), 3 CZ passes, then a fused 5-qubit schedule."""
+    """Short fixed kernel sequence for counter collection (rocprofv3 --pmc):
+    every pass kind the benchmarks run, on a state of --probe-qubits qubits.
+
+      copy x3           k_copy16, the calibration of FETCH_SIZE / WRITE_SIZE
+      U3 x3             one-qubit passes on target bits n-1, 2 and 0 (tuned shapes)
+      CZ x3             high bits (k_phase_v4), bits 0,1 and 2,3 (k_phase_lines)
+      brickwork k=5     dense 5-qubit blocks (k_matk_mfma, k_matk)
+      brickwork tiled   LDS tile passes (k_tile)
+      QFT k=5           MFMA blocks carrying diagonal tables and the folded
+                        bit reversal (k_matk_mfma*, k_diagt_v4)
+      QFT tiled         tile passes and the restoring bit permutation (k_bitperm)
+      sample            1000 shots (k_block_sums, k_scan_blocks, k_sample)
+    """
     hip.init(0)
-    n = N_QUBITS
+    n = args.probe_qubits
     a = hip.State(n, hip.C64)
     b = hip.State(n, hip.C64)
     a.gaussian(1)
     a.autotune(reps=1)
-    gates = circuits.brickwork(n, 2, seed=SEED)
-    lowered = ops.lower_all(gates)
-    u3 = [o for o in lowered if o["kind"] == hip.OP_MAT1 and o["targets"][0] in (0, n - 3, n - 1)][:3]
-    cz = [o for o in lowered if o["kind"] == hip.OP_PHASE][:3]
-    sch = hip.Schedule(lowered, n, 5)
+    log(f"probe: state of {n} qubits, launch shapes tuned")
+    brick = ops.lower_all(circuits.brickwork(n, 2, seed=SEED))
+    u3 = [o for o in brick if o["kind"] == hip.OP_MAT1 and o["targets"][0] in (0, n - 3, n - 1)][:3]
+    cz = ops.lower_all([circuits.gate("CZ", 0, 1), circuits.gate("CZ", n - 2, n - 1),
+                        circuits.gate("CZ", n - 4, n - 3)])
+    qft = ops.lower_all(circuits.fourier_transform(n, final_swaps=True))
+    schedules = [("brickwork k=5", hip.Schedule(brick, n, 5)),
+                 ("brickwork tiled", hip.Schedule(brick, n, mode="tiled")),
+                 ("QFT k=5", hip.Schedule(qft, n, 5)),
+                 ("QFT tiled", hip.Schedule(qft, n, mode="tiled"))]
     for _ in range(3):
         b.copy_from(a)
     b.sync()
+    log(f"probe: copy x3 ({8 << n} B each)")
     a.apply(u3)
-    a.sync()
     a.apply(cz)
     a.sync()
-    for _ in range(1):
+    log("probe: U3 x3, CZ x3")
+    for label, sch in schedules:
         a.apply_schedule(sch)
-    a.sync()
-    log(f"probe done: copy x3 ({8 << n} B each), U3 x3, CZ x3, fused schedule ({sch.info()[0]} passes)")
+        a.sync()
+        log(f"probe: {label}: {[k for k, _ in sch.passes()]}")
+        sch.close()
+    a.normalize()
+    idx = a.sample(np.random.default_rng(5).uniform(size=1000))
+    log(f"probe: sampled {len(idx)} shots; done")
 
 
 def run(args):
@@ -551,6 +610,7 @@ def main():
                    help="repetitions of the reference's algorithm benchmarks (0 skips them)")
     p.add_argument("--probe", action="store_true",
                    help="run a short fixed kernel sequence for hardware-counter collection and exit")
+    p.add_argument("--probe-qubits", type=int, default=N_QUBITS, help="state size of the --probe sequence")
     args = p.parse_args()
     if args.probe:
         probe(args)

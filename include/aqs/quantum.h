@@ -175,7 +175,7 @@ class QCircuit {
     const std::string& representation() const noexcept { return representation_; }
 
     /* Compiles the not-yet-compiled gates into device passes: <= 5-qubit
-     * dense blocks, or from 22 qubits up LDS-resident tile passes when the
+     * dense blocks, or from 20 qubits up LDS-resident tile passes when the
      * library's cost model rates them cheaper (aqs_fuse_auto). */
     void compile();
     void clear();

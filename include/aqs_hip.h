@@ -277,6 +277,7 @@ typedef struct aqs_server_stats_t {
     uint64_t admitted, completed, rejected, evictions, pending;
     int32_t running, max_concurrent;
     uint64_t resident_bytes, peak_resident_bytes, budget_bytes;
+    uint64_t cancelled;  /* requests that ended through aqs_server_cancel */
 } aqs_server_stats_t;
 
 /* fuse_qubits: 1..5 = aqs_fuse blocks of that width, 0 = aqs_fuse_auto. */
@@ -290,9 +291,16 @@ aqs_status aqs_server_submit(aqs_server s, uint64_t id, int nqubits, const aqs_o
                              const double* uniforms, uint32_t shots, int keep_state);
 /* Wait for request id (timeout_ms < 0: no limit); indices receives `shots`
  * sampled basis indices, state_host the final complex64 state when the
- * request was submitted with keep_state. */
+ * request was submitted with keep_state.  A result is handed over once: a
+ * second wait on the same id (or one that loses the race to another waiter)
+ * returns AQS_ERR_OUT_OF_RANGE. */
 aqs_status aqs_server_wait(aqs_server s, uint64_t id, int timeout_ms, aqs_request_result* res, uint64_t* indices,
                            void* state_host);
+/* Cancel request id: a queued request is dropped, a running one stops at its
+ * next stage (after compile, after the gate passes); either then completes
+ * with AQS_ERR_RUNTIME "request cancelled", to be collected by
+ * aqs_server_wait.  *cancelled = 0 when it had already finished. */
+aqs_status aqs_server_cancel(aqs_server s, uint64_t id, int* cancelled);
 aqs_status aqs_server_stats(aqs_server s, aqs_server_stats_t* out);
 /* Finishes queued work, then frees every resident state. */
 aqs_status aqs_server_destroy(aqs_server s);

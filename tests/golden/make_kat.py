@@ -168,11 +168,119 @@ MEAS_CASES = [
 ]
 
 
+# Host QState known answers (test/tests.cpp:42-245).  "in" is the pair of
+# amplitudes handed to the constructor (re0, im0, re1, im1), before the
+# constructor's FP32 normalisation; "op" is what is applied; "expect" the
+# resulting pair; tol 0 = exact equality, else fequal's relative tolerance.
+QSTATE_CASES = [
+    {"name": "default", "lines": "42-58", "op": "default", "in": None, "expect": [1.0, 0.0, 0.0, 0.0], "tol": 0},
+    {"name": "ctor", "lines": "42-58", "op": "none", "in": [0.6, 0.0, 0.0, 0.8], "expect": [0.6, 0.0, 0.0, 0.8],
+     "tol": 0},
+    {"name": "copy", "lines": "42-73", "op": "copy", "in": [0.6, 0.0, 0.0, 0.8], "expect": [0.6, 0.0, 0.0, 0.8],
+     "tol": 0},
+    {"name": "normalize_34", "lines": "78-97", "op": "none", "in": [3.0, 0.0, 4.0, 0.0],
+     "expect": [0.6, 0.0, 0.8, 0.0], "tol": 0},
+    {"name": "X_op", "lines": "195-206", "op": "X", "in": [0.5, -0.5, 0.7, 0.1], "expect": [0.7, 0.1, 0.5, -0.5],
+     "tol": 0},
+    {"name": "Y_op", "lines": "195-212", "op": "Y", "in": [0.5, -0.5, 0.7, 0.1], "expect": [0.1, -0.7, 0.5, 0.5],
+     "tol": 0},
+    {"name": "Z_op", "lines": "195-217", "op": "Z", "in": [0.5, -0.5, 0.7, 0.1], "expect": [0.5, -0.5, -0.7, -0.1],
+     "tol": 0},
+    # H_op is compared with QState({1.2,-0.4},{-0.2,-0.6}) after its normalisation
+    {"name": "H_op", "lines": "219-226", "op": "H", "in": [0.5, -0.5, 0.7, 0.1],
+     "expect_normalized": [1.2, -0.4, -0.2, -0.6], "tol": 1e-4},
+]
+# Probabilities of a QState (test/tests.cpp:102-131): in -> (P(0), P(1)).
+QSTATE_PROB_CASES = [
+    {"in": [1.0, 0.0, 0.0, 0.0], "p0": 1.0, "p1": 0.0, "tol": 0},
+    {"in": [0.0, 1.0, 0.0, 0.0], "p0": 1.0, "p1": 0.0, "tol": 0},
+    {"in": [0.0, 0.0, 1.0, 0.0], "p0": 0.0, "p1": 1.0, "tol": 0},
+    {"in": [0.0, 0.0, 0.0, 1.0], "p0": 0.0, "p1": 1.0, "tol": 0},
+    {"in": [0.0, 1.0, 0.0, 1.0], "p0": 0.5, "p1": 0.5, "tol": 1e-4},
+    {"in": [0.6, 0.0, 0.0, 0.8], "p0": 0.36, "p1": 0.64, "tol": 1e-4},
+]
+
+# QSimulator construction (test/tests.cpp:251-267): every qubit set to `qubit`,
+# amplitude `index` of the generated state.
+QSIM_CASES = [
+    {"name": "all_same", "lines": "251-267", "n": 3, "qubit": [0.6, 0.0, 0.0, 0.8], "index": 0,
+     "expect": [0.216, 0.0], "tol": 1e-4},
+]
+
+# Collapse (test/tests.cpp:303-336).  measure_all leaves the basis state it
+# returned; measure(q) keeps the outcome's half, renormalised: "after_one" /
+# "after_zero" list amplitudes for either outcome.
+COLLAPSE_CASES = [
+    {"name": "measure_all", "lines": "303-313", "n": 3, "init": [[0.0, 1.0, -1.0, 0.0], Z0, [0.6, 0.0, 0.0, 0.8]],
+     "op": "measure_all", "support": [0b000, 0b001, 0b100, 0b101]},
+    {"name": "measure_q0", "lines": "315-336", "n": 3, "init": [[0.0, 1.0, -1.0, 0.0], Z0, [0.6, 0.0, 0.0, 0.8]],
+     "op": "measure", "qubit": 0,
+     "after_one": [[0b100, -0.6, 0.0], [0b101, 0.0, -0.8]],
+     "after_zero": [[0b000, 0.0, 0.6], [0b001, -0.8, 0.0]], "tol": 1e-4},
+]
+
+# Statistical profiles (test/tests.cpp:136-190, 343-406): counts over `reps`
+# shots must fall in the 99.9% band (z = 3.291) around reps * p.
+PROFILE_CASES = [
+    {"name": "qsim_one_qubit_set", "lines": "349-381", "n": 5, "reps": 10000,
+     "init": [Z0, Z0, Z0, O1, Z0]},
+    {"name": "qsim_mixed", "lines": "383-403", "n": 5, "reps": 10000,
+     "init": [[1.0, 0.0, 1.0, 0.0], [-2.0, 0.0, 3.0, 0.0], Z0, O1, [0.0, 0.6, 0.8, 0.0]]},
+]
+
+
+# Builders of quantum_gates.h (test/tests.cpp:1011-1135) as data.  A builder
+# node: {"builder": name, args...}; "Circuit"/"ControlCircuit" nodes wrap a
+# sub-circuit (gate list or builder) at a position.
+SPECIAL_PEEK = [
+    # NControl_Gate(6, {0,2,4,5}, 3, X) embedded at 0; init per qubit, expected peek_measure_all
+    {"name": "ncontrol_6", "lines": "1015-1047", "n": 6,
+     "circuit": [{"kind": "Circuit", "begin": 0,
+                  "builder": {"builder": "NControl_list", "qubits": 6, "controls": [0, 2, 4, 5], "target": 3,
+                              "gate": "X"}}],
+     "runs": [{"init": [O1] * 6, "peek_all": 0b111011},
+              {"init": [O1, O1, O1, Z0, O1, O1], "peek_all": 0b111111},
+              {"init": [O1, Z0, O1, Z0, O1, O1], "peek_all": 0b101111},
+              {"init": [O1, Z0, O1, O1, O1, O1], "peek_all": 0b101011}]},
+]
+SPECIAL_EQUALITIES = [
+    {"name": "ncontrol_one_below", "lines": "1049-1060", "n": 4,
+     "lhs": [{"kind": "Circuit", "begin": 0,
+              "builder": {"builder": "NControl_list", "qubits": 4, "controls": [3], "target": 0, "gate": "X"}}],
+     "rhs": [g("CX", 3, 0)], "tol": 0},
+    {"name": "ncontrol_above_below", "lines": "1062-1071", "n": 4,
+     "lhs": [{"kind": "Circuit", "begin": 0,
+              "builder": {"builder": "NControl_list", "qubits": 4, "controls": [0, 3], "target": 2, "gate": "X"}}],
+     "rhs": [g("CCX", 0, 3, 2)], "tol": 0},
+    {"name": "control_group", "lines": "1074-1084", "n": 4,
+     "lhs": [{"kind": "Circuit", "begin": 0,
+              "builder": {"builder": "Control_Group", "qubits": 4, "control": 1, "targets": [0, 2, 3], "gate": "X"}}],
+     "rhs": [g("CX", 1, 0), g("CX", 1, 2), g("CX", 1, 3)], "tol": 0},
+    {"name": "group", "lines": "1086-1094", "n": 4,
+     "lhs_builder": {"builder": "Group", "qubits": 4, "targets": [0, 2, 3], "gate": "X", "compile": True},
+     "rhs": [g("X", 0), g("X", 2), g("X", 3)], "tol": 0},
+    {"name": "rewire", "lines": "1096-1108", "n": 4,
+     "lhs_builder": {"builder": "Rewire", "qubits": 4, "positions": [2, 1, 3, 0], "compile": True,
+                     "sub_n": 4, "sub": [g("H", 0), g("CX", 0, 2), g("H", 1), g("Z", 3)]},
+     "rhs": [g("H", 2), g("CX", 2, 3), g("H", 1), g("Z", 0)], "tol": 0},
+    {"name": "adjoint", "lines": "1110-1131", "n": 4,
+     "lhs_builder": {"builder": "Adjoint", "sub_n": 4,
+                     "sub": [g("Swap", 1, 2), g("CPhase", 2, 1, angle=f32(-PI / 3)),
+                             g("CRotX", 1, 3, angle=f32(PI / 4)), g("X", 0),
+                             g("CRotY", 3, 0, angle=f32(-PI / 2))]},
+     "rhs": [g("CRotY", 3, 0, angle=f32(PI / 2)), g("X", 0), g("CRotX", 1, 3, angle=f32(-PI / 4)),
+             g("CPhase", 2, 1, angle=f32(PI / 3)), g("Swap", 1, 2)], "tol": 1e-5},
+]
+
+
 def main():
     out = {"source": "test/tests.cpp of arrayfire/afQuantumSim (transcribed)",
            "qubit_order": "qubit 0 is the most significant bit",
            "gate_cases": CASES, "equalities": EQUALITIES, "ch_cases": CH_CASES,
-           "prob_cases": PROB_CASES, "measure_cases": MEAS_CASES}
+           "prob_cases": PROB_CASES, "measure_cases": MEAS_CASES,
+           "qstate_cases": QSTATE_CASES, "qstate_prob_cases": QSTATE_PROB_CASES, "qsim_cases": QSIM_CASES,
+           "collapse_cases": COLLAPSE_CASES, "profile_cases": PROFILE_CASES,
+           "special_peek": SPECIAL_PEEK, "special_equalities": SPECIAL_EQUALITIES}
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "kat_gates.json")
     with open(path, "w") as f:
         json.dump(out, f, indent=1)

@@ -583,26 +583,72 @@ def test_33_qubit_launch_sizes():
     st.close()
 
 
-@pytest.mark.parametrize("n", [26])
-def test_large_state_round_trip(n):
-    """Size-independent properties at a large state: a brickwork layer
-    followed by its inverse restores the input; permutations are exact."""
+@pytest.mark.parametrize("schedule", ["plain", "blocks5", "tiles"])
+def test_brickwork_20q_depth20_against_oracle(schedule):
+    """The measured workload's circuit (U3 on every qubit, CZ on alternating
+    pairs, depth 20) at 20 qubits, on every execution path, against the FP64
+    oracle gate for gate."""
+    n = 20
+    gates = circuits.brickwork(n, 20, seed=2)
+    want = oracle.run(oracle.basis_state(n, 0), n, gates, threads=8)
     st = hip.State(n, hip.C64)
-    st.gaussian(5)
-    ref = st.download()
-    gates = circuits.brickwork(n, 2, seed=9)
-    inv = []
-    for g in reversed(gates):
-        if g["kind"] == "U3":
-            inv.append(circuits.gate("U3", g["q"][0], angle=-g["angle"], phi=-g["lam"], lam=-g["phi"]))
-        else:
-            inv.append(g)
+    st.set_basis(0)
+    lowered = ops.lower_all(gates)
+    if schedule == "plain":
+        st.apply(lowered)
+    elif schedule == "blocks5":
+        st.apply_schedule(hip.Schedule(lowered, n, 5))
+    else:
+        st.apply_schedule(hip.Schedule(lowered, n, mode="tiled"))
+    got = st.download()
+    assert oracle.infidelity(got, want) < 1e-10
+    assert np.abs(got - want).max() < 2e-6
+    st.close()
+
+
+def test_random_layers_20q_depth40_against_oracle():
+    """The 20-qubit random-circuit workload (1200 gates) plain, against the oracle."""
+    n = 20
+    gates = circuits.random_layered(n, 40, seed=1)
+    want = oracle.run(oracle.basis_state(n, 0), n, gates, threads=8)
+    st = hip.State(n, hip.C64)
+    st.set_basis(0)
     st.apply(ops.lower_all(gates))
-    st.apply(ops.lower_all(inv))
-    out = st.download()
-    assert oracle.infidelity(out, ref) < 1e-9
+    got = st.download()
+    assert oracle.infidelity(got, want) < 1e-10
+    assert np.abs(got - want).max() < 2e-6
+    st.close()
+
+
+def test_brickwork_28q_against_complex128():
+    """At 28 qubits (the measured size), four brickwork layers in complex64
+    (the tuned one-qubit and phase passes) against the same layers in
+    complex128, which runs on the generic pair kernel: infidelity <= 1e-9.
+    The layers must also move the state far from its input, so a pass that
+    did nothing would fail."""
+    n = 28
+    gates = circuits.brickwork(n, 4, seed=9)
+    ref = hip.State(n, hip.C128)
+    ref.gaussian(5)
+    st = hip.State(n, hip.C64)
+    st.copy_from(ref)
+    before = hip.State(n, hip.C128)
+    before.copy_from(ref)
+    lowered = ops.lower_all(gates)
+    st.apply(lowered)
+    ref.apply(lowered)
+    ov = st.inner(ref)
+    infid = 1.0 - abs(ov) ** 2 / (st.norm2() * ref.norm2())
+    assert infid < 1e-9, infid
+    moved = abs(before.inner(ref)) ** 2
+    assert moved < 0.01, moved
     perm = [circuits.gate("CX", 0, n - 1), circuits.gate("Swap", 3, n - 2), circuits.gate("X", n - 1),
             circuits.gate("CCX", 1, 2, 20)]
+    snap = hip.State(n, hip.C64)
+    snap.copy_from(st)
     st.apply(ops.lower_all(perm))
+    assert abs(st.inner(snap)) ** 2 < 0.5  # the permutation moved amplitudes
     st.apply(ops.lower_all(list(reversed(perm))))
-    assert np.array_equal(st.download().view(np.uint32), out.view(np.uint32))
+    assert np.array_equal(st.download().view(np.uint32), snap.download().view(np.uint32))
+    for s_ in (st, ref, before, snap):
+        s_.close()

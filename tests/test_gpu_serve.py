@@ -90,3 +90,30 @@ def test_duplicate_id_rejected():
         srv.submit(7, 8, g, [0.5])
     srv.wait(7)
     srv.close()
+
+
+def test_cancel_queued_and_running():
+    # one worker: a long 24-qubit request runs, a small one waits behind it
+    srv = hip.Server(budget_bytes=1 << 30, workers=1)
+    big = ops.lower_all(circuits.brickwork(24, 40, seed=5))
+    small = ops.lower_all(circuits.brickwork(10, 3, seed=6))
+    srv.submit(1, 24, big, [0.5])
+    srv.submit(2, 10, small, [0.5])
+    assert srv.cancel(2)
+    with pytest.raises(hip.AqsError, match="cancelled"):
+        srv.wait(2)
+    srv.cancel(1)  # stops after its current stage, or it had already finished
+    try:
+        srv.wait(1)
+    except hip.AqsError as e:
+        assert "cancelled" in str(e)
+    # the server keeps serving, with correct results
+    srv.submit(3, 10, small, np.linspace(0.05, 0.95, 20), keep_state=True)
+    res, idx, state = srv.wait(3)
+    want = oracle.run(oracle.basis_state(10, 0), 10, circuits.brickwork(10, 3, seed=6))
+    assert oracle.infidelity(state, want) < 1e-9
+    assert np.array_equal(idx.astype(np.uint32), oracle.sample_c64(state, 10, np.linspace(0.05, 0.95, 20)))
+    with pytest.raises(hip.OutOfRange):
+        srv.wait(3)  # a result is handed over once
+    assert srv.stats()["cancelled"] >= 1
+    srv.close()

@@ -87,6 +87,11 @@ def lib():
         L.orc_sample_c128.argtypes = [dp, ctypes.c_int, dp, ctypes.c_int, up]
         L.orc_sample_fp64_cdf.argtypes = [dp, ctypes.c_int, dp, ctypes.c_int, up]
         L.orc_max_threads.restype = ctypes.c_int
+        L.csv_apply.argtypes = [fp, ctypes.c_int, gp, ctypes.c_int]
+        L.csv_apply.restype = ctypes.c_int
+        L.csv_run.argtypes = [fp, ctypes.c_int, gp, ctypes.c_int, ctypes.c_int]
+        L.csv_run.restype = ctypes.c_int
+        L.csv_sample.argtypes = [fp, ctypes.c_int, dp, dp, ctypes.c_int, up, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -124,6 +129,29 @@ def run(sv: np.ndarray, n: int, gates, threads: int = 1) -> np.ndarray:
     out = np.ascontiguousarray(sv, dtype=np.complex128).copy()
     arr, _keep = _pack(gates)
     lib().orc_run_threads(_dp(out.view(np.float64)), n, arr, len(gates), threads)
+    return out
+
+
+def cpu_sv_run(sv: np.ndarray, n: int, gates, threads: int = 1) -> np.ndarray:
+    """The plain complex64 CPU baseline (cpu_sv.c): gates applied in place to
+    ``sv`` (complex64, modified and returned)."""
+    assert sv.dtype == np.complex64 and sv.flags.c_contiguous and sv.size == 1 << n
+    arr, _keep = _pack(gates)
+    done = lib().csv_run(sv.view(np.float32).ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n, arr, len(gates),
+                         threads)
+    if done != len(gates):
+        raise ValueError(f"cpu_sv does not implement gate {gates[done]['kind']}")
+    return sv
+
+
+def cpu_sv_sample(sv: np.ndarray, n: int, u: np.ndarray, threads: int = 1, cdf: np.ndarray = None) -> np.ndarray:
+    """Sampling on the complex64 CPU baseline: FP64 inclusive CDF, binary search per shot."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    if cdf is None:
+        cdf = np.empty(1 << n, dtype=np.float64)
+    out = np.zeros(len(u), dtype=np.uint32)
+    lib().csv_sample(sv.view(np.float32).ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n, _dp(cdf), _dp(u),
+                     len(u), out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), threads)
     return out
 
 

@@ -129,6 +129,15 @@ int orc_max_threads(void);
  * one N x N operator; returns -1 when that fails. */
 int orc_refmethod_run(double* sv, int n, const orc_gate* gates, int count, int threads);
 
+/* ---- plain complex64 CPU state vector (cpu_sv.c): the timed CPU baseline --
+ * sv: 2^n interleaved FP32 pairs, updated in place with `threads` OpenMP
+ * threads.  csv_apply returns 0 for a gate kind it does not implement;
+ * csv_run returns the number of gates applied. */
+int csv_apply(float* sv, int n, const orc_gate* g, int threads);
+int csv_run(float* sv, int n, const orc_gate* gates, int count, int threads);
+/* cdf: caller's scratch of 2^n doubles. */
+void csv_sample(const float* sv, int n, double* cdf, const double* u, int shots, uint32_t* out, int threads);
+
 #ifdef __cplusplus
 }
 #endif

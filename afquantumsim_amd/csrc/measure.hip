@@ -15,6 +15,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "internal.h"
@@ -27,6 +29,39 @@ constexpr int kBlock = 256;
 constexpr int kRedGrid = 1024;       // fixed reduction grid (deterministic order)
 constexpr int kSampleBlock = 1024;   // amplitudes per sampling block
 constexpr double kFixScale = 1152921504606846976.0;  // 2^60
+
+// Device scratch of aqs_sample_noalloc: the sampling tables of a state of up
+// to AQS_NOALLOC_MAX_QUBITS qubits (3 words per 1024 amplitudes plus two),
+// 24 MiB + 16 B, held by every device the library runs on until the process
+// ends.  Host code never uses the array's name as an address: it uses the
+// per-device pointer from hipGetSymbolAddress (noalloc_scratch_ptr).
+constexpr size_t kNoallocScratchBytes = (size_t)3 * ((1ull << AQS_NOALLOC_MAX_QUBITS) / kSampleBlock) * 8 + 256;
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mutex;
+void* g_noalloc_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per
+// device (each device holds its own copy of the array).
+void* noalloc_scratch_ptr() {
+    int dev = 0;
+    AQS_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) fail(AQS_ERR_DEVICE, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        AQS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) fail(AQS_ERR_DEVICE, "hipGetSymbolAddress returned no address for the sampling scratch");
+        g_noalloc_ptr[dev] = p;
+    }
+    return g_noalloc_ptr[dev];
+}
 
 template <typename T2>
 __device__ __forceinline__ double prob_of(const T2& v) {
@@ -521,6 +556,26 @@ aqs_status aqs_measure_qubit(aqs_state s, int qubit, double u, int* outcome, dou
 }
 
 aqs_status aqs_collapse_to(aqs_state s, uint64_t index) { return aqs_state_set_basis(s, index); }
+
+aqs_status aqs_sample_noalloc(const void* amps, int nqubits, int precision, const double* u, uint32_t shots,
+                              uint64_t* out_index, void* stream) {
+    return guard([&] {
+        if (nqubits < 1 || nqubits > AQS_NOALLOC_MAX_QUBITS)
+            fail(AQS_ERR_INVALID_ARGUMENT, "aqs_sample_noalloc: nqubits must be 1.." +
+                                               std::to_string(AQS_NOALLOC_MAX_QUBITS) + " (fixed device scratch)");
+        if (precision != AQS_C64 && precision != AQS_C128) fail(AQS_ERR_INVALID_ARGUMENT, "unknown precision");
+        if (shots == 0) return;
+        if (!amps || !u || !out_index) fail(AQS_ERR_INVALID_ARGUMENT, "null state, uniforms or output");
+        const uint64_t N = 1ull << nqubits;
+        const uint64_t nb = (N + kSampleBlock - 1) / kSampleBlock;
+        static_assert(sample_table_bytes(((1ull << AQS_NOALLOC_MAX_QUBITS) + kSampleBlock - 1) / kSampleBlock) <=
+                          kNoallocScratchBytes,
+                      "scratch holds the tables of the largest state");
+        void* base = noalloc_scratch_ptr();
+        launch_sampling(amps, precision, N, sample_tables(base, nb), u, shots, out_index,
+                        stream ? (hipStream_t)stream : default_stream());
+    });
+}
 
 aqs_status aqs_inner(aqs_state a, aqs_state b, double* re_im) {
     return guard([&] {

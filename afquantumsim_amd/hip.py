@@ -23,6 +23,7 @@ AQS_ERR_NO_MEMORY = 6
 
 C64 = 0
 C128 = 1
+NOALLOC_MAX_QUBITS = 30
 
 OP_MAT1 = 1
 OP_DIAG1 = 2
@@ -86,6 +87,7 @@ EXPORTS = [
     "aqs_batch_create", "aqs_batch_autotune", "aqs_batch_configure", "aqs_batch_run", "aqs_batch_state", "aqs_batch_info",
     "aqs_batch_destroy", "aqs_autotune", "aqs_tune_set", "aqs_server_create", "aqs_server_set_starvation_limit",
     "aqs_server_submit", "aqs_server_wait", "aqs_server_stats", "aqs_server_destroy", "aqs_server_cancel",
+    "aqs_sample_noalloc",
 ]
 
 
@@ -173,6 +175,7 @@ def load(path: str = LIB_PATH):
         "aqs_server_stats": [vp, ctypes.POINTER(ServerStats)],
         "aqs_server_cancel": [vp, u64, ctypes.POINTER(i32)],
         "aqs_server_destroy": [vp],
+        "aqs_sample_noalloc": [vp, i32, i32, vp, ctypes.c_uint32, vp, vp],
     }
     for name, args in sigs.items():
         f = getattr(L, name)

@@ -228,6 +228,23 @@ aqs_status aqs_collapse_to(aqs_state s, uint64_t index);
 /* <a|b> in FP64 for two states of equal qubit count (any precisions). */
 aqs_status aqs_inner(aqs_state a, aqs_state b, double* re_im);
 
+/* ---- sampling without allocation (no reference counterpart) --------------
+ * For callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  Same rule
+ * and bit-identical indices as aqs_sample, on caller-owned device buffers:
+ * amps = 2^nqubits amplitudes (complex64 or complex128 as `precision` says),
+ * u = `shots` uniforms (double), out_index = `shots` indices (uint64), all
+ * device pointers; stream = a hipStream_t (NULL: the library's stream).  The
+ * work is enqueued and the call returns without waiting.  It calls no
+ * hipMalloc / hipFree: the sampling tables live in one fixed device array per
+ * device (24 MiB + 256 B, held until the process ends), which limits nqubits
+ * to AQS_NOALLOC_MAX_QUBITS (larger: AQS_ERR_INVALID_ARGUMENT).  Every call
+ * on a device shares that array, so calls that overlap on one device (other
+ * streams, other threads) race on it and must be serialised by the caller. */
+#define AQS_NOALLOC_MAX_QUBITS 30
+aqs_status aqs_sample_noalloc(const void* amps, int nqubits, int precision, const double* u, uint32_t shots,
+                              uint64_t* out_index, void* stream);
+
 /* ---- batched small states (no reference counterpart: the reference runs
  * one state per QSimulator) ----------------------------------------------- */
 

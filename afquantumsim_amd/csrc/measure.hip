@@ -31,11 +31,13 @@ constexpr int kSampleBlock = 1024;   // amplitudes per sampling block
 constexpr double kFixScale = 1152921504606846976.0;  // 2^60
 
 // Device scratch of aqs_sample_noalloc: the sampling tables of a state of up
-// to AQS_NOALLOC_MAX_QUBITS qubits (3 words per 1024 amplitudes plus two),
-// 24 MiB + 16 B, held by every device the library runs on until the process
-// ends.  Host code never uses the array's name as an address: it uses the
+// to AQS_NOALLOC_MAX_QUBITS qubits (3 words per 1024 amplitudes, 3 per 2^20
+// amplitudes and two), 24 MiB + 24 KiB + 256 B, held by every device the
+// library runs on until the process ends.  Host code never uses the array's name as an address: it uses the
 // per-device pointer from hipGetSymbolAddress (noalloc_scratch_ptr).
-constexpr size_t kNoallocScratchBytes = (size_t)3 * ((1ull << AQS_NOALLOC_MAX_QUBITS) / kSampleBlock) * 8 + 256;
+constexpr size_t kNoallocScratchBytes =
+    (size_t)3 * ((1ull << AQS_NOALLOC_MAX_QUBITS) / kSampleBlock) * 8 +
+    (size_t)3 * ((1ull << AQS_NOALLOC_MAX_QUBITS) / kSampleBlock / 1024) * 8 + 256;
 
 }  // namespace
 
@@ -177,7 +179,9 @@ __global__ __launch_bounds__(kBlock) void k_probs(const T2* __restrict__ a, uint
 // ---- sampling -------------------------------------------------------------
 
 // sums[b] = exact sum of quantised |a|^2 over block b; lastnz[b] = last index
-// with non-zero quantised probability in block b (-1 when none).
+// with non-zero quantised probability in block b (-1 when none).  Each
+// thread reads 16 bytes per load: complex64 amplitude pairs, complex128
+// single amplitudes.
 template <typename T2>
 __global__ __launch_bounds__(kBlock) void k_block_sums(const T2* __restrict__ a, uint64_t N, uint64_t* __restrict__ sums,
                                                        int64_t* __restrict__ lastnz) {
@@ -186,12 +190,32 @@ __global__ __launch_bounds__(kBlock) void k_block_sums(const T2* __restrict__ a,
     const uint64_t b = blockIdx.x;
     uint64_t acc = 0;
     int64_t last = -1;
-    for (int j = 0; j < kSampleBlock / kBlock; ++j) {
-        const uint64_t i = b * kSampleBlock + (uint64_t)j * kBlock + threadIdx.x;
-        if (i < N) {
-            const uint64_t q = quantise(prob_of(a[i]));
-            acc += q;
-            if (q) last = (int64_t)i;
+    if constexpr (sizeof(T2) == 8) {
+        constexpr int kPairs = kSampleBlock / 2 / kBlock;  // float4 loads per thread
+        const float4* a4 = reinterpret_cast<const float4*>(a);
+        float4 v[kPairs];
+#pragma unroll
+        for (int j = 0; j < kPairs; ++j) {
+            const uint64_t i = b * kSampleBlock + 2 * ((uint64_t)j * kBlock + threadIdx.x);
+            v[j] = i < N ? a4[i >> 1] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int j = 0; j < kPairs; ++j) {
+            const uint64_t i = b * kSampleBlock + 2 * ((uint64_t)j * kBlock + threadIdx.x);
+            const uint64_t q0 = quantise(prob_of(make_float2(v[j].x, v[j].y)));
+            const uint64_t q1 = quantise(prob_of(make_float2(v[j].z, v[j].w)));
+            acc += q0 + q1;
+            if (q0) last = max(last, (int64_t)i);
+            if (q1) last = max(last, (int64_t)i + 1);
+        }
+    } else {
+        for (int j = 0; j < kSampleBlock / kBlock; ++j) {
+            const uint64_t i = b * kSampleBlock + (uint64_t)j * kBlock + threadIdx.x;
+            if (i < N) {
+                const uint64_t q = quantise(prob_of(a[i]));
+                acc += q;
+                if (q) last = (int64_t)i;
+            }
         }
     }
     uint64_t s = block_reduce_sum(acc, sh);
@@ -207,44 +231,82 @@ __global__ __launch_bounds__(kBlock) void k_block_sums(const T2* __restrict__ a,
     }
 }
 
-// Inclusive prefix of the block sums (exact), plus the global total and the
-// global last non-zero index.  One workgroup of 1024 threads.
-__global__ __launch_bounds__(1024) void k_scan_blocks(const uint64_t* __restrict__ sums, const int64_t* __restrict__ lastnz,
-                                                      uint64_t nb, uint64_t* __restrict__ incl, uint64_t* __restrict__ misc) {
-    __shared__ uint64_t sh[1024];
-    __shared__ int64_t shl[1024];
+// The exact inclusive scan of the block sums in three steps, so that no
+// single workgroup walks all of them: each workgroup scans a chunk of
+// kScanChunk blocks (k_scan_chunks), one workgroup scans the chunk totals
+// (k_scan_top, which also forms the total and the last non-zero index), and
+// every block adds its chunk's offset (k_scan_fixup).
+constexpr int kScanChunk = 1024;
+
+__global__ __launch_bounds__(kScanChunk) void k_scan_chunks(const uint64_t* __restrict__ sums,
+                                                            const int64_t* __restrict__ lastnz, uint64_t nb,
+                                                            uint64_t* __restrict__ incl, uint64_t* __restrict__ ctot,
+                                                            int64_t* __restrict__ clast) {
+    __shared__ uint64_t sh[kScanChunk];
+    __shared__ int64_t shl[kScanChunk];
     const int t = threadIdx.x;
-    const uint64_t per = (nb + 1023) / 1024;
-    const uint64_t lo = (uint64_t)t * per, hi = min(nb, lo + per);
-    uint64_t acc = 0;
-    int64_t last = -1;
-    for (uint64_t i = lo; i < hi; ++i) {
-        acc += sums[i];
-        last = max(last, lastnz[i]);
-    }
-    sh[t] = acc;
-    shl[t] = last;
+    const uint64_t b = (uint64_t)blockIdx.x * kScanChunk + t;
+    sh[t] = b < nb ? sums[b] : 0;
+    shl[t] = b < nb ? lastnz[b] : -1;
     __syncthreads();
-    // inclusive Hillis-Steele scan over 1024 chunk totals
-    for (int off = 1; off < 1024; off <<= 1) {
-        uint64_t v = (t >= off) ? sh[t - off] : 0;
+    for (int off = 1; off < kScanChunk; off <<= 1) {
+        const uint64_t v = t >= off ? sh[t - off] : 0;
         __syncthreads();
         sh[t] += v;
         __syncthreads();
     }
-    uint64_t run = (t > 0) ? sh[t - 1] : 0;
+    if (b < nb) incl[b] = sh[t];
+    if (t == kScanChunk - 1) ctot[blockIdx.x] = sh[t];
+    for (int st = kScanChunk / 2; st > 0; st >>= 1) {
+        if (t < st) shl[t] = max(shl[t], shl[t + st]);
+        __syncthreads();
+    }
+    if (t == 0) clast[blockIdx.x] = shl[0];
+}
+
+// Exclusive prefix of the nc chunk totals (one workgroup of 1024 threads,
+// each over a contiguous range), the total and the last non-zero index.
+__global__ __launch_bounds__(1024) void k_scan_top(const uint64_t* __restrict__ ctot, const int64_t* __restrict__ clast,
+                                                   uint64_t nc, uint64_t* __restrict__ cexcl, uint64_t* __restrict__ misc) {
+    __shared__ uint64_t sh[1024];
+    __shared__ int64_t shl[1024];
+    const int t = threadIdx.x;
+    const uint64_t per = (nc + 1023) / 1024;
+    const uint64_t lo = (uint64_t)t * per, hi = min(nc, lo + per);
+    uint64_t acc = 0;
+    int64_t last = -1;
     for (uint64_t i = lo; i < hi; ++i) {
-        run += sums[i];
-        incl[i] = run;
+        acc += ctot[i];
+        last = max(last, clast[i]);
+    }
+    sh[t] = acc;
+    shl[t] = last;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const uint64_t v = t >= off ? sh[t - off] : 0;
+        __syncthreads();
+        sh[t] += v;
+        __syncthreads();
+    }
+    uint64_t run = t > 0 ? sh[t - 1] : 0;
+    for (uint64_t i = lo; i < hi; ++i) {
+        cexcl[i] = run;
+        run += ctot[i];
     }
     for (int st = 512; st > 0; st >>= 1) {
         if (t < st) shl[t] = max(shl[t], shl[t + st]);
         __syncthreads();
     }
     if (t == 0) {
-        misc[0] = sh[1023];           // total
-        misc[1] = (uint64_t)shl[0];   // last non-zero index (as int64 bits)
+        misc[0] = sh[1023];          // total
+        misc[1] = (uint64_t)shl[0];  // last non-zero index (as int64 bits)
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_fixup(uint64_t* __restrict__ incl, const uint64_t* __restrict__ cexcl,
+                                                       uint64_t nb) {
+    const uint64_t b = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (b < nb) incl[b] += cexcl[b / kScanChunk];
 }
 
 // One wavefront per shot.
@@ -322,9 +384,15 @@ struct SampleTables {
     uint64_t* incl;
     int64_t* lastnz;
     uint64_t* misc;
+    uint64_t* ctot;   // per chunk of kScanChunk blocks: total,
+    int64_t* clast;   // last non-zero index,
+    uint64_t* cexcl;  // exclusive prefix
 };
 
-constexpr size_t sample_table_bytes(uint64_t nb) { return (size_t)(3 * nb + 2) * sizeof(uint64_t); }
+constexpr uint64_t scan_chunks(uint64_t nb) { return (nb + kScanChunk - 1) / kScanChunk; }
+constexpr size_t sample_table_bytes(uint64_t nb) {
+    return (size_t)(3 * nb + 2 + 3 * scan_chunks(nb)) * sizeof(uint64_t);
+}
 
 SampleTables sample_tables(void* base, uint64_t nb) {
     SampleTables t;
@@ -332,6 +400,9 @@ SampleTables sample_tables(void* base, uint64_t nb) {
     t.incl = t.sums + nb;
     t.lastnz = (int64_t*)(t.incl + nb);
     t.misc = (uint64_t*)(t.lastnz + nb);
+    t.ctot = t.misc + 2;
+    t.clast = (int64_t*)(t.ctot + scan_chunks(nb));
+    t.cexcl = (uint64_t*)(t.clast + scan_chunks(nb));
     return t;
 }
 
@@ -345,8 +416,13 @@ void launch_sampling(const void* amps, int precision, uint64_t N, const SampleTa
     else
         hipLaunchKernelGGL(k_block_sums<double2>, dim3((unsigned)nb), dim3(kBlock), 0, stream, (const double2*)amps,
                            N, t.sums, t.lastnz);
-    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, stream, (const uint64_t*)t.sums,
-                       (const int64_t*)t.lastnz, nb, t.incl, t.misc);
+    const uint64_t nc = scan_chunks(nb);
+    hipLaunchKernelGGL(k_scan_chunks, dim3((unsigned)nc), dim3(kScanChunk), 0, stream, (const uint64_t*)t.sums,
+                       (const int64_t*)t.lastnz, nb, t.incl, t.ctot, t.clast);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, stream, (const uint64_t*)t.ctot, (const int64_t*)t.clast, nc,
+                       t.cexcl, t.misc);
+    hipLaunchKernelGGL(k_scan_fixup, dim3((unsigned)((nb + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream, t.incl,
+                       (const uint64_t*)t.cexcl, nb);
     const unsigned g = (shots + (kBlock / 64) - 1) / (kBlock / 64);
     if (precision == AQS_C64)
         hipLaunchKernelGGL(k_sample<float2>, dim3(g), dim3(kBlock), 0, stream, (const float2*)amps, N,

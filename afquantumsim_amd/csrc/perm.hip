@@ -57,6 +57,11 @@ __device__ __forceinline__ uint64_t permute_bits(uint64_t v, const int* map, int
 // 5.. and the permuted position of every tile element.
 constexpr int kMaxTileBits = 11;
 constexpr int kMaxTile = 1 << kMaxTileBits;
+// Elements per thread and tile: complex64 moves pairs (2^11 / 2 / 256 = 4),
+// complex128 single amplitudes of tiles of at most 2^10 (4).
+constexpr int kUnroll = kMaxTile / 2 / kBlock;
+static_assert(kUnroll * kBlock * 2 >= kMaxTile, "a thread's share of a complex64 tile fits the unroll");
+static_assert(kUnroll * kBlock >= (1 << 10), "a thread's share of a complex128 tile fits the unroll");
 
 // LDS slot of tile element p: one element of padding per 32.  Permuted
 // writes (a bit reversal sends consecutive elements 32 apart) then spread
@@ -89,23 +94,40 @@ __global__ __launch_bounds__(kBlock) void k_bitperm(PermArgs a) {
         if (o2 < o) continue;  // the partner tile's workgroup does this pair
         const uint64_t b1 = deposit(o, a.rpos, a.nr), b2 = deposit(o2, a.rpos, a.nr);
         const bool pair = o2 != o;
+        // Every load of the pair is issued before the first LDS write: a
+        // thread moves up to kUnroll elements of each tile, so the whole tile
+        // pair is in flight at once instead of one load round trip per element.
         if constexpr (sizeof(T2) == 8) {
             // complex64: amplitude pairs (16 bytes) along the contiguous bits
             float4* sv4 = reinterpret_cast<float4*>(a.sv);
-            for (int f = threadIdx.x; f < tile / 2; f += kBlock) {
+            const int units = tile / 2;
+            float4 xa[kUnroll], xb[kUnroll];
+#pragma unroll
+            for (int k = 0; k < kUnroll; ++k) {
+                const int f = threadIdx.x + k * kBlock;
+                if (f >= units) break;
                 const int t = 2 * f;
                 const uint64_t off = (uint64_t)(t & 31) | hi_off[t >> 5];
-                const float4 x = ld_nt(&sv4[(b1 | off) >> 1]);
-                A[pmap[t]] = make_float2(x.x, x.y);
-                A[pmap[t + 1]] = make_float2(x.z, x.w);
+                xa[k] = ld_nt(&sv4[(b1 | off) >> 1]);
+                if (pair) xb[k] = ld_nt(&sv4[(b2 | off) >> 1]);
+            }
+#pragma unroll
+            for (int k = 0; k < kUnroll; ++k) {
+                const int f = threadIdx.x + k * kBlock;
+                if (f >= units) break;
+                const int t = 2 * f;
+                A[pmap[t]] = make_float2(xa[k].x, xa[k].y);
+                A[pmap[t + 1]] = make_float2(xa[k].z, xa[k].w);
                 if (pair) {
-                    const float4 y = ld_nt(&sv4[(b2 | off) >> 1]);
-                    B[pmap[t]] = make_float2(y.x, y.y);
-                    B[pmap[t + 1]] = make_float2(y.z, y.w);
+                    B[pmap[t]] = make_float2(xb[k].x, xb[k].y);
+                    B[pmap[t + 1]] = make_float2(xb[k].z, xb[k].w);
                 }
             }
             __syncthreads();
-            for (int f = threadIdx.x; f < tile / 2; f += kBlock) {
+#pragma unroll
+            for (int k = 0; k < kUnroll; ++k) {
+                const int f = threadIdx.x + k * kBlock;
+                if (f >= units) break;
                 const int t = 2 * f;
                 const uint64_t off = (uint64_t)(t & 31) | hi_off[t >> 5];
                 const int l = lds_slot(t);  // t even: t + 1 shares its run
@@ -117,13 +139,27 @@ __global__ __launch_bounds__(kBlock) void k_bitperm(PermArgs a) {
                 }
             }
         } else {
-            for (int t = threadIdx.x; t < tile; t += kBlock) {
+            T2 xa[kUnroll], xb[kUnroll];
+#pragma unroll
+            for (int k = 0; k < kUnroll; ++k) {
+                const int t = threadIdx.x + k * kBlock;
+                if (t >= tile) break;
                 const uint64_t off = (uint64_t)(t & 31) | hi_off[t >> 5];
-                A[pmap[t]] = sv[b1 | off];
-                if (pair) B[pmap[t]] = sv[b2 | off];
+                xa[k] = sv[b1 | off];
+                if (pair) xb[k] = sv[b2 | off];
+            }
+#pragma unroll
+            for (int k = 0; k < kUnroll; ++k) {
+                const int t = threadIdx.x + k * kBlock;
+                if (t >= tile) break;
+                A[pmap[t]] = xa[k];
+                if (pair) B[pmap[t]] = xb[k];
             }
             __syncthreads();
-            for (int t = threadIdx.x; t < tile; t += kBlock) {
+#pragma unroll
+            for (int k = 0; k < kUnroll; ++k) {
+                const int t = threadIdx.x + k * kBlock;
+                if (t >= tile) break;
                 const uint64_t off = (uint64_t)(t & 31) | hi_off[t >> 5];
                 sv[(pair ? b2 : b1) | off] = A[lds_slot(t)];
                 if (pair) sv[b1 | off] = B[lds_slot(t)];

@@ -374,26 +374,13 @@ def served_requests(count, seed):
     return out
 
 
-def served(args):
-    """Poisson arrivals of mixed-size brickwork requests into the
-    continuous-batching server; p50/p99 latency (arrival -> sampled
-    indices), throughput, and the same requests one at a time."""
+def serve_stream(reqs, rate, budget, workers, seed):
+    """Poisson arrivals at `rate` per second into a fresh server; returns
+    per-request latency (arrival -> sampled indices) and queueing, wall time
+    and server statistics."""
     import threading
-    budget = 64 << 30
-    workers = 4
-    # service-time calibration: requests alone, one after another
-    calib = served_requests(24, seed=77)
-    srv = hip.Server(budget, workers=1)
-    t0 = time.perf_counter()
-    for r in calib:
-        srv.submit(r["id"], r["n"], r["ops"], r["u"])
-        srv.wait(r["id"])
-    alone_s = (time.perf_counter() - t0) / len(calib)
-    srv.close()
-    lam = 0.7 / alone_s
-    reqs = served_requests(args.serve_requests, seed=78)
-    rng = np.random.default_rng(79)
-    arrivals = np.cumsum(rng.exponential(1.0 / lam, size=len(reqs)))
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(reqs)))
     srv = hip.Server(budget, workers=workers)
     lat = {}
     submitted = threading.Semaphore(0)
@@ -401,7 +388,7 @@ def served(args):
     def collector():
         for r in reqs:
             submitted.acquire()
-            res, idx, _ = srv.wait(r["id"])
+            res, _, _ = srv.wait(r["id"])
             lat[r["id"]] = (res["t_done"] - res["t_submit"], res["t_admit"] - res["t_submit"])
 
     th = threading.Thread(target=collector)
@@ -420,10 +407,63 @@ def served(args):
     l = np.array([lat[r["id"]][0] for r in reqs])
     q = np.array([lat[r["id"]][1] for r in reqs])
     gates = sum(len(r["ops"]) for r in reqs)
-    # accuracy: the served result of a few requests against complex128
+    return {"arrival_rate_per_s": round(rate, 2), "p50_ms": round(1e3 * float(np.percentile(l, 50)), 2),
+            "p99_ms": round(1e3 * float(np.percentile(l, 99)), 2), "mean_queue_ms": round(1e3 * float(q.mean()), 2),
+            "throughput_req_per_s": round(len(reqs) / wall, 2), "gate_apps_per_s": round(gates / wall, 1),
+            "max_concurrent": stats["max_concurrent"], "peak_resident_bytes": stats["peak_resident_bytes"]}
+
+
+def served(args):
+    """Poisson arrivals of mixed-size brickwork requests into the request
+    server (4 workers, compile() fusion per request): p50/p99 latency
+    (arrival -> sampled indices) and throughput at arrival rates from 0.7x
+    one worker's capacity up to and past the saturation of all four; the
+    same requests run one at a time with the plain per-gate kernels as the
+    comparator; infidelity of served results against complex128."""
+    budget = 64 << 30
+    workers = 4
+    # service-time calibration: requests alone, one after another
+    calib = served_requests(24, seed=77)
+    srv = hip.Server(budget, workers=1)
+    t0 = time.perf_counter()
+    for r in calib:
+        srv.submit(r["id"], r["n"], r["ops"], r["u"])
+        srv.wait(r["id"])
+    alone_s = (time.perf_counter() - t0) / len(calib)
+    srv.close()
+    reqs = served_requests(args.serve_requests, seed=78)
+    sweep = []
+    for mult in (0.7, 2.0, 4.0, 8.0):
+        entry = serve_stream(reqs, mult / alone_s, budget, workers, seed=79)
+        entry["rate_vs_one_worker_capacity"] = mult
+        sweep.append(entry)
+        log(f"served at {mult}x one worker's capacity: {entry}")
+    main = sweep[0]
+    # plain comparator: the calibration requests one at a time with the
+    # per-gate kernels (no fusion, no concurrency), same uniforms; the fused
+    # one-worker service time above is on the same requests
+    plain = calib
+    st_cache = {}
+    t0 = time.perf_counter()
+    worst_plain = 0.0
+    for r in plain:
+        st = st_cache.get(r["n"])
+        if st is None:
+            st = st_cache[r["n"]] = hip.State(r["n"], hip.C64)
+        st.set_basis(0)
+        st.apply(r["ops"])
+        st.sample(r["u"])
+    plain_s = time.perf_counter() - t0
+    for st in st_cache.values():
+        st.close()
+    plain_gates = sum(len(r["ops"]) for r in plain)
+    # accuracy of served results (every request of at most 24 qubits in the
+    # plain sample and beyond, up to 16) against complex128, and of the
+    # plain path on the same requests
     worst = 0.0
+    checked = 0
     chk = hip.Server(budget, workers=2)
-    for r in [x for x in reqs if x["n"] <= 22][:6]:
+    for r in [x for x in reqs if x["n"] <= 24][:16]:
         chk.submit(10_000 + r["id"], r["n"], r["ops"], r["u"][:10], keep_state=True)
         _, _, psi = chk.wait(10_000 + r["id"])
         ref = hip.State(r["n"], hip.C128)
@@ -431,17 +471,27 @@ def served(args):
         x = ref.download()
         ov = np.vdot(x, psi.astype(np.complex128))
         worst = max(worst, 1.0 - abs(ov) ** 2 / (np.vdot(psi, psi).real * np.vdot(x, x).real))
+        st = hip.State(r["n"], hip.C64)
+        st.set_basis(0)
+        st.apply(r["ops"])
+        ovp = st.inner(ref)
+        worst_plain = max(worst_plain, 1.0 - abs(ovp) ** 2 / (st.norm2() * ref.norm2()))
+        st.close()
         ref.close()
+        checked += 1
     chk.close()
-    out = {"requests": len(reqs), "arrival_rate_per_s": round(lam, 2), "workers": workers,
-           "hbm_budget_bytes": budget, "p50_ms": round(1e3 * float(np.percentile(l, 50)), 2),
-           "p99_ms": round(1e3 * float(np.percentile(l, 99)), 2),
-           "mean_queue_ms": round(1e3 * float(q.mean()), 2),
-           "throughput_req_per_s": round(len(reqs) / wall, 2),
-           "gate_apps_per_s": round(gates / wall, 1),
-           "alone_mean_service_ms": round(1e3 * alone_s, 2),
-           "max_concurrent": stats["max_concurrent"], "peak_resident_bytes": stats["peak_resident_bytes"],
-           "max_infidelity_checked": worst, "shots_per_request": 1000,
+    out = {"requests": len(reqs), "workers": workers, "hbm_budget_bytes": budget,
+           "arrival_rate_per_s": main["arrival_rate_per_s"], "p50_ms": main["p50_ms"], "p99_ms": main["p99_ms"],
+           "mean_queue_ms": main["mean_queue_ms"], "throughput_req_per_s": main["throughput_req_per_s"],
+           "gate_apps_per_s": main["gate_apps_per_s"], "max_concurrent": main["max_concurrent"],
+           "alone_mean_service_ms": round(1e3 * alone_s, 2), "rate_sweep": sweep,
+           "plain_per_gate_one_at_a_time": {
+               "requests": len(plain), "mean_service_ms": round(1e3 * plain_s / len(plain), 2),
+               "throughput_req_per_s": round(len(plain) / plain_s, 2),
+               "gate_apps_per_s": round(plain_gates / plain_s, 1), "max_infidelity_checked": worst_plain,
+               "fused_one_worker_mean_service_ms": round(1e3 * alone_s, 2),
+               "sample": f"the {len(plain)} calibration requests (same distribution), back to back, no queueing"},
+           "max_infidelity_checked": worst, "requests_checked": checked, "shots_per_request": 1000,
            "qubits": "log-uniform 12..28", "depth": "U{10..100}"}
     log(f"served: {out}")
     return out

@@ -69,6 +69,38 @@ std::vector<BatchOp> batch_absorb(const aqs_op* ops, size_t count, int n);
 void batch_phases(const std::vector<BatchOp>& flat, int n, int m, std::vector<BatchOp>& ops,
                   std::vector<BatchSeg>& segs, std::vector<BatchPhase>& phases);
 
+// ---- request groups: small circuits of the request server run together ----
+
+// Largest state of a group member, and the launch shape of a group.
+constexpr int kGroupMaxQubits = 16;
+constexpr int kGroupLdsQubits = 13;
+constexpr int kGroupBlock = 512;
+
+// One member of a group: its circuit from |0..0>, its uniforms (one per
+// shot) and where its results go (host memory).
+struct GroupItem {
+    int n = 0;                     // 1..kGroupMaxQubits
+    const aqs_op* ops = nullptr;   // MAT1, DIAG1, PHASE, X, SWAP
+    size_t nops = 0;
+    const double* u = nullptr;
+    uint32_t shots = 0;
+    uint64_t* indices = nullptr;   // shots entries
+    double* norm2 = nullptr;       // FP64 norm^2 of the final state
+    float* state = nullptr;        // complex64 interleaved (2^n pairs), or null
+    size_t passes = 0;             // out: sweeps (phases) of its program
+};
+
+// Upper bound of the arena bytes one member takes (state, program, sampling
+// tables, uniforms, indices and its share of the arena's fixed part).
+size_t group_item_bytes(int n, size_t nops, uint32_t shots);
+
+// Runs the members' circuits as one batched launch (k_batch, tiles of
+// 2^kGroupLdsQubits amplitudes) on `stream` inside the device arena, then
+// samples every final state with the exact fixed-point CDF of aqs_sample and
+// reduces its norm; returns when the results are in host memory.  The
+// members' group_item_bytes must sum to at most arena_bytes.
+void run_group(GroupItem* items, int count, void* arena, size_t arena_bytes, hipStream_t stream);
+
 }  // namespace aqs_hip
 
 #endif

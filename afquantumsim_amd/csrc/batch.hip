@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <complex>
+#include <cstring>
 #include <map>
 #include <numeric>
 #include <vector>
@@ -792,6 +793,131 @@ aqs_batch check_batch(aqs_batch b) {
 }
 
 }  // namespace
+
+// ---- request groups --------------------------------------------------------
+
+namespace aqs_hip {
+
+namespace {
+
+constexpr size_t kArenaAlign = 256;
+constexpr int kGroupRegions = 10;  // aligned regions of the arena layout
+
+size_t align_up(size_t x) { return (x + kArenaAlign - 1) / kArenaAlign * kArenaAlign; }
+
+}  // namespace
+
+// A member's program holds at most nops absorbed ops, each cut into at most
+// three (a swap becomes three X gates), at most one segment per op and at
+// most one phase per absorbed op plus one.
+size_t group_item_bytes(int n, size_t nops, uint32_t shots) {
+    const size_t prog = sizeof(Desc) + (nops + 1) * sizeof(BatchPhase) + 3 * nops * sizeof(BatchSeg) +
+                        3 * nops * sizeof(BatchOp);
+    const size_t io = sizeof(double) * 2 + 2 * sizeof(uint64_t) * (size_t)shots;  // expectation, norm, u, indices
+    return prog + io + align_up(sizeof(float2) << n) + align_up(sample_norm_scratch_bytes(n)) +
+           kGroupRegions * kArenaAlign;
+}
+
+void run_group(GroupItem* items, int count, void* arena, size_t arena_bytes, hipStream_t stream) {
+    if (count < 1) return;
+    // programs, biggest state first (the persistent grid takes states in order)
+    std::vector<int> order(count);
+    std::vector<uint64_t> cost(count);
+    std::vector<std::vector<BatchOp>> flat(count);
+    for (int k = 0; k < count; ++k) {
+        const GroupItem& it = items[k];
+        if (it.n < 1 || it.n > kGroupMaxQubits) fail(AQS_ERR_INVALID_ARGUMENT, "group member qubits out of range");
+        flat[k] = batch_absorb(it.ops, it.nops, it.n);
+        cost[k] = (uint64_t)(flat[k].size() + 1) << it.n;
+    }
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return cost[a] > cost[c]; });
+    std::vector<Desc> desc;
+    std::vector<BatchOp> ops;
+    std::vector<BatchSeg> segs;
+    std::vector<BatchPhase> phases;
+    size_t shots_total = 0;
+    std::vector<size_t> shot_off(count);
+    for (int k = 0; k < count; ++k) {
+        shot_off[k] = shots_total;
+        shots_total += items[k].shots;
+    }
+    // arena layout: queue | <sum Z> | upload (desc, phases, segs, ops, u) |
+    // download (norms, indices) | states | sampling scratch
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = off;
+        off += align_up(bytes);
+        return at;
+    };
+    for (int k : order) {
+        Desc d{};
+        d.phase_begin = (uint32_t)phases.size();
+        batch_phases(flat[k], items[k].n, kGroupLdsQubits, ops, segs, phases);
+        d.phase_count = (uint32_t)(phases.size() - d.phase_begin);
+        d.n = items[k].n;
+        d.index = k;
+        items[k].passes = d.phase_count;
+        desc.push_back(d);
+    }
+    const size_t o_queue = take(sizeof(unsigned));
+    const size_t o_zout = take(sizeof(double) * count);
+    const size_t o_desc = take(sizeof(Desc) * desc.size());
+    const size_t o_phases = take(sizeof(BatchPhase) * phases.size());
+    const size_t o_segs = take(sizeof(BatchSeg) * std::max<size_t>(1, segs.size()));
+    const size_t o_ops = take(sizeof(BatchOp) * std::max<size_t>(1, ops.size()));
+    const size_t o_u = take(sizeof(double) * std::max<size_t>(1, shots_total));
+    const size_t o_norm = take(sizeof(double) * count);
+    const size_t o_idx = take(sizeof(uint64_t) * std::max<size_t>(1, shots_total));
+    std::vector<size_t> o_state(count), o_scr(count);
+    for (int k = 0; k < count; ++k) o_state[k] = take(sizeof(float2) << items[k].n);
+    for (int k = 0; k < count; ++k) o_scr[k] = take(sample_norm_scratch_bytes(items[k].n));
+    if (off > arena_bytes) fail(AQS_ERR_RUNTIME, "request group exceeds its device arena");
+    char* base = (char*)arena;
+    // amplitude offsets from the first state (the kernel's pool)
+    for (Desc& d : desc) d.amp_offset = (o_state[d.index] - o_state[0]) / sizeof(float2);
+
+    // one upload: desc .. u are contiguous
+    std::vector<char> up(o_norm - o_desc);
+    std::memcpy(up.data(), desc.data(), sizeof(Desc) * desc.size());
+    std::memcpy(up.data() + (o_phases - o_desc), phases.data(), sizeof(BatchPhase) * phases.size());
+    if (!segs.empty()) std::memcpy(up.data() + (o_segs - o_desc), segs.data(), sizeof(BatchSeg) * segs.size());
+    if (!ops.empty()) std::memcpy(up.data() + (o_ops - o_desc), ops.data(), sizeof(BatchOp) * ops.size());
+    for (int k = 0; k < count; ++k)
+        if (items[k].shots)
+            std::memcpy(up.data() + (o_u - o_desc) + sizeof(double) * shot_off[k], items[k].u,
+                        sizeof(double) * items[k].shots);
+    AQS_HIP(hipMemcpyAsync(base + o_desc, up.data(), up.size(), hipMemcpyHostToDevice, stream));
+
+    set_lds_limit();
+    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, stream, (unsigned*)(base + o_queue));
+    const unsigned grid = (unsigned)std::min(grid_for(kGroupBlock, kGroupLdsQubits), count);
+    hipLaunchKernelGGL(k_batch<kGroupBlock>, dim3(grid), dim3(kGroupBlock), lds_bytes_for(kGroupLdsQubits), stream,
+                       (const Desc*)(base + o_desc), count, (const BatchPhase*)(base + o_phases),
+                       (const BatchSeg*)(base + o_segs), (const BatchOp*)(base + o_ops),
+                       (float2*)(base + o_state[0]), (double*)(base + o_zout), (unsigned*)(base + o_queue),
+                       kGroupLdsQubits);
+    AQS_HIP(hipGetLastError());
+    for (int k = 0; k < count; ++k)
+        queue_sample_norm(base + o_state[k], items[k].n, base + o_scr[k],
+                          (const double*)(base + o_u) + shot_off[k], items[k].shots,
+                          (uint64_t*)(base + o_idx) + shot_off[k], (double*)(base + o_norm) + k, stream);
+    std::vector<char> down(o_state[0] - o_norm);
+    AQS_HIP(hipMemcpyAsync(down.data(), base + o_norm, down.size(), hipMemcpyDeviceToHost, stream));
+    for (int k = 0; k < count; ++k)
+        if (items[k].state)
+            AQS_HIP(hipMemcpyAsync(items[k].state, base + o_state[k], sizeof(float2) << items[k].n,
+                                   hipMemcpyDeviceToHost, stream));
+    AQS_HIP(hipStreamSynchronize(stream));
+    for (int k = 0; k < count; ++k) {
+        if (items[k].norm2) std::memcpy(items[k].norm2, down.data() + sizeof(double) * k, sizeof(double));
+        if (items[k].shots)
+            std::memcpy(items[k].indices, down.data() + (o_idx - o_norm) + sizeof(uint64_t) * shot_off[k],
+                        sizeof(uint64_t) * items[k].shots);
+    }
+}
+
+}  // namespace aqs_hip
 
 extern "C" {
 

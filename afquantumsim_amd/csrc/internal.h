@@ -167,6 +167,16 @@ void validate_op(const aqs_op& op, int nqubits);
 double reduce_norm2(aqs_state s);
 void scale_state(aqs_state s, double factor);
 
+// Device bytes that queue_sample_norm needs as scratch for a complex64 state
+// of 2^n amplitudes (sampling tables and reduction partials).
+size_t sample_norm_scratch_bytes(int n);
+// Queues on `stream`, without waiting: `shots` indices sampled from the
+// complex64 state `amps` of 2^n amplitudes (uniforms du and indices dout on
+// the device; the exact fixed-point CDF of aqs_sample) and its FP64 norm^2
+// into *dnorm (device).
+void queue_sample_norm(const void* amps, int n, void* scratch, const double* du, uint32_t shots, uint64_t* dout,
+                       double* dnorm, hipStream_t stream);
+
 // Streaming (non-temporal) 8- and 16-byte accesses: a gate pass touches each
 // amplitude once, so the lines are not kept in the caches.
 typedef float aqs_f2v __attribute__((ext_vector_type(2)));

@@ -480,6 +480,25 @@ namespace aqs_hip {
 
 double reduce_norm2(aqs_state s) { return reduce_mode<0>(s, 0); }
 
+size_t sample_norm_scratch_bytes(int n) {
+    const uint64_t N = 1ull << n;
+    const uint64_t nb = (N + kSampleBlock - 1) / kSampleBlock;
+    return sample_table_bytes(nb) + sizeof(double) * red_grid(N);
+}
+
+void queue_sample_norm(const void* amps, int n, void* scratch, const double* du, uint32_t shots, uint64_t* dout,
+                       double* dnorm, hipStream_t stream) {
+    const uint64_t N = 1ull << n;
+    const uint64_t nb = (N + kSampleBlock - 1) / kSampleBlock;
+    double* partial = (double*)((char*)scratch + sample_table_bytes(nb));
+    const unsigned g = red_grid(N);
+    hipLaunchKernelGGL((k_reduce<float2, 0>), dim3(g), dim3(kBlock), 0, stream, (const float2*)amps, N, 0ull, n,
+                       partial);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlock), 0, stream, (const double*)partial, (int)g, 1, dnorm);
+    AQS_HIP(hipGetLastError());
+    if (shots) launch_sampling(amps, AQS_C64, N, sample_tables(scratch, nb), du, shots, dout, stream);
+}
+
 void scale_state(aqs_state s, double factor) {
     const unsigned g = red_grid(s->N) * 4;
     ProfScope ps(s, 0);

@@ -8,11 +8,18 @@
 // width), then |0..0>, the fused passes and sampling on its stream.  Gate
 // kernels are deterministic, so a request's result does not depend on what
 // else runs beside it.
+//
+// Requests of at most 16 qubits with no MATK op are coalesced: a worker runs
+// a group of them as one batched launch (run_group: each state in LDS tiles,
+// no host fusion), then samples each state, inside a group arena of
+// kArenaBytes that stays resident in the state pool.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <limits>
 #include <memory>
 
+#include "batch.h"
 #include "internal.h"
 #include "serve_core.h"
 
@@ -29,6 +36,16 @@ size_t state_footprint(int n) {
     const size_t blocks = (size_t)((N + 1023) / 1024);
     return (size_t)(N * 8) + std::max<size_t>(65536, (blocks + 1024) * 32) + 65536;
 }
+
+// Device bytes of a group arena: 64 states of 16 qubits with 1500-gate
+// circuits and 1000 shots each fit.
+constexpr size_t kArenaBytes = 96ull << 20;
+
+// What the state pool holds for the backend: a request's state or a group arena.
+struct Resident {
+    aqs_state state = nullptr;
+    void* arena = nullptr;
+};
 
 class HipBackend final : public aqs_serve::Backend {
    public:
@@ -47,18 +64,58 @@ class HipBackend final : public aqs_serve::Backend {
     ~HipBackend() override { release_streams(); }
 
     void worker_start(int) override { (void)hipSetDevice(device_); }
-    size_t footprint(int n) const override { return state_footprint(n); }
+    size_t footprint(int n) const override { return n == aqs_serve::kGroupKey ? kArenaBytes : state_footprint(n); }
 
     void* create_state(int n) override {
-        aqs_state st = nullptr;
-        const aqs_status rc = aqs_state_create(n, AQS_C64, &st);
+        auto res = std::make_unique<Resident>();
+        if (n == aqs_serve::kGroupKey) {
+            const hipError_t e = hipMalloc(&res->arena, kArenaBytes);
+            if (e != hipSuccess) throw ServeError{AQS_ERR_NO_MEMORY, "group arena allocation failed"};
+            return res.release();
+        }
+        const aqs_status rc = aqs_state_create(n, AQS_C64, &res->state);
         if (rc != AQS_OK) throw ServeError{rc, aqs_last_error()};
-        return st;
+        return res.release();
     }
-    void destroy_state(void* st) override { aqs_state_destroy((aqs_state)st); }
+    void destroy_state(void* p) override {
+        std::unique_ptr<Resident> res((Resident*)p);
+        if (res->state) aqs_state_destroy(res->state);
+        if (res->arena) (void)hipFree(res->arena);
+    }
+
+    int group_max_qubits() const override { return kGroupMaxQubits; }
+    size_t group_bytes(const Request& r) const override {
+        for (const auto& op : r.ops)
+            if (op.kind == AQS_OP_MATK) return std::numeric_limits<size_t>::max();
+        return group_item_bytes(r.n, r.ops.size(), r.shots);
+    }
+
+    void run_group(std::vector<Request*>& rs, void* arena, int worker) override {
+        std::vector<GroupItem> items(rs.size());
+        for (size_t k = 0; k < rs.size(); ++k) {
+            Request& r = *rs[k];
+            r.indices.resize(r.shots);
+            if (r.keep_state) r.state.resize(2ull << r.n);
+            GroupItem& it = items[k];
+            it.n = r.n;
+            it.ops = r.ops.data();
+            it.nops = r.ops.size();
+            it.u = r.u.data();
+            it.shots = r.shots;
+            it.indices = r.indices.data();
+            it.norm2 = &r.norm2;
+            it.state = r.keep_state ? r.state.data() : nullptr;
+        }
+        const aqs_status rc = guard([&] {
+            aqs_hip::run_group(items.data(), (int)items.size(), ((Resident*)arena)->arena, kArenaBytes,
+                               streams_[(size_t)worker]);
+        });
+        if (rc != AQS_OK) throw ServeError{rc, aqs_last_error()};
+        for (size_t k = 0; k < rs.size(); ++k) rs[k]->passes = items[k].passes;
+    }
 
     void run(Request& r, void* state, int worker) override {
-        aqs_state st = (aqs_state)state;
+        aqs_state st = ((Resident*)state)->state;
         auto ck = [](aqs_status rc) {
             if (rc != AQS_OK) throw ServeError{rc, aqs_last_error()};
         };
@@ -146,6 +203,16 @@ aqs_status aqs_server_set_starvation_limit(aqs_server s, double seconds) {
     return guard([&] { check_server(s)->core->set_starvation_limit(seconds); });
 }
 
+aqs_status aqs_server_set_grouping(aqs_server s, int max_qubits, int max_requests) {
+    return guard([&] {
+        check_server(s);
+        if (max_qubits < 0 || max_qubits > kGroupMaxQubits)
+            fail(AQS_ERR_INVALID_ARGUMENT, "group qubit limit must be 0..16");
+        if (max_requests < 1 || max_requests > 256) fail(AQS_ERR_INVALID_ARGUMENT, "group size must be 1..256");
+        serve_call([&] { s->core->set_grouping(max_qubits, max_requests); });
+    });
+}
+
 aqs_status aqs_server_submit(aqs_server s, uint64_t id, int nqubits, const aqs_op* ops, size_t nops,
                              const double* uniforms, uint32_t shots, int keep_state) {
     return guard([&] {
@@ -222,6 +289,8 @@ aqs_status aqs_server_stats(aqs_server s, aqs_server_stats_t* out) {
         out->peak_resident_bytes = st.peak_resident_bytes;
         out->budget_bytes = st.budget_bytes;
         out->cancelled = st.cancelled;
+        out->groups = st.groups;
+        out->grouped = st.grouped;
     });
 }
 

@@ -15,7 +15,13 @@ Server::Server(std::unique_ptr<Backend> backend, size_t budget_bytes, int worker
     if (!backend_) throw ServeError{AQS_ERR_INVALID_ARGUMENT, "null backend"};
     if (workers < 1 || workers > 16) throw ServeError{AQS_ERR_INVALID_ARGUMENT, "workers must be 1..16"};
     st_.budget_bytes = budget_bytes;
-    slot_.assign((size_t)workers, nullptr);
+    slot_.assign((size_t)workers, {});
+    slot_group_.assign((size_t)workers, false);
+    slot_fp_.assign((size_t)workers, 0);
+    if (backend_->group_max_qubits() > 0) {
+        arena_bytes_ = backend_->footprint(kGroupKey);
+        if (arena_bytes_ <= budget_) group_qubits_ = backend_->group_max_qubits();
+    }
     try {
         for (int w = 0; w < workers; ++w) workers_.emplace_back(&Server::worker_main, this, w);
     } catch (...) {
@@ -46,9 +52,19 @@ void Server::set_starvation_limit(double seconds) {
     starvation_s_ = seconds;
 }
 
+void Server::set_grouping(int max_qubits, int max_requests) {
+    if (max_qubits < 0 || max_requests < 1)
+        throw ServeError{AQS_ERR_INVALID_ARGUMENT, "group limits: qubits >= 0, requests >= 1"};
+    std::lock_guard<std::mutex> lock(mu_);
+    const int cap = arena_bytes_ <= budget_ ? backend_->group_max_qubits() : 0;
+    group_qubits_ = std::min(max_qubits, cap);
+    group_requests_ = max_requests;
+}
+
 void Server::submit(std::shared_ptr<Request> r) {
     if (!r) throw ServeError{AQS_ERR_INVALID_ARGUMENT, "null request"};
     r->footprint = backend_->footprint(r->n);
+    r->group_cost = backend_->group_max_qubits() > 0 ? backend_->group_bytes(*r) : 0;
     r->t_submit = now_s();
     std::lock_guard<std::mutex> lock(mu_);
     if (r->footprint > budget_) {
@@ -64,35 +80,62 @@ void Server::submit(std::shared_ptr<Request> r) {
     dispatch_locked();
 }
 
+// A request that may run in a group (called with mu_ held).
+bool Server::groupable_locked(const Request& r) const {
+    return group_qubits_ > 0 && r.n <= group_qubits_ && r.group_cost <= arena_bytes_;
+}
+
 // Assign pending requests to idle workers (called with mu_ held).
 void Server::dispatch_locked() {
     const double t = now_s();
-    for (auto it = pending_.begin(); it != pending_.end();) {
+    size_t i = 0;
+    while (i < pending_.size()) {
         int w = -1;
-        for (size_t i = 0; i < slot_.size(); ++i)
-            if (!slot_[i]) {
-                w = (int)i;
+        for (size_t k = 0; k < slot_.size(); ++k)
+            if (slot_[k].empty()) {
+                w = (int)k;
                 break;
             }
         if (w < 0) return;
-        Request& r = **it;
-        // Running requests hold at most their own footprints; every other
-        // resident state is idle and can be evicted (acquire_state), so the
-        // running footprints alone decide whether this one fits.
-        const bool fits = in_use_ + r.footprint <= budget_;
-        if (!fits) {
+        Request& r = *pending_[i];
+        const bool group = groupable_locked(r);
+        // Running requests hold at most their own footprints (a group: the
+        // arena); every other resident state is idle and can be evicted
+        // (acquire_state), so the running footprints alone decide whether
+        // this one fits.
+        const size_t fp = group ? arena_bytes_ : r.footprint;
+        if (in_use_ + fp > budget_) {
             // an old request holds its place: nothing behind it may start
             if (t - r.t_submit > starvation_s_) return;
-            ++it;
+            ++i;
             continue;
         }
-        slot_[(size_t)w] = *it;
-        in_use_ += r.footprint;
-        r.t_admit = t;
-        ++st_.admitted;
-        ++st_.running;
+        std::vector<std::shared_ptr<Request>> job{pending_[i]};
+        pending_.erase(pending_.begin() + (std::ptrdiff_t)i);
+        if (group) {
+            size_t bytes = r.group_cost;
+            for (size_t j = i; j < pending_.size() && job.size() < (size_t)group_requests_;) {
+                Request& q = *pending_[j];
+                if (groupable_locked(q) && bytes + q.group_cost <= arena_bytes_) {
+                    bytes += q.group_cost;
+                    job.push_back(pending_[j]);
+                    pending_.erase(pending_.begin() + (std::ptrdiff_t)j);
+                    continue;
+                }
+                if (t - q.t_submit > starvation_s_) break;  // never overtake a starved request
+                ++j;
+            }
+        }
+        for (auto& q : job) {
+            q->t_admit = t;
+            ++st_.admitted;
+            ++st_.running;
+        }
         st_.max_concurrent = std::max(st_.max_concurrent, st_.running);
-        it = pending_.erase(it);
+        slot_[(size_t)w] = std::move(job);
+        slot_group_[(size_t)w] = group;
+        slot_fp_[(size_t)w] = fp;
+        in_use_ += fp;
         cv_work_.notify_all();
     }
 }
@@ -145,43 +188,106 @@ void Server::finish_locked(Request& r) {
     r.done = true;
 }
 
+namespace {
+
+// The failure of a request, from whatever the backend raised.
+template <typename F>
+void run_guarded(Request& r, F&& f) {
+    try {
+        f();
+    } catch (const ServeError& e) {
+        r.status = e.code;
+        r.error = e.msg;
+    } catch (const std::bad_alloc&) {
+        r.status = AQS_ERR_NO_MEMORY;
+        r.error = "host allocation failed";
+    } catch (const std::exception& e) {
+        r.status = AQS_ERR_RUNTIME;
+        r.error = e.what();
+    } catch (...) {
+        r.status = AQS_ERR_RUNTIME;
+        r.error = "unknown failure while running the request";
+    }
+}
+
+void mark_cancelled(Request& r) {
+    r.status = AQS_ERR_RUNTIME;
+    r.error = "request cancelled";
+}
+
+}  // namespace
+
+void Server::run_single(Request& r, int w) {
+    void* st = nullptr;
+    run_guarded(r, [&] {
+        if (r.cancel.load()) throw ServeError{AQS_ERR_RUNTIME, "request cancelled"};
+        st = acquire_state(r.n);
+        backend_->run(r, st, w);
+    });
+    if (st) release_state(r.n, st);
+}
+
+// Members cancelled before the launch are left out; one cancelled during it
+// completes as cancelled and its results are dropped.
+void Server::run_grouped(std::vector<std::shared_ptr<Request>>& job, int w) {
+    std::vector<Request*> live;
+    for (auto& q : job) {
+        if (q->cancel.load())
+            mark_cancelled(*q);
+        else
+            live.push_back(q.get());
+    }
+    if (live.empty()) return;
+    void* arena = nullptr;
+    Request failed;
+    run_guarded(failed, [&] {
+        arena = acquire_state(kGroupKey);
+        backend_->run_group(live, arena, w);
+    });
+    if (arena) release_state(kGroupKey, arena);
+    for (Request* q : live) {
+        if (failed.status != AQS_OK) {
+            q->status = failed.status;
+            q->error = failed.error;
+        } else if (q->cancel.load()) {
+            mark_cancelled(*q);
+            q->indices.clear();
+            q->state.clear();
+        }
+    }
+}
+
 void Server::worker_main(int w) {
     backend_->worker_start(w);
     for (;;) {
-        std::shared_ptr<Request> r;
+        std::vector<std::shared_ptr<Request>> job;
+        bool group = false;
         {
             std::unique_lock<std::mutex> lock(mu_);
-            cv_work_.wait(lock, [&] { return stopping_ || slot_[(size_t)w]; });
-            if (!slot_[(size_t)w]) return;  // stopping with nothing assigned
-            r = slot_[(size_t)w];
+            cv_work_.wait(lock, [&] { return stopping_ || !slot_[(size_t)w].empty(); });
+            if (slot_[(size_t)w].empty()) return;  // stopping with nothing assigned
+            job = slot_[(size_t)w];
+            group = slot_group_[(size_t)w];
         }
-        void* st = nullptr;
-        try {
-            if (r->cancel.load()) throw ServeError{AQS_ERR_RUNTIME, "request cancelled"};
-            st = acquire_state(r->n);
-            backend_->run(*r, st, w);
-        } catch (const ServeError& e) {
-            r->status = e.code;
-            r->error = e.msg;
-        } catch (const std::bad_alloc&) {
-            r->status = AQS_ERR_NO_MEMORY;
-            r->error = "host allocation failed";
-        } catch (const std::exception& e) {
-            r->status = AQS_ERR_RUNTIME;
-            r->error = e.what();
-        } catch (...) {
-            r->status = AQS_ERR_RUNTIME;
-            r->error = "unknown failure while running the request";
-        }
-        if (st) release_state(r->n, st);
+        if (group)
+            run_grouped(job, w);
+        else
+            run_single(*job[0], w);
         {
             std::lock_guard<std::mutex> lock(mu_);
-            finish_locked(*r);
-            if (r->status == AQS_ERR_RUNTIME && r->cancel.load()) ++st_.cancelled;
-            slot_[(size_t)w] = nullptr;
-            in_use_ -= r->footprint;
-            --st_.running;
-            ++st_.completed;
+            for (auto& q : job) {
+                finish_locked(*q);
+                if (q->status == AQS_ERR_RUNTIME && q->cancel.load()) ++st_.cancelled;
+                --st_.running;
+                ++st_.completed;
+            }
+            if (group) {
+                ++st_.groups;
+                st_.grouped += job.size();
+            }
+            slot_[(size_t)w].clear();
+            in_use_ -= slot_fp_[(size_t)w];
+            slot_fp_[(size_t)w] = 0;
             dispatch_locked();
         }
         cv_done_.notify_all();

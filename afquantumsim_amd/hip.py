@@ -87,7 +87,7 @@ EXPORTS = [
     "aqs_batch_create", "aqs_batch_autotune", "aqs_batch_configure", "aqs_batch_run", "aqs_batch_state", "aqs_batch_info",
     "aqs_batch_destroy", "aqs_autotune", "aqs_tune_set", "aqs_server_create", "aqs_server_set_starvation_limit",
     "aqs_server_submit", "aqs_server_wait", "aqs_server_stats", "aqs_server_destroy", "aqs_server_cancel",
-    "aqs_sample_noalloc",
+    "aqs_sample_noalloc", "aqs_server_set_grouping",
 ]
 
 
@@ -103,7 +103,7 @@ class ServerStats(ctypes.Structure):
                 ("evictions", ctypes.c_uint64), ("pending", ctypes.c_uint64), ("running", ctypes.c_int32),
                 ("max_concurrent", ctypes.c_int32), ("resident_bytes", ctypes.c_uint64),
                 ("peak_resident_bytes", ctypes.c_uint64), ("budget_bytes", ctypes.c_uint64),
-                ("cancelled", ctypes.c_uint64)]
+                ("cancelled", ctypes.c_uint64), ("groups", ctypes.c_uint64), ("grouped", ctypes.c_uint64)]
 
 _lib = None
 
@@ -170,6 +170,7 @@ def load(path: str = LIB_PATH):
         "aqs_tune_set": [vp, i32, i32],
         "aqs_server_create": [sz, i32, i32, ctypes.POINTER(vp)],
         "aqs_server_set_starvation_limit": [vp, ctypes.c_double],
+        "aqs_server_set_grouping": [vp, i32, i32],
         "aqs_server_submit": [vp, u64, i32, ctypes.POINTER(Op), sz, dp, ctypes.c_uint32, i32],
         "aqs_server_wait": [vp, u64, i32, ctypes.POINTER(RequestResult), ctypes.POINTER(ctypes.c_uint64), vp],
         "aqs_server_stats": [vp, ctypes.POINTER(ServerStats)],
@@ -268,9 +269,8 @@ class State:
 
     # gates
     def apply(self, ops):
-        arr, keep = pack_ops(ops)
-        check(_lib.aqs_apply(self.h, arr, len(ops)))
-        del keep
+        p = as_packed(ops)
+        check(_lib.aqs_apply(self.h, p.arr, p.count))
 
     def autotune(self, reps: int = 3):
         """Pick the fastest one-qubit launch shape per target qubit (state unchanged)."""
@@ -371,19 +371,19 @@ class Schedule:
 
     def __init__(self, ops, n: int, max_qubits: int = 5, mode: str = "blocks", tile_qubits: int = 0):
         L = load()
-        arr, keep = pack_ops(ops)
+        p = as_packed(ops)
+        arr = p.arr
         self.n = n
         self.h = None
         if mode not in ("blocks", "tiled", "auto"):
             raise ValueError(f"unknown schedule mode {mode!r}")
         h = ctypes.c_void_p()
         if mode == "blocks":
-            check(L.aqs_fuse(arr, len(ops), n, max_qubits, ctypes.byref(h)))
+            check(L.aqs_fuse(arr, p.count, n, max_qubits, ctypes.byref(h)))
         elif mode == "tiled":
-            check(L.aqs_fuse_tiled(arr, len(ops), n, tile_qubits, ctypes.byref(h)))
+            check(L.aqs_fuse_tiled(arr, p.count, n, tile_qubits, ctypes.byref(h)))
         else:
-            check(L.aqs_fuse_auto(arr, len(ops), n, ctypes.byref(h)))
-        del keep
+            check(L.aqs_fuse_auto(arr, p.count, n, ctypes.byref(h)))
         self.h = h
         c = ctypes.c_double()
         check(L.aqs_schedule_cost(h, ctypes.byref(c)))
@@ -496,11 +496,15 @@ class Server:
     def set_starvation_limit(self, seconds: float):
         check(_lib.aqs_server_set_starvation_limit(self.h, seconds))
 
+    def set_grouping(self, max_qubits: int, max_requests: int = 64):
+        """Coalesce requests of at most max_qubits qubits (0: off), at most
+        max_requests per batched launch."""
+        check(_lib.aqs_server_set_grouping(self.h, max_qubits, max_requests))
+
     def submit(self, rid: int, n: int, ops, uniforms, keep_state: bool = False):
-        arr, keep = pack_ops(ops)
+        p = as_packed(ops)
         u = np.ascontiguousarray(uniforms, dtype=np.float64)
-        check(_lib.aqs_server_submit(self.h, rid, n, arr, len(ops), _dp(u), len(u), int(keep_state)))
-        del keep
+        check(_lib.aqs_server_submit(self.h, rid, n, p.arr, p.count, _dp(u), len(u), int(keep_state)))
         self._shots[rid] = len(u)
         self._n[rid] = n
         self._keep[rid] = keep_state
@@ -541,6 +545,22 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+class PackedOps:
+    """An op list packed once (pack_ops), for repeated or timed submission;
+    State.apply, Server.submit and Schedule take it in place of the list."""
+
+    def __init__(self, ops):
+        self.arr, self.keep = pack_ops(ops)
+        self.count = len(ops)
+
+    def __len__(self):
+        return self.count
+
+
+def as_packed(ops) -> PackedOps:
+    return ops if isinstance(ops, PackedOps) else PackedOps(ops)
 
 
 def pack_ops(ops):

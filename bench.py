@@ -369,19 +369,22 @@ def served_requests(count, seed):
         n = int(rng.choice(ns, p=p))
         depth = int(rng.integers(10, 101))
         gates = circuits.brickwork(n, depth, seed=50_000 + i)
-        out.append({"id": i, "n": n, "depth": depth, "ops": ops.lower_all(gates), "gates": gates,
+        lowered = ops.lower_all(gates)
+        out.append({"id": i, "n": n, "depth": depth, "ops": lowered, "packed": hip.PackedOps(lowered), "gates": gates,
                     "u": rng.uniform(size=1000)})
     return out
 
 
-def serve_stream(reqs, rate, budget, workers, seed):
-    """Poisson arrivals at `rate` per second into a fresh server; returns
-    per-request latency (arrival -> sampled indices) and queueing, wall time
-    and server statistics."""
+def serve_stream(reqs, rate, budget, workers, seed, coalesce=True):
+    """Poisson arrivals at `rate` per second into a fresh server (requests
+    packed beforehand); returns per-request latency (arrival -> sampled
+    indices) and queueing, wall time and server statistics."""
     import threading
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(reqs)))
     srv = hip.Server(budget, workers=workers)
+    if not coalesce:
+        srv.set_grouping(0)
     lat = {}
     submitted = threading.Semaphore(0)
 
@@ -398,7 +401,7 @@ def serve_stream(reqs, rate, budget, workers, seed):
         delay = start + at - time.perf_counter()
         if delay > 0:
             time.sleep(delay)
-        srv.submit(r["id"], r["n"], r["ops"], r["u"])
+        srv.submit(r["id"], r["n"], r["packed"], r["u"])
         submitted.release()
     th.join()
     wall = time.perf_counter() - start
@@ -410,16 +413,19 @@ def serve_stream(reqs, rate, budget, workers, seed):
     return {"arrival_rate_per_s": round(rate, 2), "p50_ms": round(1e3 * float(np.percentile(l, 50)), 2),
             "p99_ms": round(1e3 * float(np.percentile(l, 99)), 2), "mean_queue_ms": round(1e3 * float(q.mean()), 2),
             "throughput_req_per_s": round(len(reqs) / wall, 2), "gate_apps_per_s": round(gates / wall, 1),
-            "max_concurrent": stats["max_concurrent"], "peak_resident_bytes": stats["peak_resident_bytes"]}
+            "max_concurrent": stats["max_concurrent"], "peak_resident_bytes": stats["peak_resident_bytes"],
+            "coalesced_launches": stats["groups"], "coalesced_requests": stats["grouped"]}
 
 
 def served(args):
     """Poisson arrivals of mixed-size brickwork requests into the request
-    server (4 workers, compile() fusion per request): p50/p99 latency
-    (arrival -> sampled indices) and throughput at arrival rates from 0.7x
-    one worker's capacity up to and past the saturation of all four; the
-    same requests run one at a time with the plain per-gate kernels as the
-    comparator; infidelity of served results against complex128."""
+    server (4 workers; compile() fusion per request above 16 qubits, queued
+    requests of at most 16 qubits coalesced into batched launches): p50/p99
+    latency (arrival -> sampled indices) and throughput at arrival rates from
+    0.7x one worker's capacity up to and past the saturation of all four, and
+    at the two highest rates with coalescing off; the same requests run one
+    at a time with the plain per-gate kernels as the comparator; infidelity
+    of served results against complex128."""
     budget = 64 << 30
     workers = 4
     # service-time calibration: requests alone, one after another
@@ -427,7 +433,7 @@ def served(args):
     srv = hip.Server(budget, workers=1)
     t0 = time.perf_counter()
     for r in calib:
-        srv.submit(r["id"], r["n"], r["ops"], r["u"])
+        srv.submit(r["id"], r["n"], r["packed"], r["u"])
         srv.wait(r["id"])
     alone_s = (time.perf_counter() - t0) / len(calib)
     srv.close()
@@ -438,6 +444,12 @@ def served(args):
         entry["rate_vs_one_worker_capacity"] = mult
         sweep.append(entry)
         log(f"served at {mult}x one worker's capacity: {entry}")
+    no_coalescing = []
+    for mult in (4.0, 8.0):
+        entry = serve_stream(reqs, mult / alone_s, budget, workers, seed=79, coalesce=False)
+        entry["rate_vs_one_worker_capacity"] = mult
+        no_coalescing.append(entry)
+        log(f"served at {mult}x, coalescing off: {entry}")
     main = sweep[0]
     # plain comparator: the calibration requests one at a time with the
     # per-gate kernels (no fusion, no concurrency), same uniforms; the fused
@@ -451,7 +463,7 @@ def served(args):
         if st is None:
             st = st_cache[r["n"]] = hip.State(r["n"], hip.C64)
         st.set_basis(0)
-        st.apply(r["ops"])
+        st.apply(r["packed"])
         st.sample(r["u"])
     plain_s = time.perf_counter() - t0
     for st in st_cache.values():
@@ -464,7 +476,7 @@ def served(args):
     checked = 0
     chk = hip.Server(budget, workers=2)
     for r in [x for x in reqs if x["n"] <= 24][:16]:
-        chk.submit(10_000 + r["id"], r["n"], r["ops"], r["u"][:10], keep_state=True)
+        chk.submit(10_000 + r["id"], r["n"], r["packed"], r["u"][:10], keep_state=True)
         _, _, psi = chk.wait(10_000 + r["id"])
         ref = hip.State(r["n"], hip.C128)
         ref.apply(ops.lower_all(r["gates"], exact=True))
@@ -473,7 +485,7 @@ def served(args):
         worst = max(worst, 1.0 - abs(ov) ** 2 / (np.vdot(psi, psi).real * np.vdot(x, x).real))
         st = hip.State(r["n"], hip.C64)
         st.set_basis(0)
-        st.apply(r["ops"])
+        st.apply(r["packed"])
         ovp = st.inner(ref)
         worst_plain = max(worst_plain, 1.0 - abs(ovp) ** 2 / (st.norm2() * ref.norm2()))
         st.close()
@@ -485,6 +497,7 @@ def served(args):
            "mean_queue_ms": main["mean_queue_ms"], "throughput_req_per_s": main["throughput_req_per_s"],
            "gate_apps_per_s": main["gate_apps_per_s"], "max_concurrent": main["max_concurrent"],
            "alone_mean_service_ms": round(1e3 * alone_s, 2), "rate_sweep": sweep,
+           "rate_sweep_coalescing_off": no_coalescing,
            "plain_per_gate_one_at_a_time": {
                "requests": len(plain), "mean_service_ms": round(1e3 * plain_s / len(plain), 2),
                "throughput_req_per_s": round(len(plain) / plain_s, 2),

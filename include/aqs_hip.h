@@ -295,6 +295,8 @@ typedef struct aqs_server_stats_t {
     int32_t running, max_concurrent;
     uint64_t resident_bytes, peak_resident_bytes, budget_bytes;
     uint64_t cancelled;  /* requests that ended through aqs_server_cancel */
+    uint64_t groups;     /* coalesced launches (aqs_server_set_grouping) */
+    uint64_t grouped;    /* requests that ran in them */
 } aqs_server_stats_t;
 
 /* fuse_qubits: 1..5 = aqs_fuse blocks of that width, 0 = aqs_fuse_auto. */
@@ -302,6 +304,14 @@ aqs_status aqs_server_create(size_t hbm_budget_bytes, int workers, int fuse_qubi
 /* A request waiting longer than this blocks later requests from jumping
  * ahead of it (default 0.25 s). */
 aqs_status aqs_server_set_starvation_limit(aqs_server s, double seconds);
+/* Coalescing of small requests (default on: 16 qubits, 64 requests).  A
+ * worker that takes a request of at most max_qubits qubits (0..16; 0 turns
+ * coalescing off) and no MATK op also takes the later queued requests of that
+ * kind, up to max_requests (1..256) and a 96 MiB device arena, and runs them
+ * as one batched launch (every state in LDS tiles, no host fusion), then
+ * samples each state as aqs_sample does.  The arena counts against the HBM
+ * budget while resident; coalescing is off when it does not fit. */
+aqs_status aqs_server_set_grouping(aqs_server s, int max_qubits, int max_requests);
 /* Queue a circuit (ops and uniforms are copied).  Fails with
  * AQS_ERR_INVALID_ARGUMENT when the request can never fit the budget. */
 aqs_status aqs_server_submit(aqs_server s, uint64_t id, int nqubits, const aqs_op* ops, size_t nops,

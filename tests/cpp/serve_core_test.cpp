@@ -44,7 +44,7 @@ class FakeBackend final : public Backend {
     ~FakeBackend() override {
         if (on_destroy_) on_destroy_(created_.load(), destroyed_.load(), violation_.load());
     }
-    size_t footprint(int n) const override { return (size_t)1 << n; }
+    size_t footprint(int n) const override { return n == kGroupKey ? arena_ : (size_t)1 << n; }
     void* create_state(int n) override {
         const size_t b = footprint(n);
         const size_t now = live_bytes_.fetch_add(b) + b;
@@ -87,6 +87,55 @@ class FakeBackend final : public Backend {
         r.norm2 = 1.0;
         r.passes = r.ops.size();
     }
+
+    // Groups: members of at most group_qubits_ qubits, each taking item_bytes_
+    // of an arena of arena_ bytes; "running" a group sleeps for the longest
+    // member's time.  Every group's member ids are recorded in launch order.
+    int group_max_qubits() const override { return group_qubits_; }
+    size_t group_bytes(const Request& r) const override { return r.n == unfit_qubits_ ? arena_ + 1 : item_bytes_; }
+    void run_group(std::vector<Request*>& rs, void* arena, int) override {
+        if (*(int*)arena != kGroupKey) violation_ = true;
+        size_t bytes = 0;
+        double ms = 0.0;
+        for (Request* r : rs) {
+            if (r->n > group_qubits_) violation_ = true;
+            bytes += item_bytes_;
+            ms = std::max(ms, r->u.empty() ? 0.0 : r->u[0]);
+        }
+        if (bytes > arena_) violation_ = true;
+        {
+            std::lock_guard<std::mutex> lock(groups_mu_);
+            std::vector<uint64_t> ids;
+            for (Request* r : rs) ids.push_back(r->id);
+            groups_.push_back(ids);
+        }
+        const int now = running_.fetch_add((int)rs.size()) + (int)rs.size();
+        const size_t held = running_bytes_.fetch_add(arena_) + arena_;
+        int prev = max_running_.load();
+        while (now > prev && !max_running_.compare_exchange_weak(prev, now)) {
+        }
+        if (held > budget_) violation_ = true;
+        std::this_thread::sleep_for(std::chrono::microseconds((long)(ms * 1000)));
+        running_bytes_.fetch_sub(arena_);
+        running_.fetch_sub((int)rs.size());
+        for (Request* r : rs)
+            if (r->n == fail_qubits_) throw std::runtime_error("group failure");
+        for (Request* r : rs) {
+            r->indices.resize(r->shots);
+            for (uint32_t k = 0; k < r->shots; ++k) r->indices[k] = (uint64_t)(r->u[k] * 1000.0) + r->id;
+            r->norm2 = 1.0;
+            r->passes = 1;
+        }
+    }
+    std::vector<std::vector<uint64_t>> groups() {
+        std::lock_guard<std::mutex> lock(groups_mu_);
+        return groups_;
+    }
+
+    int group_qubits_ = 0, unfit_qubits_ = -1;
+    size_t arena_ = 0, item_bytes_ = 0;
+    std::mutex groups_mu_;
+    std::vector<std::vector<uint64_t>> groups_;
 
     size_t budget_;
     std::atomic<size_t> live_bytes_{0}, running_bytes_{0};
@@ -333,6 +382,170 @@ void concurrent() {
     require(!be->violation_, "device-side invariants");
 }
 
+FakeBackend* group_backend(size_t budget, int qubits, size_t arena, size_t item) {
+    auto* be = new FakeBackend(budget);
+    be->group_qubits_ = qubits;
+    be->arena_ = arena;
+    be->item_bytes_ = item;
+    return be;
+}
+
+size_t group_of(const std::vector<std::vector<uint64_t>>& groups, uint64_t id) {
+    for (size_t g = 0; g < groups.size(); ++g)
+        if (std::find(groups[g].begin(), groups[g].end(), id) != groups[g].end()) return g;
+    return (size_t)-1;
+}
+
+// Small requests queued behind a busy worker run as groups: in arrival
+// order, at most the request cap and the arena's bytes per group; larger
+// requests, and small ones the backend cannot group, run alone.
+void grouping() {
+    g_case = "grouping";
+    auto* be = group_backend(1 << 20, 9, 4096, 1000);  // four members fill the arena
+    be->unfit_qubits_ = 7;
+    auto s_owner = std::make_unique<Server>(std::unique_ptr<Backend>(be), 1 << 20, 1);
+    Server& s = *s_owner;
+    s.submit(make(1, 12, 30.0));  // occupies the one worker
+    for (uint64_t id = 2; id < 12; ++id) s.submit(make(id, id == 5 ? 11 : id == 8 ? 7 : 8, 1.0));
+    for (uint64_t id = 1; id < 12; ++id) {
+        auto r = s.wait(id, -1);
+        require(r->status == AQS_OK, "status");
+        require(r->indices.size() == 4 && r->indices[1] == 1 + id, "indices belong to the request");
+    }
+    const auto groups = be->groups();
+    // 8 groupable requests (2,3,4,6,7,9,10,11) in groups of at most 4
+    require(groups.size() == 2, "two groups");
+    require(groups[0] == std::vector<uint64_t>({2, 3, 4, 6}), "first group: arrival order, bytes cap");
+    require(groups[1] == std::vector<uint64_t>({7, 9, 10, 11}), "second group");
+    require(group_of(groups, 5) == (size_t)-1 && group_of(groups, 8) == (size_t)-1, "large and unfit run alone");
+    const Stats st = s.stats();
+    require(st.groups == 2 && st.grouped == 8, "group counts");
+    require(st.completed == 11 && st.admitted == 11, "counts");
+    require(!be->violation_, "device-side invariants");
+
+    g_case = "group request cap";
+    auto* be2 = group_backend(1 << 20, 9, 1 << 16, 1000);
+    auto s2_owner = std::make_unique<Server>(std::unique_ptr<Backend>(be2), 1 << 20, 1);
+    Server& s2 = *s2_owner;
+    s2.set_grouping(9, 3);
+    s2.submit(make(1, 12, 30.0));
+    for (uint64_t id = 2; id < 9; ++id) s2.submit(make(id, 8, 1.0));
+    for (uint64_t id = 1; id < 9; ++id) require(s2.wait(id, -1)->status == AQS_OK, "status");
+    const auto g2 = be2->groups();
+    require(g2.size() == 3 && g2[0].size() == 3 && g2[1].size() == 3 && g2[2].size() == 1, "at most 3 per group");
+
+    g_case = "grouping off";
+    auto* be3 = group_backend(1 << 20, 9, 1 << 16, 1000);
+    auto s3_owner = std::make_unique<Server>(std::unique_ptr<Backend>(be3), 1 << 20, 1);
+    Server& s3 = *s3_owner;
+    s3.set_grouping(0, 64);
+    s3.submit(make(1, 12, 10.0));
+    for (uint64_t id = 2; id < 6; ++id) s3.submit(make(id, 8, 1.0));
+    for (uint64_t id = 1; id < 6; ++id) require(s3.wait(id, -1)->status == AQS_OK, "status");
+    require(be3->groups().empty() && s3.stats().groups == 0, "no groups when off");
+
+    g_case = "arena over budget";
+    auto* be4 = group_backend(2048, 9, 4096, 100);  // the arena never fits: coalescing stays off
+    auto s4_owner = std::make_unique<Server>(std::unique_ptr<Backend>(be4), 2048, 1);
+    Server& s4 = *s4_owner;
+    s4.submit(make(1, 10, 10.0));
+    for (uint64_t id = 2; id < 5; ++id) s4.submit(make(id, 8, 1.0));
+    for (uint64_t id = 1; id < 5; ++id) require(s4.wait(id, -1)->status == AQS_OK, "status");
+    require(be4->groups().empty(), "no groups without room for the arena");
+    require(!be4->violation_, "device-side invariants");
+}
+
+// Collecting a group stops at a starved request: a later small request does
+// not start before it.
+void group_starvation() {
+    g_case = "group starvation";
+    const size_t arena = 1024;
+    const size_t budget = 4096 + 1024;  // the 12-qubit state plus one arena (or one 10-qubit state)
+    auto* be = group_backend(budget, 9, arena, 100);
+    auto s_owner = std::make_unique<Server>(std::unique_ptr<Backend>(be), budget, 2);
+    Server& s = *s_owner;
+    s.set_starvation_limit(0.02);
+    s.submit(make(1, 12, 100.0));  // worker 0, 4096 bytes
+    s.submit(make(2, 10, 60.0));   // worker 1, 1024 bytes: the budget is full
+    s.submit(make(3, 8, 1.0));     // small
+    s.submit(make(4, 12, 1.0));    // large: needs request 1's room
+    s.submit(make(5, 8, 1.0));     // small, behind the large one
+    auto r4 = s.wait(4, -1);
+    auto r5 = s.wait(5, -1);
+    auto r3 = s.wait(3, -1);
+    require(r3->status == AQS_OK && r4->status == AQS_OK && r5->status == AQS_OK, "status");
+    require(r5->t_admit >= r4->t_admit, "a group does not overtake a starved request");
+    require(group_of(be->groups(), 3) != group_of(be->groups(), 5), "not coalesced past it");
+    s.wait(1, -1);
+    s.wait(2, -1);
+    require(!be->violation_, "device-side invariants");
+}
+
+// A member cancelled while its group runs completes as cancelled; the other
+// members keep their results.  A failing group fails every member.
+void group_cancel_and_failure() {
+    g_case = "group cancel";
+    auto* be = group_backend(1 << 20, 9, 1 << 16, 100);
+    be->fail_qubits_ = 6;
+    auto s_owner = std::make_unique<Server>(std::unique_ptr<Backend>(be), 1 << 20, 1);
+    Server& s = *s_owner;
+    s.submit(make(1, 12, 20.0));
+    for (uint64_t id = 2; id < 5; ++id) s.submit(make(id, 8, 60.0));  // one group of three, 60 ms
+    std::this_thread::sleep_for(std::chrono::milliseconds(40));       // the group is running
+    require(s.cancel(3), "running member cancelled");
+    auto r3 = s.wait(3, -1);
+    require(r3->status == AQS_ERR_RUNTIME && r3->error == "request cancelled", "cancelled member");
+    require(s.wait(2, -1)->status == AQS_OK && s.wait(4, -1)->status == AQS_OK, "other members finish");
+    require(s.wait(1, -1)->status == AQS_OK, "first request");
+
+    g_case = "group failure";
+    s.submit(make(10, 12, 20.0));
+    s.submit(make(11, 8, 1.0));
+    s.submit(make(12, 6, 1.0));  // the backend fails any group holding a 6-qubit member
+    s.submit(make(13, 8, 1.0));
+    require(s.wait(10, -1)->status == AQS_OK, "alone");
+    for (uint64_t id = 11; id < 14; ++id) {
+        auto r = s.wait(id, -1);
+        require(r->status == AQS_ERR_RUNTIME && r->error == "group failure", "every member fails");
+    }
+    s.submit(make(14, 8, 1.0));
+    require(s.wait(14, -1)->status == AQS_OK, "server keeps working");
+    require(s.stats().cancelled == 1, "cancel count");
+    require(!be->violation_, "device-side invariants");
+}
+
+// The concurrent case with coalescing on: submits, double waits and cancels
+// from many threads while groups form.
+void concurrent_grouped() {
+    g_case = "concurrent grouped";
+    auto* be = group_backend(64 * 1024, 10, 8 * 1024, 1000);
+    auto s_owner = std::make_unique<Server>(std::unique_ptr<Backend>(be), 64 * 1024, 3);
+    Server& s = *s_owner;
+    s.set_starvation_limit(0.005);
+    std::atomic<int> ok{0}, cancelled{0}, errors{0};
+    std::vector<std::thread> threads;
+    for (int t = 0; t < 6; ++t)
+        threads.emplace_back([&, t] {
+            std::mt19937 rng(777 + t);
+            for (int k = 0; k < 40; ++k) {
+                const uint64_t id = (uint64_t)t * 1000 + k;
+                s.submit(make(id, 6 + (int)(rng() % 10), 0.2 + 0.1 * (rng() % 5)));
+                if (k % 7 == 3) s.cancel(id);
+                auto r = s.wait(id, -1);
+                if (r->status == AQS_OK) ok.fetch_add(1);
+                else if (r->error == "request cancelled") cancelled.fetch_add(1);
+                else errors.fetch_add(1);
+            }
+        });
+    for (auto& th : threads) th.join();
+    require(errors.load() == 0, "no unexpected outcome");
+    require(ok.load() + cancelled.load() == 6 * 40, "every request accounted for");
+    const Stats st = s.stats();
+    require(st.pending == 0 && st.running == 0, "drained");
+    require(st.groups > 0, "groups formed");
+    require(!be->violation_, "device-side invariants");
+}
+
 }  // namespace
 
 int main() {
@@ -343,6 +556,10 @@ int main() {
     cancellation();
     failures();
     concurrent();
+    grouping();
+    group_starvation();
+    group_cancel_and_failure();
+    concurrent_grouped();
     std::printf("%d checks, ALL PASSED\n", g_checks);
     return 0;
 }

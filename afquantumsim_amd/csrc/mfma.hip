@@ -339,6 +339,17 @@ __device__ __forceinline__ uint64_t permute_index(uint64_t v, const int* map, in
 // place (B is already read: every lane stores exactly the addresses it
 // loaded), then B's product into A's place.  The store gathers from LDS
 // through the tile-local permutation, so HBM writes stay coalesced runs.
+//
+// Which t a wave skips depends on t's low bits, so a plain grid stride (the
+// low bits fixed per wave) would leave some waves with nearly every pair and
+// others with none.  Wave b instead takes t = m G + (b ^ (m mod G)) in step
+// m (G = gridDim.x, a power of two): every t once, each wave seeing every
+// low-bit pattern.  The next pair's first tile is loaded while the current
+// pair's second tile is multiplied.
+__device__ __forceinline__ uint64_t perm_step_tile(uint64_t m, uint64_t G, uint64_t b) {
+    return m * G + (b ^ (m & (G - 1)));
+}
+
 template <int K, int DIAG>
 __global__ __launch_bounds__(64) void k_matk_mfma_perm(MfmaArgs a) {
     static_assert(DIAG == 0 || DIAG == 2, "the permuted store takes no epilogue");
@@ -367,14 +378,29 @@ __global__ __launch_bounds__(64) void k_matk_mfma_perm(MfmaArgs a) {
     const uint32_t wr_lane = a.tr[4 * half] | a.tc[col];
     __syncthreads();
 
-    for (uint64_t t = blockIdx.x; t < a.tiles; t += gridDim.x) {
+    const uint64_t G = gridDim.x, b = blockIdx.x, steps = a.tiles / G;
+    // the first step m >= from whose tile leads its pair (t <= beta(t))
+    auto next_lead = [&](uint64_t from) {
+        uint64_t m = from;
+        for (; m < steps; ++m) {
+            const uint64_t t = perm_step_tile(m, G, b);
+            if (permute_index(t, a.perm.rmap, a.perm.nr) >= t) break;
+        }
+        return m;
+    };
+    uint64_t m = next_lead(0);
+    float4 pf[CHUNKS];
+    if (m < steps) {
+        const uint64_t b4 = tile_base(perm_step_tile(m, G, b), a) >> 1;
+#pragma unroll
+        for (int j = 0; j < CHUNKS; ++j) pf[j] = ld_nt(&a.sv4[b4 + a.gj[j] + glane]);
+    }
+    while (m < steps) {
+        const uint64_t t = perm_step_tile(m, G, b);
         const uint64_t t2 = permute_index(t, a.perm.rmap, a.perm.nr);
-        if (t2 < t) continue;  // the wave that draws t2 handles this pair
         const int ntiles = t2 == t ? 1 : 2;
         const uint64_t b4a = tile_base(t, a) >> 1, b4b = tile_base(t2, a) >> 1;
-        float4 pf[CHUNKS];
-#pragma unroll
-        for (int j = 0; j < CHUNKS; ++j) pf[j] = ld_nt(&a.sv4[b4a + a.gj[j] + glane]);
+        const uint64_t mn = next_lead(m + 1);
         for (int s = 0; s < ntiles; ++s) {
             const uint64_t src = s ? b4b : b4a;
             if (EPI) {
@@ -388,9 +414,15 @@ __global__ __launch_bounds__(64) void k_matk_mfma_perm(MfmaArgs a) {
                 *reinterpret_cast<float4*>(&lds[padded(tau)]) = pf[j];
             }
             __syncthreads();
-            if (s + 1 < ntiles) {  // the partner tile, read before this tile's result lands on it
+            if (s + 1 < ntiles) {
+                // the partner tile, read before this tile's result lands on it
 #pragma unroll
                 for (int j = 0; j < CHUNKS; ++j) pf[j] = ld_nt(&a.sv4[b4b + a.gj[j] + glane]);
+            } else if (mn < steps) {
+                // the next pair's first tile: no other wave touches it
+                const uint64_t n4 = tile_base(perm_step_tile(mn, G, b), a) >> 1;
+#pragma unroll
+                for (int j = 0; j < CHUNKS; ++j) pf[j] = ld_nt(&a.sv4[n4 + a.gj[j] + glane]);
             }
             tile_product<K>(lds, A, a, rd_lane, wr_lane);
             __syncthreads();
@@ -403,6 +435,7 @@ __global__ __launch_bounds__(64) void k_matk_mfma_perm(MfmaArgs a) {
             }
             __syncthreads();
         }
+        m = mn;
     }
 }
 
@@ -540,7 +573,12 @@ bool apply_matk_mfma(aqs_state s, const aqs_op& op, const float* dev_apack, cons
         }
     }
     const int cus = device_cu_count();
-    const uint64_t blocks = std::min<uint64_t>(a.tiles, (uint64_t)cus * 16);
+    uint64_t blocks = std::min<uint64_t>(a.tiles, (uint64_t)cus * 16);
+    if (beta) {
+        // the permuted store's wave-to-tile map needs a power-of-two grid
+        // (tiles is one too, so the grid divides it)
+        while (blocks & (blocks - 1)) blocks &= blocks - 1;
+    }
     ProfScope ps(s, kSlotMfma);
     const int mode = !epi ? 0 : before ? 2 : 1;
     const dim3 grid((unsigned)blocks), block(64);

@@ -89,7 +89,14 @@ __global__ __launch_bounds__(kBlock) void k_bitperm(PermArgs a) {
         pmap[t] = (uint16_t)lds_slot((int)permute_bits((uint64_t)t, a.tmap, a.nt));
     __syncthreads();
     T2* sv = reinterpret_cast<T2*>(a.sv);
-    for (uint64_t o = blockIdx.x; o < a.outer; o += gridDim.x) {
+    // Whether a workgroup skips o depends on o's low bits, so with a plain
+    // grid stride (low bits fixed per workgroup) some workgroups would move
+    // nearly every pair and others none.  Workgroup b takes o = m G + (b ^
+    // (m mod G)) in step m (G = gridDim.x, a power of two dividing outer):
+    // every o once, each workgroup seeing every low-bit pattern.
+    const uint64_t G = gridDim.x, steps = a.outer / G;
+    for (uint64_t m = 0; m < steps; ++m) {
+        const uint64_t o = m * G + ((uint64_t)blockIdx.x ^ (m & (G - 1)));
         const uint64_t o2 = permute_bits(o, a.rmap, a.nr);
         if (o2 < o) continue;  // the partner tile's workgroup does this pair
         const uint64_t b1 = deposit(o, a.rpos, a.nr), b2 = deposit(o2, a.rpos, a.nr);
@@ -214,7 +221,7 @@ void apply_bit_involution(aqs_state s, const std::vector<int>& beta) {
 
     a.outer = 1ull << a.nr;
     if (a.nt < 1 || a.nt > cap) fail(AQS_ERR_INVALID_ARGUMENT, "bit permutation tile out of range");
-    const uint64_t blocks = std::min<uint64_t>(a.outer, 16384);
+    const uint64_t blocks = std::min<uint64_t>(a.outer, 16384);  // powers of two: blocks divides outer
     const size_t slots = ((size_t)1 << a.nt) + (((size_t)1 << a.nt) >> 5);
     const size_t lds = 2 * slots * s->amp_bytes();
     ProfScope ps(s, kSlotPermute);

@@ -285,7 +285,12 @@ struct Term {
 
 class Fuser {
    public:
-    Fuser(aqs_schedule_s* sch, int n, int kmax) : sch_(sch), n_(n), kmax_(kmax) {}
+    // keep_partial: when an op does not fit the union of the open blocks it
+    // touches, close only the blocks it must (keep_for) instead of all of
+    // them, so that small blocks keep growing rather than leaving as
+    // fragments.
+    Fuser(aqs_schedule_s* sch, int n, int kmax, bool keep_partial = false)
+        : sch_(sch), n_(n), kmax_(kmax), keep_partial_(keep_partial) {}
 
     void run(const aqs_op* ops_in, size_t count, bool hoist) {
         // Uncontrolled swaps become relabellings: L[q] is the physical qubit
@@ -362,6 +367,7 @@ class Fuser {
    private:
     aqs_schedule_s* sch_;
     int n_, kmax_;
+    bool keep_partial_;
     std::vector<Block> open_;
     std::vector<Term> terms_;
 
@@ -496,6 +502,42 @@ class Fuser {
         for (size_t j = hit.size(); j-- > 0;) close_block(hit[j]);
     }
 
+    // Which of the open blocks `hit` (touching S) to keep open so that op
+    // can join them: the subset whose union with S is largest within kmax
+    // (ties: more blocks kept).  Open blocks act on disjoint qubits, so
+    // closing some of them first only reorders commuting gates.
+    std::vector<size_t> keep_for(const std::vector<int>& S, const std::vector<size_t>& hit) {
+        size_t best = 0;
+        int best_size = -1, best_count = -1;
+        for (size_t mask = 0; mask < ((size_t)1 << hit.size()); ++mask) {
+            std::vector<size_t> sub;
+            for (size_t j = 0; j < hit.size(); ++j)
+                if ((mask >> j) & 1) sub.push_back(hit[j]);
+            const int size = (int)union_with(S, sub).size();
+            const int count = __builtin_popcountll(mask);
+            if (size > kmax_) continue;
+            if (size > best_size || (size == best_size && count > best_count)) {
+                best = mask;
+                best_size = size;
+                best_count = count;
+            }
+        }
+        std::vector<size_t> keep;
+        for (size_t j = 0; j < hit.size(); ++j)
+            if ((best >> j) & 1) keep.push_back(hit[j]);
+        return keep;
+    }
+
+    // Close the blocks of `hit` outside `keep` and merge op into the rest
+    // (keep not empty).
+    void merge_keeping(const aqs_op& op, const std::vector<int>& S, const std::vector<size_t>& hit,
+                       const std::vector<size_t>& keep) {
+        for (size_t j = hit.size(); j-- > 0;)
+            if (std::find(keep.begin(), keep.end(), hit[j]) == keep.end()) close_block(hit[j]);
+        auto kept = touching(S);
+        merge(op, kept, union_with(S, kept));
+    }
+
     void nondiag(const aqs_op& op) {
         std::vector<int> S = op_qubits(op);
         // deferred phase terms on these qubits go first
@@ -514,6 +556,13 @@ class Fuser {
         if ((int)U.size() <= kmax_ && (!hit.empty() || !is_permutation(op))) {
             merge(op, hit, U);
             return;
+        }
+        if (keep_partial_ && !hit.empty()) {
+            const auto keep = keep_for(S, hit);
+            if (!keep.empty()) {
+                merge_keeping(op, S, hit, keep);
+                return;
+            }
         }
         close_touching(S);
         if (is_permutation(op)) {
@@ -551,6 +600,13 @@ class Fuser {
             if (!held) fresh = true;
         }
         if (!fresh || (int)S.size() > 5) {
+            if (keep_partial_ && (int)S.size() <= kmax_) {
+                const auto keep = keep_for(S, hit);
+                if (!keep.empty()) {
+                    merge_keeping(op, S, hit, keep);
+                    return;
+                }
+            }
             close_touching(S);
             if ((int)S.size() > kmax_) {
                 raw(op);
@@ -812,12 +868,13 @@ aqs_status aqs_fuse(const aqs_op* ops, size_t count, int nqubits, int max_qubits
         for (size_t i = 0; i < count; ++i) validate_op(ops[i], nqubits);
         aqs_schedule_s* best = nullptr;
         double best_cost = 0;
-        for (bool hoist : {false, true}) {
+        for (int variant = 0; variant < 4; ++variant) {
+            const bool hoist = variant & 1, keep_partial = variant >> 1;
             auto* sch = new aqs_schedule_s();
             sch->n = nqubits;
             sch->source_ops = count;
             try {
-                Fuser f(sch, nqubits, max_qubits);
+                Fuser f(sch, nqubits, max_qubits, keep_partial);
                 f.run(ops, count, hoist);
                 for (size_t p = 0; p < sch->passes.size(); ++p)
                     if (sch->passes[p].kind == AQS_OP_MATK) sch->passes[p].u = sch->mats[p].data();

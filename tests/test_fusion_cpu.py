@@ -89,3 +89,28 @@ def test_blocks_hoisting_keeps_cheaper():
         fwd = hip.Schedule(ops.lower_all(circuits.fourier_transform(n, final_swaps=False)), n, 5)
         inv = hip.Schedule(ops.lower_all(circuits.inverse_fourier_transform(n)), n, 5)
         assert inv.cost() <= fwd.cost() + 0.5
+
+
+@pytest.mark.parametrize("n,depth,most", [(12, 8, 14), (17, 12, 32), (20, 20, 50)])
+def test_blocks_brickwork_keep_growing(emu, n, depth, most):
+    """A CZ between two open blocks that do not fit together closes only the
+    full one and joins the other, so blocks keep growing instead of leaving
+    as 2-3 qubit fragments.  Closing every touched block gave 21, 42 and 73
+    passes here (17, 36 and 66 blocks, of which 5, 12 and 26 of 5 qubits);
+    the state still equals the oracle's."""
+    gates = circuits.brickwork(n, depth, seed=n)
+    passes, blocks, tables, perms = check(emu, n, gates, seed=n)
+    sch = hip.Schedule(ops.lower_all(gates), n, 5)
+    sizes = [len(t) for k, t in sch.passes() if k == hip.OP_MATK]
+    sch.close()
+    assert passes <= most, passes
+    assert sum(s == 5 for s in sizes) >= 0.6 * len(sizes), sizes
+
+
+def test_blocks_keep_growing_random_controls(emu):
+    """Random circuits of every gate kind with many 2-3 qubit gates across
+    blocks (the partial close path with controlled swaps, Toffolis and
+    any-of controls) equal the oracle at every block width."""
+    for kmax in (2, 3, 4, 5):
+        for seed in range(3):
+            check(emu, 10, random_all_kinds(10, 150, seed=500 + seed), kmax=kmax, seed=seed)

@@ -299,12 +299,16 @@ __global__ __launch_bounds__(64) void k_matk_mfma(MfmaArgs a) {
             *reinterpret_cast<float4*>(&lds[padded(tau)]) = pf[j];
         }
         __syncthreads();
-        // prefetch the next tile
+        // prefetch the next tile, at a raised wave priority so that its loads
+        // issue ahead of other waves' LDS and matrix work (measured: 1.5%
+        // faster k = 5 passes at 28 qubits)
         const uint64_t tn = t + stride;
         if (tn < a.tiles) {
             const uint64_t n4 = tile_base(tn, a) >> 1;
+            __builtin_amdgcn_s_setprio(2);
 #pragma unroll
             for (int j = 0; j < CHUNKS; ++j) pf[j] = ld_nt(&a.sv4[n4 + a.gj[j] + glane]);
+            __builtin_amdgcn_s_setprio(0);
         }
         tile_product<K>(lds, A, a, rd_lane, wr_lane);
         __syncthreads();

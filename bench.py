@@ -530,7 +530,8 @@ def probe(args):
       QFT k=5           MFMA blocks carrying diagonal tables and the folded
                         bit reversal (k_matk_mfma*, k_diagt_v4)
       QFT tiled         tile passes and the restoring bit permutation (k_bitperm)
-      sample            1000 shots (k_block_sums, k_scan_blocks, k_sample)
+      sample            1000 shots (k_block_sums, k_scan_chunks, k_scan_top,
+                        k_scan_fixup, k_sample)
     """
     hip.init(0)
     n = args.probe_qubits

@@ -36,7 +36,10 @@
 #include <algorithm>
 #include <complex>
 #include <cstring>
+#include <exception>
 #include <map>
+#include <memory>
+#include <thread>
 #include <vector>
 
 #include "internal.h"
@@ -858,6 +861,9 @@ const std::vector<int>* single_involution(const aqs_schedule_s* sch, size_t p) {
 
 extern "C" {
 
+// Op lists at least this long fuse their variants on parallel threads.
+constexpr size_t kParallelFuseOps = 256;
+
 aqs_status aqs_fuse(const aqs_op* ops, size_t count, int nqubits, int max_qubits, aqs_schedule* out) {
     return guard([&] {
         if (!out) fail(AQS_ERR_INVALID_ARGUMENT, "null output schedule");
@@ -866,33 +872,53 @@ aqs_status aqs_fuse(const aqs_op* ops, size_t count, int nqubits, int max_qubits
         if (nqubits < 1 || nqubits > AQS_MAX_QUBITS) fail(AQS_ERR_INVALID_ARGUMENT, "bad qubit count");
         if (max_qubits < 1 || max_qubits > 5) fail(AQS_ERR_INVALID_ARGUMENT, "max_qubits must be 1..5");
         for (size_t i = 0; i < count; ++i) validate_op(ops[i], nqubits);
-        aqs_schedule_s* best = nullptr;
-        double best_cost = 0;
-        for (int variant = 0; variant < 4; ++variant) {
-            const bool hoist = variant & 1, keep_partial = variant >> 1;
-            auto* sch = new aqs_schedule_s();
-            sch->n = nqubits;
-            sch->source_ops = count;
+        // Four independent fusions (closing policy x program order), built on
+        // their own threads for long op lists; the cheapest by the cost model
+        // wins, the first of equals in variant order.
+        constexpr int kVariants = 4;
+        std::unique_ptr<aqs_schedule_s> sch[kVariants];
+        std::exception_ptr err[kVariants];
+        auto build = [&](int variant) {
             try {
-                Fuser f(sch, nqubits, max_qubits, keep_partial);
-                f.run(ops, count, hoist);
-                for (size_t p = 0; p < sch->passes.size(); ++p)
-                    if (sch->passes[p].kind == AQS_OP_MATK) sch->passes[p].u = sch->mats[p].data();
+                auto s = std::make_unique<aqs_schedule_s>();
+                s->n = nqubits;
+                s->source_ops = count;
+                Fuser f(s.get(), nqubits, max_qubits, variant >> 1);
+                f.run(ops, count, variant & 1);
+                for (size_t p = 0; p < s->passes.size(); ++p)
+                    if (s->passes[p].kind == AQS_OP_MATK) s->passes[p].u = s->mats[p].data();
+                sch[variant] = std::move(s);
             } catch (...) {
-                delete sch;
-                delete best;
-                throw;
+                err[variant] = std::current_exception();
             }
-            const double c = schedule_cost(sch);
-            if (!best || c < best_cost) {
-                delete best;
-                best = sch;
-                best_cost = c;
-            } else {
-                delete sch;
+        };
+        std::vector<std::thread> threads;
+        int started = 1;  // variants 1 .. started-1 run on their own threads
+        if (count >= kParallelFuseOps) {
+            try {
+                for (int v = 1; v < kVariants; ++v) {
+                    threads.emplace_back(build, v);
+                    started = v + 1;
+                }
+            } catch (...) {
+                // no thread to be had: the rest run here
             }
         }
-        *out = best;
+        build(0);
+        for (int v = started; v < kVariants; ++v) build(v);
+        for (auto& t : threads) t.join();
+        for (int v = 0; v < kVariants; ++v)
+            if (err[v]) std::rethrow_exception(err[v]);
+        int best = 0;
+        double best_cost = schedule_cost(sch[0].get());
+        for (int v = 1; v < kVariants; ++v) {
+            const double c = schedule_cost(sch[v].get());
+            if (c < best_cost) {
+                best = v;
+                best_cost = c;
+            }
+        }
+        *out = sch[best].release();
     });
 }
 

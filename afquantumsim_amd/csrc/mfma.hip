@@ -48,6 +48,9 @@ namespace {
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
+// Row length of the LDS copy of a table chunk's slice (32 entries + 1 pad).
+constexpr int kTinRow = 33;
+
 // Diagonal table applied in the epilogue (amplitude bit positions).
 struct MfmaEpilogue {
     const float2* tab;    // [2^ng][nc][32]
@@ -98,14 +101,16 @@ __device__ __forceinline__ uint32_t bits_of(uint64_t i, int start, const int* po
 }
 
 // Factor of amplitude i: the per-tile part pg[g] times the inside chunk's
-// entry tin[g][v] (tin: that chunk's table slice in LDS).
+// entry tin[g][v] (tin: that chunk's table slice in LDS, rows of kTinRow
+// entries: the pad entry puts lanes that read one v under different anchor
+// values g on different banks).
 __device__ __forceinline__ float2 epi_factor(const MfmaEpilogue& e, const float2* pg, const float2* tin, uint64_t i) {
     if ((i & e.reqmask) != e.reqmask) return make_float2(1.f, 0.f);
     const uint32_t g = bits_of(i, e.gstart, e.gpos, e.ng);
     float2 f = pg[g];
     if (e.inside >= 0) {
         const int c = e.inside;
-        const float2 t = tin[g * 32 + bits_of(i, e.cstart[c], e.cpos[c], e.clen[c])];
+        const float2 t = tin[g * kTinRow + bits_of(i, e.cstart[c], e.cpos[c], e.clen[c])];
         f = make_float2(f.x * t.x - f.y * t.y, f.x * t.y + f.y * t.x);
     }
     return f;
@@ -141,7 +146,7 @@ __device__ __forceinline__ float2 epi_lane_factor(const MfmaEpilogue& e, const f
     const uint32_t g = ent & 31u;
     float2 f = pg[g];
     if (e.inside >= 0) {
-        const float2 t = tin[g * 32 + (((ent >> 5) & 31u) | vout)];
+        const float2 t = tin[g * kTinRow + (((ent >> 5) & 31u) | vout)];
         f = make_float2(f.x * t.x - f.y * t.y, f.x * t.y + f.y * t.x);
     }
     return f;
@@ -230,10 +235,10 @@ __global__ __launch_bounds__(64) void k_matk_mfma(MfmaArgs a) {
     constexpr int LDS_F2 = TILE + TILE / 16;
     __shared__ __attribute__((aligned(16))) float2 lds[LDS_F2];
     __shared__ float2 pg[EPI ? 32 : 1];     // epilogue: per-tile factor per anchor value
-    __shared__ float2 tin[EPI ? 1024 : 1];  // epilogue: the inside chunk's table slice [g][v]
+    __shared__ float2 tin[EPI ? 32 * kTinRow : 1];  // epilogue: the inside chunk's table slice [g][v]
     if (EPI && a.epi.inside >= 0) {
         for (int e = threadIdx.x; e < (32 << a.epi.ng); e += 64)
-            tin[e] = a.epi.tab[((e >> 5) * a.epi.nc + a.epi.inside) * 32 + (e & 31)];
+            tin[(e >> 5) * kTinRow + (e & 31)] = a.epi.tab[((e >> 5) * a.epi.nc + a.epi.inside) * 32 + (e & 31)];
     }
     const int lane = threadIdx.x;
     const int col = lane & 31;
@@ -364,11 +369,11 @@ __global__ __launch_bounds__(64) void k_matk_mfma_perm(MfmaArgs a) {
     __shared__ __attribute__((aligned(16))) float2 lds[LDS_F2];
     __shared__ uint16_t pmap[TILE];         // destination tile element -> source element
     __shared__ float2 pg[EPI ? 32 : 1];
-    __shared__ float2 tin[EPI ? 1024 : 1];
+    __shared__ float2 tin[EPI ? 32 * kTinRow : 1];
     const int lane = threadIdx.x;
     if (EPI && a.epi.inside >= 0) {
         for (int e = lane; e < (32 << a.epi.ng); e += 64)
-            tin[e] = a.epi.tab[((e >> 5) * a.epi.nc + a.epi.inside) * 32 + (e & 31)];
+            tin[(e >> 5) * kTinRow + (e & 31)] = a.epi.tab[((e >> 5) * a.epi.nc + a.epi.inside) * 32 + (e & 31)];
     }
     for (int e = lane; e < TILE; e += 64) pmap[e] = (uint16_t)permute_index((uint64_t)e, a.perm.tmap, a.ntile_bits);
     const int col = lane & 31;

@@ -553,14 +553,19 @@ std::vector<float> pack_mfma_operand(int k, const double* u) {
 
 namespace {
 
-// Plain 5-qubit blocks on states of at least 12 qubits take the wide tiles.
+// Plain 5-qubit blocks on states of at least kWideMinQubits qubits take the
+// wide tiles.  Below that a launch has too few tiles of 4096 amplitudes for
+// the chip (brickwork depth 20, block schedule: 0.21 against 0.18-0.20 ms at
+// 14 qubits, equal at 18, 0.38 against 0.40 ms at 20, 51.5 against 57.4 ms
+// at 28).
 constexpr bool kWideBlocks = true;
+constexpr int kWideMinQubits = 18;
 
 // A plain k = 5 block on wide tiles (k_matk_mfma_wide); false when it does
 // not apply.
 bool apply_matk_mfma_wide(aqs_state s, const aqs_op& op, const float* dev_apack) {
     const int n = s->n;
-    if (!kWideBlocks || op.ntargets != 5 || n < kWideBits) return false;
+    if (!kWideBlocks || op.ntargets != 5 || n < kWideMinQubits) return false;
     std::vector<int> tbits;
     for (int j = 0; j < 5; ++j) tbits.push_back(qpos(n, op.targets[j]));
     std::vector<int> freeb;

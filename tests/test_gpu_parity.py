@@ -583,6 +583,35 @@ def test_33_qubit_launch_sizes():
     st.close()
 
 
+@pytest.mark.parametrize("targets", [(13, 14, 15, 16, 17), (0, 1, 2, 3, 4), (2, 5, 9, 11, 16), (0, 6, 7, 8, 17),
+                                     (17, 3, 12, 8, 1)])
+def test_mfma_wide_tiles_18q(targets):
+    """A 5-qubit block on 18 qubits runs on the wide tiles (its targets and
+    the 7 lowest other bits): qubits at the top, the bottom, scattered, and
+    listed out of order (the sub-index follows the op's target order)
+    against the complex128 path, which multiplies group by group."""
+    n = 18
+    rng = np.random.default_rng(sum(targets))
+    U = _haar(5, rng)
+    op = {"kind": hip.OP_MATK, "targets": tuple(targets), "u": U}
+    sv = random_state(n, 17)
+    st = hip.State(n, hip.C128)
+    st.upload(sv)
+    st.apply([op])
+    want = st.download()
+    st.close()
+    st64 = hip.State(n, hip.C64)
+    st64.upload(sv.astype(np.complex64))
+    st64.profile(True)
+    st64.apply([op])
+    prof = st64.profile_read()
+    got = st64.download()
+    st64.close()
+    assert prof.get("MATK_MFMA", (0.0, 0))[1] == 1, prof
+    np.testing.assert_allclose(got, want, atol=TOL64)
+    assert oracle.infidelity(got, want) < 1e-10
+
+
 @pytest.mark.parametrize("schedule", ["plain", "blocks5", "tiles"])
 def test_brickwork_20q_depth20_against_oracle(schedule):
     """The measured workload's circuit (U3 on every qubit, CZ on alternating

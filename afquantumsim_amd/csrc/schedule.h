@@ -36,28 +36,42 @@ enum TileOpKind : int32_t {
     TILE_XCH = 1,     // exchange across register slot `slot`
     TILE_PHASES = 2,  // a run of diagonal gates as phase terms [term_begin, +term_count)
     TILE_MATR = 3,    // real 2x2 on register slot `slot` (half the multiplies of TILE_MAT)
+    TILE_MATP = 4,    // uncontrolled: the slot's 1-half times z, then a real 2x2 (3/4 of TILE_MAT)
 };
 
 // Kernel dispatch code of an op (one switch per op, no divergent branch
 // around the register array):
-//   form 0  no control inside the tile (controls outside it: one test per tile)
+//   form 0  no control
 //   form 1  controls on register slots only (one uniform test per pair)
-//   form 2  controls on thread-held positions, or "any" controls (per-thread
-//           predicate, results selected)
+//   form 2  controls on thread-held positions or outside the tile, or "any"
+//           controls (per-thread predicate, results selected)
 // MAT: 0 + 4*form + slot, XCH: 12 + 4*form + slot, PHASES: 24 direct, 25 zeta,
-// MATR: 26 + 4*form + slot.
+// MATR: 26 + 4*form + slot, MATP: 38 + slot (form 0 only), MATP set:
+// 42 + slot mask (two or more slots) on the first of popcount(mask)
+// consecutive MATP ops whose slots are the mask's bits in ascending order --
+// uncontrolled gates on distinct qubits, dispatched once.
 constexpr int kTileCodePhases = 24;
 constexpr int kTileCodeMatR = 26;
-constexpr int kTileCodeCount = 38;
+constexpr int kTileCodeMatP = 38;
+constexpr int kTileCodeMatPSet = 42;
+constexpr int kTileCodeCount = 58;
 inline int tile_code(int kind, int form, int slot, bool zeta) {
     if (kind == TILE_PHASES) return kTileCodePhases + (zeta ? 1 : 0);
+    if (kind == TILE_MATP) return form == 0 ? kTileCodeMatP + slot : -1;
     if (kind == TILE_MATR) return kTileCodeMatR + 4 * form + slot;
     return (kind == TILE_XCH ? 12 : 0) + 4 * form + slot;
 }
 constexpr bool tile_code_is_phases(int code) { return code == kTileCodePhases || code == kTileCodePhases + 1; }
-// form and slot of a MAT / XCH / MATR code
-constexpr int tile_code_form(int code) { return code >= kTileCodeMatR ? (code - kTileCodeMatR) / 4 : (code % 12) / 4; }
-constexpr int tile_code_slot(int code) { return (code >= kTileCodeMatR ? code - kTileCodeMatR : code) % 4; }
+// form and slot of a MAT / XCH / MATR / MATP code
+constexpr int tile_code_form(int code) {
+    return code >= kTileCodeMatP ? 0 : code >= kTileCodeMatR ? (code - kTileCodeMatR) / 4 : (code % 12) / 4;
+}
+constexpr int tile_code_slot(int code) {
+    return code >= kTileCodeMatPSet ? __builtin_ctz((unsigned)(code - kTileCodeMatPSet))
+           : code >= kTileCodeMatP  ? code - kTileCodeMatP
+                                    : (code >= kTileCodeMatR ? code - kTileCodeMatR : code) % 4;
+}
+constexpr bool tile_code_is_matp(int code) { return code >= kTileCodeMatP; }
 
 // 64 bytes, read with scalar loads.  Complex64 kernels read the FP32 matrix
 // here; complex128 kernels read TilePass::m128[8 * op].
@@ -68,16 +82,19 @@ struct TileOpDev {
     uint32_t c_thr;   // control local positions held by the thread (mask over local positions)
     uint64_t c_glb;   // control index bits outside the tile
     uint32_t term_begin, term_count;
-    float mf[8];      // MAT: row-major 2x2 in FP32 (complex); MATR: mf[0..3] = real row-major 2x2
+    float mf[8];      // MAT: row-major 2x2 in FP32 (complex); MATR: mf[0..3] = real row-major 2x2;
+                      // MATP: mf[0..3] real 2x2, mf[4..5] = z
 };
 static_assert(sizeof(TileOpDev) == 64, "compact tile op");
 
 // Phase term: multiply by z where every bit of the three masks is set
 // (register slots, thread-held local positions, index bits outside the
 // tile).  Terms of one PHASES op are sorted by register mask; the first term
-// of each equal-mask group carries the group length.  z in FP32 here, in
-// FP64 in TilePass::z128[2 * term].  A folded term (fold >= 0) takes its z
-// from the per-tile value of fold `fold` instead.
+// of each equal-mask group carries the group length, and real = 1 when
+// every z of the group is real (no folded term): the group's product then
+// scales amplitudes by a real factor.  z in FP32 here, in FP64 in
+// TilePass::z128[2 * term].  A folded term (fold >= 0) takes its z from the
+// per-tile value of fold `fold` instead.
 struct TileTermDev {
     uint16_t m_reg;
     uint16_t group;
@@ -85,7 +102,7 @@ struct TileTermDev {
     uint64_t m_glb;
     float zf[2];
     int32_t fold;
-    uint32_t pad;
+    uint32_t real;
 };
 static_assert(sizeof(TileTermDev) == 32, "compact tile term");
 

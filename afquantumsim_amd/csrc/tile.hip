@@ -10,6 +10,11 @@
 // slots, the thread id is spread over the other m-4 -- and applies the
 // segment's ops to them:
 //   MAT     2x2 matrix across one register slot (with controls anywhere);
+//   MATR    real 2x2 across one register slot (H, RotY, controlled or not);
+//   MATP    uncontrolled one-qubit gate as a phase on the slot's 1-half and
+//           a real 2x2 (the gate's other phases are carried by the host
+//           into the qubit's next gate); consecutive MATP ops on distinct
+//           slots are dispatched once, as a set;
 //   XCH     exchange across one register slot (X, CX, Toffoli, Or);
 //   PHASES  a run of diagonal gates (Z, S, T, Phase, RZ, CZ, CPhase and
 //           controlled forms) as phase terms "multiply by z where these bits
@@ -90,8 +95,23 @@ __device__ __forceinline__ void butterfly_real(T2& p, T2& q, const R (&m)[8]) {
     q = v;
 }
 
+// q times z (m[4], m[5]), then the real 2x2 m[0..3]: a one-qubit gate
+// factored as a rotation between phases, 12 multiplies per pair.
+template <typename T2, typename R>
+__device__ __forceinline__ void butterfly_rotp(T2& p, T2& q, const R (&m)[8]) {
+    T2 w, u, v;
+    w.x = m[4] * q.x - m[5] * q.y;
+    w.y = m[4] * q.y + m[5] * q.x;
+    u.x = m[0] * p.x + m[1] * w.x;
+    u.y = m[0] * p.y + m[1] * w.y;
+    v.x = m[2] * p.x + m[3] * w.x;
+    v.y = m[2] * p.y + m[3] * w.y;
+    p = u;
+    q = v;
+}
+
 // Pair operations across register slot SL (OP: 0 complex 2x2, 1 exchange,
-// 2 real 2x2).  Form 0: every pair.  Form 1:
+// 2 real 2x2, 3 phase then real 2x2).  Form 0: every pair.  Form 1:
 // pairs whose register index holds the control slots `creg` (uniform test,
 // the partner has the same control bits since a control is never the
 // target).  Form 2: per-thread predicate `ok(j)`, results selected.
@@ -109,6 +129,8 @@ __device__ __forceinline__ void pairs(T2 (&x)[16], const R (&m)[8], uint32_t cre
             q = t;
         } else if (OP == 2) {
             butterfly_real(p, q, m);
+        } else if (OP == 3) {
+            butterfly_rotp(p, q, m);
         } else {
             butterfly(p, q, m);
         }
@@ -121,6 +143,35 @@ __device__ __forceinline__ void pairs(T2 (&x)[16], const R (&m)[8], uint32_t cre
             x[k] = q;
         }
     }
+}
+
+// One slot of a MATP set starting at op `at`: the op for slot SL is the
+// k-th of the set, k = number of the mask's slots below SL.
+template <int SL, int M, typename T2, typename R, typename OK>
+__device__ __forceinline__ void matp_set_slot(T2 (&x)[16], const TileOpDev* __restrict__ ops,
+                                              const double* __restrict__ m128, uint32_t at, OK ok) {
+    if constexpr (((M >> SL) & 1) != 0) {
+        constexpr int k = __builtin_popcount(M & ((1 << SL) - 1));
+        R m[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            if constexpr (sizeof(R) == 4)
+                m[i] = ops[at + k].mf[i];
+            else
+                m[i] = m128[8 * (size_t)(at + k) + i];
+        }
+        pairs<SL, 3, 0>(x, m, 0u, ok);
+    }
+}
+
+// A MATP set: one-qubit gates on the register slots of mask M, in slot order.
+template <int M, typename T2, typename R, typename OK>
+__device__ __forceinline__ void matp_set(T2 (&x)[16], const TileOpDev* __restrict__ ops,
+                                         const double* __restrict__ m128, uint32_t at, OK ok) {
+    matp_set_slot<0, M, T2, R>(x, ops, m128, at, ok);
+    matp_set_slot<1, M, T2, R>(x, ops, m128, at, ok);
+    matp_set_slot<2, M, T2, R>(x, ops, m128, at, ok);
+    matp_set_slot<3, M, T2, R>(x, ops, m128, at, ok);
 }
 
 template <typename R>
@@ -167,12 +218,36 @@ __device__ __forceinline__ void phases_direct(T2 (&x)[16], const TileTermDev* __
                                            uint32_t lthr, uint64_t base) {
     for (uint32_t k = 0; k < count;) {
         const uint32_t g = tt[k].group, mr = tt[k].m_reg;
+        const bool real = tt[k].real != 0;
         R pr, pi;
         group_product<T2, R>(tt, z128, fold_val, k, g, lthr, base, pr, pi);
         k += g;
         // one case per register mask: each multiplies a fixed set of
         // registers in place (a runtime test per register made the compiler
-        // carry extra copies of the register array through the op loop)
+        // carry extra copies of the register array through the op loop);
+        // real products (CZ, Z, controlled Z) scale: 2 multiplies per
+        // amplitude instead of 4
+        if (real) {
+            switch (mr) {
+#define AQS_REAL_CASE(M)                                                      \
+    case M:                                                                   \
+        _Pragma("unroll") for (int j = 0; j < 16; ++j) if ((j & (M)) == (M)) { \
+            x[j].x *= pr;                                                     \
+            x[j].y *= pr;                                                     \
+        }                                                                     \
+        break;
+                AQS_REAL_CASE(0) AQS_REAL_CASE(1) AQS_REAL_CASE(2) AQS_REAL_CASE(3)
+                AQS_REAL_CASE(4) AQS_REAL_CASE(5) AQS_REAL_CASE(6) AQS_REAL_CASE(7)
+                AQS_REAL_CASE(8) AQS_REAL_CASE(9) AQS_REAL_CASE(10) AQS_REAL_CASE(11)
+                AQS_REAL_CASE(12) AQS_REAL_CASE(13) AQS_REAL_CASE(14)
+                default:  // mask 15
+                    x[15].x *= pr;
+                    x[15].y *= pr;
+                    break;
+#undef AQS_REAL_CASE
+            }
+            continue;
+        }
         switch (mr) {
 #define AQS_MASK_CASE(M)                                                      \
     case M:                                                                   \
@@ -356,13 +431,12 @@ __global__ __launch_bounds__(1 << W, (sizeof(T2) == 8 ? 4 : 2)) void k_tile(
                     continue;
                 }
                 const bool any = op.any != 0;
-                // all-set controls outside the tile: one test per tile
-                if (!any && (base & op.c_glb) != op.c_glb) {
-                    continue;
-                }
                 const uint32_t creg = op.c_reg, cthr = op.c_thr;
-                const bool g_any = (base & op.c_glb) != 0;
-                const bool t_all = (lthr & cthr) == cthr, t_any = (lthr & cthr) != 0u;
+                // forms 0 and 1 have no control outside the tile; form 2
+                // selects its results per thread (all-set or any-set
+                // controls on thread-held and outer bits)
+                const bool g_all = (base & op.c_glb) == op.c_glb, g_any = (base & op.c_glb) != 0;
+                const bool t_all = (lthr & cthr) == cthr && g_all, t_any = (lthr & cthr) != 0u;
                 auto ok = [&](int j) {
                     if (any) return g_any || t_any || ((uint32_t)j & creg) != 0u;
                     return t_all && ((uint32_t)j & creg) == creg;
@@ -389,8 +463,17 @@ __global__ __launch_bounds__(1 << W, (sizeof(T2) == 8 ? 4 : 2)) void k_tile(
                     AQS_PAIRS4(kTileCodeMatR, 2, 0)
                     AQS_PAIRS4(kTileCodeMatR + 4, 2, 1)
                     AQS_PAIRS4(kTileCodeMatR + 8, 2, 2)
+                    AQS_PAIRS4(kTileCodeMatP, 3, 0)
 #undef AQS_PAIRS4
 #undef AQS_PAIRS
+#define AQS_MATPS(M)                                                   \
+    case kTileCodeMatPSet + M:                                         \
+        matp_set<M, T2, R>(x, ops, m128, ob + o, ok);                  \
+        o += __builtin_popcount(M) - 1;                                \
+        break;
+                    AQS_MATPS(3) AQS_MATPS(5) AQS_MATPS(6) AQS_MATPS(7) AQS_MATPS(9) AQS_MATPS(10)
+                    AQS_MATPS(11) AQS_MATPS(12) AQS_MATPS(13) AQS_MATPS(14) AQS_MATPS(15)
+#undef AQS_MATPS
                     default: break;
                 }
             }

@@ -67,6 +67,8 @@ class Tiler {
     void run(const aqs_op* ops_in, size_t count) {
         std::vector<int> L(n_);
         for (int q = 0; q < n_; ++q) L[q] = q;
+        pend0_.assign(n_, cd(1));
+        pend1_.assign(n_, cd(1));
         for (size_t i = 0; i < count; ++i) {
             aqs_op op = ops_in[i];
             if (op.kind == AQS_OP_SWAP && op.ncontrols == 0 && !(op.flags & AQS_CTRL_ANY)) {
@@ -81,6 +83,7 @@ class Tiler {
             op.more_controls = more;
             lower(op);
         }
+        for (int b = 0; b < n_; ++b) flush_pending(b);
         std::vector<char> done(g_.size(), 0);
         size_t first = 0;
         while (true) {
@@ -100,6 +103,31 @@ class Tiler {
     aqs_schedule_s* sch_;
     int n_, m_;
     std::vector<GOp> g_;
+    // Per index bit, a diagonal diag(p0, p1) not yet applied: phases of
+    // one-qubit gates on the bit, carried forward (a diagonal commutes with
+    // every gate that uses the bit only diagonally) into the next rotation
+    // on the bit, or emitted as phase terms before the next gate that uses
+    // the bit otherwise.
+    std::vector<cd> pend0_, pend1_;
+
+    void flush_pending(int b) {
+        const cd p0 = pend0_[b], p1 = pend1_[b];
+        pend0_[b] = pend1_[b] = cd(1);
+        if (p0 == cd(1) && p1 == cd(1)) return;
+        GOp ph;
+        ph.kind = TILE_PHASES;
+        ph.dg = {b};
+        if (p0 != cd(1)) ph.terms.push_back({{}, p0});
+        if (p1 != p0) ph.terms.push_back({{b}, p1 / p0});
+        g_.push_back(std::move(ph));
+    }
+
+    // Every gate enters the list here, after the pending diagonals of the
+    // bits it uses non-diagonally.
+    void push(GOp&& o) {
+        for (int b : o.nd) flush_pending(b);
+        g_.push_back(std::move(o));
+    }
 
     int bit(int q) const { return n_ - 1 - q; }
 
@@ -118,7 +146,16 @@ class Tiler {
                 o.nd = {o.tbit};
                 o.dg = o.ctrl;
                 std::memcpy(o.m, op.m, sizeof(o.m));
-                if (op.kind == AQS_OP_MAT1 && !o.any && lower_real(o)) return;
+                if (op.kind == AQS_OP_MAT1 && !o.any) {
+                    if (o.ctrl.empty() && lower_rot(o)) return;
+                    if (lower_real(o)) return;
+                }
+                if (op.kind == AQS_OP_X && o.ctrl.empty() && !o.any) {
+                    // X diag(p0, p1) = diag(p1, p0) X
+                    std::swap(pend0_[o.tbit], pend1_[o.tbit]);
+                    g_.push_back(std::move(o));
+                    return;
+                }
                 break;
             case AQS_OP_DIAG1: {
                 if (o.any) break;
@@ -128,6 +165,11 @@ class Tiler {
                 o.dg = o.ctrl;
                 o.dg.push_back(t);
                 const cd d0(op.m[0], op.m[1]), d1(op.m[6], op.m[7]);
+                if (o.ctrl.empty() && d0 != cd(0) && d1 != cd(0)) {
+                    pend0_[t] *= d0;
+                    pend1_[t] *= d1;
+                    return;
+                }
                 if (d0 == cd(0)) break;  // not unitary: keep the plain pass
                 o.terms.push_back({o.ctrl, d0});
                 o.terms.push_back({o.dg, d1 / d0});
@@ -135,6 +177,10 @@ class Tiler {
             }
             case AQS_OP_PHASE:
                 if (o.any) break;
+                if (o.ctrl.empty() && op.ntargets == 1 && cd(op.m[0], op.m[1]) != cd(0)) {
+                    pend1_[bit(op.targets[0])] *= cd(op.m[0], op.m[1]);
+                    return;
+                }
                 o.kind = TILE_PHASES;
                 o.dg = o.ctrl;
                 for (int j = 0; j < op.ntargets; ++j) o.dg.push_back(bit(op.targets[j]));
@@ -171,14 +217,60 @@ class Tiler {
             for (int j = 0; j < op.ntargets; ++j) o.nd.push_back(bit(op.targets[j]));
             for (int c : o.ctrl) o.nd.push_back(c);
         }
-        g_.push_back(std::move(o));
+        push(std::move(o));
     }
 
-    // A one-qubit gate with a real matrix (H, RotY) as a real 2x2: half the
-    // multiplies of a complex one.  (Factoring complex gates into a real
-    // rotation between phases was measured 2x slower on the brickwork: the
-    // phase runs it adds between consecutive rotations cost more than the
-    // saved multiplies.)
+    // An uncontrolled one-qubit gate as U = diag(g0, g1) R diag(1, z) with
+    // R = [[c, -s], [s, c]] real (c = |u00|, s = |u10|).  The bit's pending
+    // diagonal diag(p0, p1) is applied first, so the op run here is
+    // R diag(1, z p1 / p0) (TILE_MATP: 12 real multiplies per amplitude pair
+    // against 16 for a complex 2x2) and p0 diag(g0, g1) becomes the pending
+    // diagonal.  A qubit's phases between rotations then cost nothing.
+    // Real matrices with nothing pending stay TILE_MATR.  The form holds
+    // when u00 u11 / (u01 u10) is real and negative (every unitary; FP32
+    // rounding of a complex one can break it): a matrix it does not
+    // reproduce to 1e-12 stays TILE_MAT.
+    bool lower_rot(const GOp& o) {
+        const double* m = o.op.m;
+        const int b = o.tbit;
+        const cd u00(m[0], m[1]), u01(m[2], m[3]), u10(m[4], m[5]), u11(m[6], m[7]);
+        const cd p0 = pend0_[b], p1 = pend1_[b];
+        const bool real = m[1] == 0 && m[3] == 0 && m[5] == 0 && m[7] == 0;
+        if (real && p0 == cd(1) && p1 == cd(1)) return false;
+        if (u01 == cd(0) && u10 == cd(0)) {
+            if (u00 == cd(0) || u11 == cd(0)) return false;
+            pend0_[b] = p0 * u00;
+            pend1_[b] = p1 * u11;
+            return true;
+        }
+        const double c = std::abs(u00), s = std::abs(u10);
+        if (s == 0) return false;
+        cd g0, g1 = u10 / s, z(1);
+        if (c >= s) {
+            g0 = u00 / c;
+            z = u11 / (g1 * c);
+        } else if (c > 0) {
+            g0 = u00 / c;
+            z = -u01 / (g0 * s);
+        } else {
+            g0 = -u01 / s;
+        }
+        const double err = std::max(std::max(std::abs(g0 * c - u00), std::abs(-g0 * s * z - u01)),
+                                    std::max(std::abs(g1 * s - u10), std::abs(g1 * c * z - u11)));
+        if (!(err <= 1e-12 * std::max(c, s))) return false;
+        const cd ze = z * p1 / p0;
+        GOp r = o;
+        r.kind = ze == cd(1) ? TILE_MATR : TILE_MATP;
+        const double mm[8] = {c, -s, s, c, ze.real(), ze.imag(), 0, 0};
+        std::memcpy(r.m, mm, sizeof(r.m));
+        g_.push_back(std::move(r));
+        pend0_[b] = p0 * g0;
+        pend1_[b] = p0 * g1;
+        return true;
+    }
+
+    // A one-qubit gate with a real matrix (H, RotY), controlled or not, as a
+    // real 2x2: half the multiplies of a complex one.
     bool lower_real(const GOp& o) {
         const double* m = o.op.m;
         if (m[1] != 0 || m[3] != 0 || m[5] != 0 || m[7] != 0) return false;
@@ -186,7 +278,7 @@ class Tiler {
         r.kind = TILE_MATR;
         const double re[8] = {m[0], m[2], m[4], m[6], 0, 0, 0, 0};
         std::memcpy(r.m, re, sizeof(r.m));
-        g_.push_back(std::move(r));
+        push(std::move(r));
         return true;
     }
 
@@ -203,7 +295,7 @@ class Tiler {
             if ((t.more_controls >> q) & 1ull) o.ctrl.push_back(bit(q));
         o.ctrl.push_back(bit(extra));
         o.dg = o.ctrl;
-        g_.push_back(std::move(o));
+        push(std::move(o));
     }
 
     void plain(const aqs_op& op) {
@@ -324,6 +416,7 @@ class Tiler {
                 --left;
             }
             flush_phases(run, lp, reg, tp);
+            group_rotations(tp, sd.op_begin);
             sd.op_count = (uint32_t)tp.ops.size() - sd.op_begin;
             if (left == left_before) fail(AQS_ERR_RUNTIME, "tile schedule made no progress");
             // (phase terms that multiply to exactly 1 leave nothing to run)
@@ -347,20 +440,38 @@ class Tiler {
                 if (ok) seen |= 1u << s.thr[b];
             }
             ok = ok && (uint64_t)s.op_begin + s.op_count <= tp.ops.size();
+            // a MATP set: the following ops of the segment are MATP ops on
+            // the mask's other slots, ascending
+            for (uint32_t o = s.op_begin; ok && o < s.op_begin + s.op_count; ++o) {
+                const int code = tp.ops[o].code;
+                if (code < kTileCodeMatPSet || code >= kTileCodeCount) continue;
+                const uint32_t mask = (uint32_t)(code - kTileCodeMatPSet);
+                const uint32_t pc = (uint32_t)__builtin_popcount(mask);
+                ok = pc >= 2 && o + pc <= s.op_begin + s.op_count;
+                uint32_t rest = mask & (mask - 1);
+                for (uint32_t k = 1; ok && k < pc; ++k, rest &= rest - 1)
+                    ok = tp.ops[o + k].code == kTileCodeMatP + __builtin_ctz(rest);
+            }
         }
         for (const auto& o : tp.ops) {
             ok = ok && o.code >= 0 && o.code < kTileCodeCount && o.c_reg < (1u << kTileRegBits) &&
                  o.c_thr < (1u << m) && (o.c_glb >> n_) == 0;
             if (!tile_code_is_phases(o.code)) {
                 const int form = tile_code_form(o.code);
-                ok = ok && (form == 2 || (!o.any && o.c_thr == 0 && (form == 1) == (o.c_reg != 0)));
+                ok = ok && (form == 2 || (!o.any && o.c_thr == 0 && o.c_glb == 0 && (form == 1) == (o.c_reg != 0)));
             }
             if (tile_code_is_phases(o.code)) {
                 ok = ok && o.term_count > 0 && (uint64_t)o.term_begin + o.term_count <= tp.terms.size();
-                // groups tile the op's terms exactly
+                // groups tile the op's terms exactly; a real group holds
+                // real, unfolded terms only
                 for (uint32_t k = 0; ok && k < o.term_count;) {
-                    const uint32_t g = tp.terms[o.term_begin + k].group;
-                    ok = g > 0 && k + g <= o.term_count;
+                    const TileTermDev& first = tp.terms[o.term_begin + k];
+                    const uint32_t g = first.group;
+                    ok = g > 0 && k + g <= o.term_count && first.real <= 1;
+                    for (uint32_t e = k; ok && first.real && e < k + g; ++e) {
+                        const size_t t = o.term_begin + e;
+                        ok = tp.terms[t].fold < 0 && tp.terms[t].zf[1] == 0.0f && tp.z128[2 * t + 1] == 0.0;
+                    }
                     k += g ? g : 1;
                 }
             }
@@ -373,6 +484,38 @@ class Tiler {
         for (const auto& f : tp.folds) ok = ok && (uint64_t)f.begin + f.count <= tp.fold_terms.size();
         for (const auto& f : tp.fold_terms) ok = ok && (f.m_glb >> n_) == 0;
         if (!ok) fail(AQS_ERR_RUNTIME, "tile schedule malformed");
+    }
+
+    // Consecutive MATP ops on distinct slots commute (uncontrolled gates on
+    // distinct qubits): each maximal run of them is put in slot order and
+    // marked on its first op as one MATP set, which the kernel dispatches
+    // once.
+    static void group_rotations(TilePass& tp, size_t begin) {
+        const size_t end = tp.ops.size();
+        for (size_t i = begin; i < end;) {
+            uint32_t mask = 0;
+            size_t j = i;
+            while (j < end && tp.ops[j].code >= kTileCodeMatP && tp.ops[j].code < kTileCodeMatPSet &&
+                   !((mask >> (tp.ops[j].code - kTileCodeMatP)) & 1u)) {
+                mask |= 1u << (tp.ops[j].code - kTileCodeMatP);
+                ++j;
+            }
+            if (j - i < 2) {
+                i = std::max(j, i + 1);
+                continue;
+            }
+            std::vector<TileOpDev> ops(tp.ops.begin() + i, tp.ops.begin() + j);
+            std::vector<double> m(tp.m128.begin() + 8 * i, tp.m128.begin() + 8 * j);
+            std::vector<size_t> order(j - i);
+            for (size_t k = 0; k < order.size(); ++k) order[k] = k;
+            std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return ops[a].code < ops[b].code; });
+            for (size_t k = 0; k < order.size(); ++k) {
+                tp.ops[i + k] = ops[order[k]];
+                std::copy(m.begin() + 8 * order[k], m.begin() + 8 * order[k] + 8, tp.m128.begin() + 8 * (i + k));
+            }
+            tp.ops[i].code = kTileCodeMatPSet + (int)mask;
+            i = j;
+        }
     }
 
     static int slot_of(const std::vector<int>& reg, int p) {
@@ -399,7 +542,11 @@ class Tiler {
         TileOpDev d{};
         d.any = o.any ? 1 : 0;
         masks(o.ctrl, lp, reg, d.c_reg, d.c_thr, d.c_glb);
-        const int form = (d.any || d.c_thr) ? 2 : d.c_reg ? 1 : 0;
+        // controls outside the tile go through the per-thread select form,
+        // so the kernel's op loop has no path that leaves the registers as
+        // they are (such a path made the compiler copy all 16 amplitudes
+        // before every op)
+        const int form = (d.any || d.c_thr || d.c_glb) ? 2 : d.c_reg ? 1 : 0;
         d.code = tile_code(o.kind, form, slot_of(reg, lp[o.tbit]), false);
         for (int i = 0; i < 8; ++i) d.mf[i] = (float)o.m[i];
         tp.ops.push_back(d);
@@ -502,8 +649,13 @@ class Tiler {
             d.fold = sorted_fold[i];
             if (i == gend) {
                 size_t e = i;
-                while (e < merged.size() && merged[e].reg == merged[i].reg && e - i < 0xFFFF) ++e;
+                bool real = true;
+                while (e < merged.size() && merged[e].reg == merged[i].reg && e - i < 0xFFFF) {
+                    real = real && sorted_fold[e] < 0 && merged[e].z.imag() == 0.0;
+                    ++e;
+                }
                 d.group = (uint16_t)(e - i);
+                d.real = real ? 1u : 0u;
                 gend = e;
                 direct += 16.0 / (double)(1u << __builtin_popcount(merged[i].reg));
                 groups += 1;
@@ -535,6 +687,8 @@ double pass_cost(const aqs_schedule_s* sch, size_t p) {
         for (const auto& o : t.ops) {
             if (o.code < 12) {
                 c += 0.107;
+            } else if (o.code >= kTileCodeMatP) {
+                c += 0.080;
             } else if (o.code >= kTileCodeMatR) {
                 c += 0.054;
             } else if (!tile_code_is_phases(o.code)) {

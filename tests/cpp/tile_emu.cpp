@@ -185,15 +185,18 @@ void tile_apply(std::vector<cplx>& v, int n, const TilePass& tp) {
                     const bool g_all = (base & op.c_glb) == op.c_glb, g_any = (base & op.c_glb) != 0;
                     if (!any && !g_all) continue;
                     const bool t_all = (lthr & op.c_thr) == op.c_thr, t_any = (lthr & op.c_thr) != 0;
-                    // code: MAT 4 * form + slot, XCH 12 + ..., MATR 26 + ...;
+                    // code: MAT 4 * form + slot, XCH 12 + ..., MATR 26 + ..., MATP 38 + slot,
+                    // MATP set 42 + mask (read as the MATP of its first slot; the
+                    // set's other ops follow as MATP ops);
                     // the form only selects how the kernel evaluates the
                     // same controls
                     const int slot = tile_code_slot(op.code);
                     const bool mat = op.code < 12;
-                    const bool real = op.code >= kTileCodeMatR;
+                    const bool real = op.code >= kTileCodeMatR && op.code < kTileCodeMatP;
+                    const bool rotp = op.code >= kTileCodeMatP;
                     const int form = tile_code_form(op.code);
-                    if (form == 0 && (op.c_reg || op.c_thr || any)) throw 1;
-                    if (form == 1 && (op.c_thr || any || !op.c_reg)) throw 1;
+                    if (form == 0 && (op.c_reg || op.c_thr || op.c_glb || any)) throw 1;
+                    if (form == 1 && (op.c_thr || op.c_glb || any || !op.c_reg)) throw 1;
                     for (int j = 0; j < 16; ++j) {
                         if ((j >> slot) & 1) continue;
                         if (!ctrl_ok(j, op.c_reg, any, g_any, t_all, t_any)) continue;
@@ -206,6 +209,11 @@ void tile_apply(std::vector<cplx>& v, int n, const TilePass& tp) {
                         } else if (real) {
                             const double* m = &tp.m128[8 * (size_t)(S.op_begin + o)];
                             const cplx p = x[j], q = x[k];
+                            x[j] = m[0] * p + m[1] * q;
+                            x[k] = m[2] * p + m[3] * q;
+                        } else if (rotp) {
+                            const double* m = &tp.m128[8 * (size_t)(S.op_begin + o)];
+                            const cplx p = x[j], q = x[k] * cplx(m[4], m[5]);
                             x[j] = m[0] * p + m[1] * q;
                             x[k] = m[2] * p + m[3] * q;
                         } else {

@@ -42,6 +42,16 @@ def emulate(lib, n, sv, gates, tile_qubits=0):
     return v, tuple(counts)
 
 
+def emulate_ops(lib, n, sv, lowered, tile_qubits=0):
+    arr, keep = hip.pack_ops(lowered)
+    v = np.ascontiguousarray(sv, dtype=np.complex128).copy()
+    counts = (ctypes.c_size_t * 4)()
+    rc = lib.tile_emulate(arr, len(lowered) and len(arr), n, tile_qubits, v.ctypes.data, counts)
+    del keep
+    assert rc == 0, (rc, hip.load().aqs_last_error())
+    return v, tuple(counts)
+
+
 def check(lib, n, gates, seed=0, tile_qubits=0):
     sv = random_state(n, seed)
     want = oracle.run(sv, n, gates, threads=8)
@@ -191,3 +201,58 @@ def test_tiled_folded_phase_terms(emu, tile):
     g += [oracle.gate("RotX", q, angle=0.2 * q) for q in range(n)]
     counts = check(emu, n, g, seed=4, tile_qubits=tile)
     assert counts[3] > 0
+
+
+def _haar2(rng):
+    z = rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2))
+    q, r = np.linalg.qr(z)
+    return q * (np.diag(r) / abs(np.diag(r)))
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_tiled_one_qubit_phases_carried(emu, seed):
+    """Uncontrolled one-qubit gates run as a phase on the target's 1-half
+    followed by a real rotation; the rest of each gate's diagonal (global
+    phases of general unitaries included) is carried to the qubit's next
+    gate, swapped by X, kept across controls and diagonal gates, and written
+    out as phase terms before any other gate on the qubit.  Chains of general
+    unitaries (Y: |u00| = 0, diagonal and real matrices, a non-unitary
+    matrix) mixed with every other kind on a few qubits must equal the
+    oracle."""
+    n = 12
+    rng = np.random.default_rng(seed)
+    few = [0, 5, 10, 11]
+    gates, lowered = [], []
+    for i in range(160):
+        r = int(rng.integers(12))
+        a, b, c = (int(x) for x in rng.choice(few, 3, replace=False))
+        if r < 4:
+            u = _haar2(rng)
+            if r == 1:
+                u = np.diag(np.exp(1j * rng.uniform(-3, 3, size=2)))  # diagonal, as a matrix
+            if r == 2 and i % 7 == 0:
+                u = 0.5 * u  # not unitary: stays a complex 2x2
+            gates.append(oracle.gate("Circuit", a, k=1, u=u))
+            lowered.append({"kind": hip.OP_MAT1, "targets": (a,), "controls": (), "m": u.reshape(4)})
+            continue
+        kind = ("Y", "H", "X", "RotZ", "Phase", "CX", "CZ", "CRotY")[r - 4]
+        g = oracle.gate(kind, a, b, c, angle=float(rng.uniform(-3, 3)))
+        gates.append(g)
+        lowered += ops.lower_all([g], exact=True)
+    sv = random_state(n, seed)
+    want = oracle.run(sv, n, gates)
+    for tile in (10, 12):
+        got, counts = emulate_ops(emu, n, sv, lowered, tile)
+        np.testing.assert_allclose(got, want, atol=1e-12, rtol=0)
+
+
+def test_tiled_u3_chain_with_diagonals(emu):
+    """A U3 chain on one qubit with RotZ and CZ between: the carried phases
+    pass the diagonal gates and end as phase terms after the last U3."""
+    n = 12
+    g = []
+    for k in range(6):
+        g.append(oracle.gate("U3", 3, angle=0.3 + k, phi=0.7 * k, lam=-0.2 * k))
+        g.append(oracle.gate("RotZ", 3, angle=0.1 * k + 0.05))
+        g.append(oracle.gate("CZ", 3, 4))
+    check(emu, n, g, seed=6)

@@ -674,8 +674,9 @@ class Tiler {
 // amplitude at the streaming rate), fitted to MI355X timings of brickwork,
 // random-layer and QFT circuits at 22..28 qubits (mean error 9%): a tile
 // pass costs its sweep (1.1) plus its in-register work -- a 2x2 product
-// 0.107 of a sweep, an exchange 0.034, a phase term 0.059 and each complex
-// product a phase run applies per thread 0.0016; an MFMA block 1.17-1.2, a
+// 0.107 of a sweep, a rotation with its phase (MATP) 0.066, an exchange
+// 0.034, a phase term 0.059 and each complex product a phase run applies
+// per thread 0.0016; an MFMA block 1.17-1.2, a
 // diagonal table 1.25 (0.05 when it rides in a neighbouring MFMA block), a
 // bit permutation 1.8, a plain one-qubit pass 1, a sparse pass its touched
 // fraction.
@@ -688,7 +689,7 @@ double pass_cost(const aqs_schedule_s* sch, size_t p) {
             if (o.code < 12) {
                 c += 0.107;
             } else if (o.code >= kTileCodeMatP) {
-                c += 0.080;
+                c += 0.066;
             } else if (o.code >= kTileCodeMatR) {
                 c += 0.054;
             } else if (!tile_code_is_phases(o.code)) {

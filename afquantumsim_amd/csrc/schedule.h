@@ -90,11 +90,12 @@ static_assert(sizeof(TileOpDev) == 64, "compact tile op");
 // Phase term: multiply by z where every bit of the three masks is set
 // (register slots, thread-held local positions, index bits outside the
 // tile).  Terms of one PHASES op are sorted by register mask; the first term
-// of each equal-mask group carries the group length, and real = 1 when
-// every z of the group is real (no folded term): the group's product then
-// scales amplitudes by a real factor.  z in FP32 here, in FP64 in
-// TilePass::z128[2 * term].  A folded term (fold >= 0) takes its z from the
-// per-tile value of fold `fold` instead.
+// of each equal-mask group carries the group length, the number of folded
+// terms (the group's last ones), and real = 1 when every z of the group is
+// real (no folded term): the group's product then scales amplitudes by a
+// real factor.  z in FP32 here, in FP64 in TilePass::z128[2 * term].  A
+// folded term (fold >= 0) takes its z from the per-tile value of fold
+// `fold` instead.
 struct TileTermDev {
     uint16_t m_reg;
     uint16_t group;
@@ -102,7 +103,8 @@ struct TileTermDev {
     uint64_t m_glb;
     float zf[2];
     int32_t fold;
-    uint32_t real;
+    uint16_t real;
+    uint16_t folded;
 };
 static_assert(sizeof(TileTermDev) == 32, "compact tile term");
 

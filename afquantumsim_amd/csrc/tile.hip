@@ -194,14 +194,14 @@ __device__ __forceinline__ void group_product(const TileTermDev* __restrict__ tt
                                               uint64_t base, R& pr, R& pi) {
     pr = R(1);
     pi = R(0);
-    for (uint32_t e = k; e < k + g; ++e) {
+    // plain terms (z from the program, scalar loads), then the folded ones
+    // (z from LDS); one loop with either source made the compiler select
+    // between the two address spaces and wait on a flat load per term
+    const uint32_t mid = k + g - tt[k].folded;
+    for (uint32_t e = k; e < mid; ++e) {
         const bool c = ((lthr & tt[e].m_thr) == tt[e].m_thr) && ((base & tt[e].m_glb) == tt[e].m_glb);
         R zr, zi;
-        if (tt[e].fold >= 0) {
-            const T2 v = fold_val[tt[e].fold];
-            zr = v.x;
-            zi = v.y;
-        } else if constexpr (sizeof(R) == 4) {
+        if constexpr (sizeof(R) == 4) {
             zr = tt[e].zf[0];
             zi = tt[e].zf[1];
         } else {
@@ -209,6 +209,11 @@ __device__ __forceinline__ void group_product(const TileTermDev* __restrict__ tt
             zi = z128[2 * e + 1];
         }
         cmul_acc(pr, pi, c ? zr : R(1), c ? zi : R(0));
+    }
+    for (uint32_t e = mid; e < k + g; ++e) {
+        const bool c = (lthr & tt[e].m_thr) == tt[e].m_thr;
+        const T2 v = fold_val[tt[e].fold];
+        cmul_acc(pr, pi, c ? v.x : R(1), c ? v.y : R(0));
     }
 }
 

@@ -467,10 +467,11 @@ class Tiler {
                 for (uint32_t k = 0; ok && k < o.term_count;) {
                     const TileTermDev& first = tp.terms[o.term_begin + k];
                     const uint32_t g = first.group;
-                    ok = g > 0 && k + g <= o.term_count && first.real <= 1;
-                    for (uint32_t e = k; ok && first.real && e < k + g; ++e) {
+                    ok = g > 0 && k + g <= o.term_count && first.real <= 1 && first.folded <= g;
+                    for (uint32_t e = k; ok && e < k + g; ++e) {
                         const size_t t = o.term_begin + e;
-                        ok = tp.terms[t].fold < 0 && tp.terms[t].zf[1] == 0.0f && tp.z128[2 * t + 1] == 0.0;
+                        ok = (tp.terms[t].fold >= 0) == (e >= k + g - first.folded);
+                        if (first.real) ok = ok && tp.terms[t].zf[1] == 0.0f && tp.z128[2 * t + 1] == 0.0;
                     }
                     k += g ? g : 1;
                 }
@@ -626,7 +627,11 @@ class Tiler {
         }
         std::vector<size_t> order(kept.size());
         for (size_t i = 0; i < order.size(); ++i) order[i] = i;
-        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return kept[a].reg < kept[b].reg; });
+        // by register mask; in a group the folded terms last (the kernel
+        // reads them from LDS in a loop of their own)
+        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+            return kept[a].reg != kept[b].reg ? kept[a].reg < kept[b].reg : fold_of[a] < 0 && fold_of[b] >= 0;
+        });
         std::vector<M> sorted_terms;
         std::vector<int32_t> sorted_fold;
         for (size_t i : order) {
@@ -650,12 +655,15 @@ class Tiler {
             if (i == gend) {
                 size_t e = i;
                 bool real = true;
+                uint32_t folded = 0;
                 while (e < merged.size() && merged[e].reg == merged[i].reg && e - i < 0xFFFF) {
                     real = real && sorted_fold[e] < 0 && merged[e].z.imag() == 0.0;
+                    folded += sorted_fold[e] >= 0 ? 1u : 0u;
                     ++e;
                 }
                 d.group = (uint16_t)(e - i);
-                d.real = real ? 1u : 0u;
+                d.real = real ? 1 : 0;
+                d.folded = (uint16_t)folded;
                 gend = e;
                 direct += 16.0 / (double)(1u << __builtin_popcount(merged[i].reg));
                 groups += 1;

@@ -197,22 +197,30 @@ __device__ __forceinline__ void group_product(const TileTermDev* __restrict__ tt
     // plain terms (z from the program, scalar loads), then the folded ones
     // (z from LDS); one loop with either source made the compiler select
     // between the two address spaces and wait on a flat load per term
+    // each term is read whole (two 16-byte scalar loads, one wait) and
+    // tested without short-circuit (a divergent branch per term otherwise)
     const uint32_t mid = k + g - tt[k].folded;
+    const uint4* w = reinterpret_cast<const uint4*>(tt);
+#pragma unroll 2
     for (uint32_t e = k; e < mid; ++e) {
-        const bool c = ((lthr & tt[e].m_thr) == tt[e].m_thr) && ((base & tt[e].m_glb) == tt[e].m_glb);
+        const uint4 a = w[2 * e], b = w[2 * e + 1];
+        const uint64_t m_glb = (uint64_t)a.z | ((uint64_t)a.w << 32);
+        const bool c = ((lthr & a.y) == a.y) & ((base & m_glb) == m_glb);
         R zr, zi;
         if constexpr (sizeof(R) == 4) {
-            zr = tt[e].zf[0];
-            zi = tt[e].zf[1];
+            zr = __uint_as_float(b.x);
+            zi = __uint_as_float(b.y);
         } else {
             zr = z128[2 * e];
             zi = z128[2 * e + 1];
         }
         cmul_acc(pr, pi, c ? zr : R(1), c ? zi : R(0));
     }
+#pragma unroll 2
     for (uint32_t e = mid; e < k + g; ++e) {
-        const bool c = (lthr & tt[e].m_thr) == tt[e].m_thr;
-        const T2 v = fold_val[tt[e].fold];
+        const uint4 a = w[2 * e], b = w[2 * e + 1];
+        const bool c = (lthr & a.y) == a.y;
+        const T2 v = fold_val[b.z];
         cmul_acc(pr, pi, c ? v.x : R(1), c ? v.y : R(0));
     }
 }

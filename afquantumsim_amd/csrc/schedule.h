@@ -5,6 +5,7 @@
 
 #include <array>
 #include <complex>
+#include <cstddef>
 #include <cstdint>
 #include <vector>
 
@@ -107,6 +108,9 @@ struct TileTermDev {
     uint16_t folded;
 };
 static_assert(sizeof(TileTermDev) == 32, "compact tile term");
+static_assert(offsetof(TileTermDev, m_thr) == 4 && offsetof(TileTermDev, m_glb) == 8 &&
+                  offsetof(TileTermDev, zf) == 16 && offsetof(TileTermDev, fold) == 24,
+              "the kernel reads a term as two 16-byte words");
 
 // Fold: the product of the phase terms that pair one local bit with bits
 // outside the tile, evaluated once per tile (it only depends on the tile's
@@ -120,6 +124,8 @@ struct TileFoldTermDev {
     uint64_t m_glb;
     float zf[2];
 };
+static_assert(sizeof(TileFoldTermDev) == 16 && offsetof(TileFoldTermDev, zf) == 8,
+              "the kernel reads a fold term as one 16-byte word");
 constexpr int kTileMaxFolds = 256;
 
 struct TileSegDev {

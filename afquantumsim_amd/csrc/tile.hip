@@ -401,12 +401,22 @@ __global__ __launch_bounds__(1 << W, (sizeof(T2) == 8 ? 4 : 2)) void k_tile(
         for (int f = tid; f < a.nfold; f += T) {
             R pr = R(1), pi = R(0);
             const TileFoldDev fd = folds[f];
+            // whole 16-byte fold terms, four loads in flight
+            const uint4* ftw = reinterpret_cast<const uint4*>(fold_terms);
+#pragma unroll 4
             for (uint32_t e = fd.begin; e < fd.begin + fd.count; ++e) {
-                if ((base & fold_terms[e].m_glb) != fold_terms[e].m_glb) continue;
-                if constexpr (sizeof(R) == 4)
-                    cmul_acc(pr, pi, fold_terms[e].zf[0], fold_terms[e].zf[1]);
-                else
-                    cmul_acc(pr, pi, fold_z128[2 * e], fold_z128[2 * e + 1]);
+                const uint4 a = ftw[e];
+                const uint64_t m = (uint64_t)a.x | ((uint64_t)a.y << 32);
+                const bool c = (base & m) == m;
+                R zr, zi;
+                if constexpr (sizeof(R) == 4) {
+                    zr = __uint_as_float(a.z);
+                    zi = __uint_as_float(a.w);
+                } else {
+                    zr = fold_z128[2 * e];
+                    zi = fold_z128[2 * e + 1];
+                }
+                cmul_acc(pr, pi, c ? zr : R(1), c ? zi : R(0));
             }
             T2 v;
             v.x = pr;

@@ -37,7 +37,7 @@ enum TileOpKind : int32_t {
     TILE_XCH = 1,     // exchange across register slot `slot`
     TILE_PHASES = 2,  // a run of diagonal gates as phase terms [term_begin, +term_count)
     TILE_MATR = 3,    // real 2x2 on register slot `slot` (half the multiplies of TILE_MAT)
-    TILE_MATP = 4,    // uncontrolled: the slot's 1-half times z, then a real 2x2 (3/4 of TILE_MAT)
+    TILE_MATP = 4,    // uncontrolled: the slot's 1-half times z, then a rotation (5/8 of TILE_MAT)
 };
 
 // Kernel dispatch code of an op (one switch per op, no divergent branch
@@ -84,7 +84,7 @@ struct TileOpDev {
     uint64_t c_glb;   // control index bits outside the tile
     uint32_t term_begin, term_count;
     float mf[8];      // MAT: row-major 2x2 in FP32 (complex); MATR: mf[0..3] = real row-major 2x2;
-                      // MATP: mf[0..3] real 2x2, mf[4..5] = z
+                      // MATP: mf[0] = t, mf[1] = s of the rotation's shears, mf[4..5] = z
 };
 static_assert(sizeof(TileOpDev) == 64, "compact tile op");
 

@@ -95,19 +95,22 @@ __device__ __forceinline__ void butterfly_real(T2& p, T2& q, const R (&m)[8]) {
     q = v;
 }
 
-// q times z (m[4], m[5]), then the real 2x2 m[0..3]: a one-qubit gate
-// factored as a rotation between phases, 12 multiplies per pair.
+// q times z (m[4], m[5]), then the rotation [[c, -s], [s, c]] (c, s >= 0)
+// as three shears, t = m[0] = s / (1 + c), s = m[1]: p -= t q, q += s p,
+// p -= t q.  A one-qubit gate factored as a rotation between phases in 10
+// multiply-adds per pair (a complex 2x2 takes 16).
 template <typename T2, typename R>
 __device__ __forceinline__ void butterfly_rotp(T2& p, T2& q, const R (&m)[8]) {
-    T2 w, u, v;
+    T2 w;
     w.x = m[4] * q.x - m[5] * q.y;
     w.y = m[4] * q.y + m[5] * q.x;
-    u.x = m[0] * p.x + m[1] * w.x;
-    u.y = m[0] * p.y + m[1] * w.y;
-    v.x = m[2] * p.x + m[3] * w.x;
-    v.y = m[2] * p.y + m[3] * w.y;
-    p = u;
-    q = v;
+    p.x -= m[0] * w.x;
+    p.y -= m[0] * w.y;
+    w.x += m[1] * p.x;
+    w.y += m[1] * p.y;
+    p.x -= m[0] * w.x;
+    p.y -= m[0] * w.y;
+    q = w;
 }
 
 // Pair operations across register slot SL (OP: 0 complex 2x2, 1 exchange,

@@ -223,8 +223,8 @@ class Tiler {
     // An uncontrolled one-qubit gate as U = diag(g0, g1) R diag(1, z) with
     // R = [[c, -s], [s, c]] real (c = |u00|, s = |u10|).  The bit's pending
     // diagonal diag(p0, p1) is applied first, so the op run here is
-    // R diag(1, z p1 / p0) (TILE_MATP: 12 real multiplies per amplitude pair
-    // against 16 for a complex 2x2) and p0 diag(g0, g1) becomes the pending
+    // R diag(1, z p1 / p0) (TILE_MATP: 10 real multiply-adds per amplitude
+    // pair against 16 for a complex 2x2) and p0 diag(g0, g1) becomes the pending
     // diagonal.  A qubit's phases between rotations then cost nothing.
     // Real matrices with nothing pending stay TILE_MATR.  The form holds
     // when u00 u11 / (u01 u10) is real and negative (every unitary; FP32
@@ -261,11 +261,20 @@ class Tiler {
         const cd ze = z * p1 / p0;
         GOp r = o;
         r.kind = ze == cd(1) ? TILE_MATR : TILE_MATP;
-        const double mm[8] = {c, -s, s, c, ze.real(), ze.imag(), 0, 0};
-        std::memcpy(r.m, mm, sizeof(r.m));
+        // MATR: the real 2x2; MATP: the shear factors t = s / (1 + c) and s
+        // of the unit rotation (c, s) / |(c, s)| (t <= 1 since c, s >= 0),
+        // then z; the norm |(c, s)| joins the pending diagonal
+        // (a unitary's norm, 1 up to rounding, is taken as exactly 1 so that
+        // trivial pending phases stay exactly trivial)
+        const double nr0 = std::sqrt(c * c + s * s), nr = std::abs(nr0 - 1.0) < 1e-15 ? 1.0 : nr0;
+        const double cu = c / nr, su = s / nr;
+        const double mr[8] = {c, -s, s, c, 0, 0, 0, 0};
+        const double mp[8] = {su / (1.0 + cu), su, 0, 0, ze.real(), ze.imag(), 0, 0};
+        std::memcpy(r.m, r.kind == TILE_MATR ? mr : mp, sizeof(r.m));
+        const double scale = r.kind == TILE_MATR ? 1.0 : nr;
         g_.push_back(std::move(r));
-        pend0_[b] = p0 * g0;
-        pend1_[b] = p0 * g1;
+        pend0_[b] = p0 * g0 * scale;
+        pend1_[b] = p0 * g1 * scale;
         return true;
     }
 

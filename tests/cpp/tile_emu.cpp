@@ -212,10 +212,14 @@ void tile_apply(std::vector<cplx>& v, int n, const TilePass& tp) {
                             x[j] = m[0] * p + m[1] * q;
                             x[k] = m[2] * p + m[3] * q;
                         } else if (rotp) {
+                            // phase, then the rotation as three shears
                             const double* m = &tp.m128[8 * (size_t)(S.op_begin + o)];
-                            const cplx p = x[j], q = x[k] * cplx(m[4], m[5]);
-                            x[j] = m[0] * p + m[1] * q;
-                            x[k] = m[2] * p + m[3] * q;
+                            cplx p = x[j], q = x[k] * cplx(m[4], m[5]);
+                            p -= m[0] * q;
+                            q += m[1] * p;
+                            p -= m[0] * q;
+                            x[j] = p;
+                            x[k] = q;
                         } else {
                             std::swap(x[j], x[k]);
                         }

@@ -688,12 +688,15 @@ class Tiler {
 };
 
 // Modelled cost of one pass in full-state sweeps (read + write of every
-// amplitude at the streaming rate), fitted to MI355X timings of brickwork,
-// random-layer and QFT circuits at 22..28 qubits (mean error 9%): a tile
-// pass costs its sweep (1.1) plus its in-register work -- a 2x2 product
-// 0.107 of a sweep, a rotation with its phase (MATP) 0.066, an exchange
-// 0.034, a phase term 0.059 and each complex product a phase run applies
-// per thread 0.0016; an MFMA block 1.17-1.2, a
+// amplitude at the streaming rate).  Tile passes are fitted to MI355X
+// timings of brickwork (depth 8, 20), random-layer (depth 10, 40) and QFT
+// circuits at 26..28 qubits (mean error 1.4%, at most 4%): a pass costs
+// 1.03 sweeps plus its in-register work -- a rotation with its phase (MATP)
+// 0.053 of a sweep, a real 2x2 0.05, an exchange 0.066, a phase term 0.02
+// and each product a phase run applies per thread 0.0045; a complex 2x2,
+// absent from those circuits, is priced at 0.085 (16 multiply-adds per
+// pair against a rotation's 10).  Other passes (fitted earlier at 22..28
+// qubits, mean error 9%): an MFMA block 1.17-1.2, a
 // diagonal table 1.25 (0.05 when it rides in a neighbouring MFMA block), a
 // bit permutation 1.8, a plain one-qubit pass 1, a sparse pass its touched
 // fraction.
@@ -701,16 +704,16 @@ double pass_cost(const aqs_schedule_s* sch, size_t p) {
     const aqs_op& op = sch->passes[p];
     if (sch->tile_index[p] >= 0) {
         const TilePass& t = sch->tiles[sch->tile_index[p]];
-        double c = 1.1;
+        double c = 1.03;
         for (const auto& o : t.ops) {
             if (o.code < 12) {
-                c += 0.107;
+                c += 0.085;
             } else if (o.code >= kTileCodeMatP) {
-                c += 0.066;
+                c += 0.053;
             } else if (o.code >= kTileCodeMatR) {
-                c += 0.054;
+                c += 0.05;
             } else if (!tile_code_is_phases(o.code)) {
-                c += 0.034;
+                c += 0.066;
             } else {
                 const bool zeta = o.code == kTileCodePhases + 1;
                 double products = zeta ? 48.0 : 0.0;
@@ -718,7 +721,7 @@ double pass_cost(const aqs_schedule_s* sch, size_t p) {
                     const TileTermDev& tt = t.terms[o.term_begin + k];
                     if (tt.group) products += zeta ? 1.0 : 16.0 / (double)(1u << __builtin_popcount(tt.m_reg));
                 }
-                c += 0.059 * o.term_count + 0.0016 * products;
+                c += 0.02 * o.term_count + 0.0045 * products;
             }
         }
         return c;

@@ -31,6 +31,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "internal.h"
@@ -826,21 +828,44 @@ aqs_status aqs_schedule_cost(aqs_schedule sch, double* sweeps) {
 }
 
 aqs_status aqs_fuse_auto(const aqs_op* ops, size_t count, int nqubits, aqs_schedule* out) {
+    if (!out) {
+        set_error("null output schedule");
+        return AQS_ERR_INVALID_ARGUMENT;
+    }
+    *out = nullptr;
     aqs_schedule blocks = nullptr, tiled = nullptr;
-    aqs_status st = aqs_fuse(ops, count, nqubits, 5, &blocks);
-    if (st != AQS_OK) return st;
     // below 2^20 amplitudes the state stays in the caches and launches of
     // the block schedule are cheaper than in-tile work (measured: at 19
     // qubits blocks win, at 20-21 tiles by 5-40% on brickwork and random
     // layers)
-    if (nqubits < kTileAutoMinQubits) {
-        *out = blocks;
-        return AQS_OK;
+    if (nqubits < kTileAutoMinQubits) return aqs_fuse(ops, count, nqubits, 5, out);
+    // both schedules, the tile one on a thread of its own (its error
+    // message, kept per thread, is carried back)
+    aqs_status tst = AQS_OK;
+    std::string terr;
+    std::thread worker;
+    bool threaded = false;
+    auto fuse_tiles = [&] {
+        tst = aqs_fuse_tiled(ops, count, nqubits, 0, &tiled);
+        if (tst != AQS_OK) terr = aqs_last_error();
+    };
+    try {
+        worker = std::thread(fuse_tiles);
+        threaded = true;
+    } catch (...) {
+        // no thread to be had: both run here
     }
-    st = aqs_fuse_tiled(ops, count, nqubits, 0, &tiled);
+    aqs_status st = aqs_fuse(ops, count, nqubits, 5, &blocks);
+    if (threaded) worker.join();
     if (st != AQS_OK) {
-        aqs_schedule_destroy(blocks);
+        if (tiled) aqs_schedule_destroy(tiled);
         return st;
+    }
+    if (!threaded) fuse_tiles();
+    if (tst != AQS_OK) {
+        aqs_schedule_destroy(blocks);
+        set_error(terr);
+        return tst;
     }
     const bool keep_tiled = schedule_cost(tiled) < schedule_cost(blocks);
     aqs_schedule_destroy(keep_tiled ? blocks : tiled);

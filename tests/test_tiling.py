@@ -185,6 +185,23 @@ def test_auto_mode():
         assert c.mode == ("tiled" if b.cost() < a.cost() else "blocks")
 
 
+def test_auto_mode_errors():
+    """aqs_fuse_auto fuses the tile schedule on a thread of its own: a bad
+    argument or op still reports its status and message on the caller's
+    thread, and leaves no schedule behind."""
+    L = hip.load()
+    arr, keep = hip.pack_ops(ops.lower_all(circuits.brickwork(22, 2, seed=1)))
+    assert L.aqs_fuse_auto(arr, len(arr), 22, None) == 1
+    assert b"null output" in L.aqs_last_error()
+    h = ctypes.c_void_p(1)
+    bad, keep2 = hip.pack_ops([{"kind": hip.OP_MAT1, "targets": (30,), "controls": (), "m": [1, 0, 0, 1]}])
+    assert L.aqs_fuse_auto(bad, 1, 22, ctypes.byref(h)) != 0
+    assert L.aqs_last_error() and h.value is None
+    # and a good call after them succeeds
+    assert L.aqs_fuse_auto(arr, len(arr), 22, ctypes.byref(h)) == 0 and h.value
+    assert L.aqs_schedule_destroy(h) == 0
+
+
 @pytest.mark.parametrize("tile", [10, 12])
 def test_tiled_folded_phase_terms(emu, tile):
     """Phase ladders between tile qubits and outer qubits are folded into one

@@ -84,6 +84,7 @@ struct MfmaArgs {
     uint32_t glane[64];   // float4 offset of lane l inside a chunk: pdep(2 l) / 2
     uint32_t tr[32];      // tile-index bits of target sub-index r
     uint32_t tc[32];      // tile-index bits of column c
+    uint32_t ptr[32];     // permuted blocks: tr under the tile-local permutation
     const float* apack;
     MfmaEpilogue epi;
     MfmaPerm perm;
@@ -193,7 +194,7 @@ __device__ __forceinline__ void prologue(const MfmaArgs& a, const float2* pg, co
 // into LDS.  Ends before the final barrier.
 template <int K, int NA, typename Args>
 __device__ __forceinline__ void tile_product(float2* lds, const float (&A)[NA], const Args& a, uint32_t rd_lane,
-                                             uint32_t wr_lane) {
+                                             uint32_t wr_lane, const uint32_t* wtr) {
     if (K == 32) {
         floatx16 t1 = {0}, t2 = {0}, t3 = {0};
 #pragma unroll
@@ -206,7 +207,7 @@ __device__ __forceinline__ void tile_product(float2* lds, const float (&A)[NA], 
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            const uint32_t tau = a.tr[8 * (r / 4) + (r % 4)] | wr_lane;
+            const uint32_t tau = wtr[8 * (r / 4) + (r % 4)] | wr_lane;
             lds[padded(tau)] = make_float2(t1[r] - t2[r], t3[r] - t1[r] - t2[r]);
         }
     } else {
@@ -220,7 +221,7 @@ __device__ __forceinline__ void tile_product(float2* lds, const float (&A)[NA], 
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
-            const uint32_t tau = a.tr[8 * (r / 4) + (r % 4)] | wr_lane;
+            const uint32_t tau = wtr[8 * (r / 4) + (r % 4)] | wr_lane;
             lds[padded(tau)] = make_float2(p[r] - q[r + 8], p[r + 8] + q[r]);
         }
     }
@@ -315,7 +316,7 @@ __global__ __launch_bounds__(64) void k_matk_mfma(MfmaArgs a) {
             for (int j = 0; j < CHUNKS; ++j) pf[j] = ld_nt(&a.sv4[n4 + a.gj[j] + glane]);
             __builtin_amdgcn_s_setprio(0);
         }
-        tile_product<K>(lds, A, a, rd_lane, wr_lane);
+        tile_product<K>(lds, A, a, rd_lane, wr_lane, a.tr);
         __syncthreads();
         // LDS -> tile (memory order)
 #pragma unroll
@@ -346,8 +347,9 @@ __device__ __forceinline__ uint64_t permute_index(uint64_t v, const int* map, in
 // and handles both tiles with one LDS buffer: tile A goes to LDS while tile
 // B's loads are in flight into registers, A's product is stored into B's
 // place (B is already read: every lane stores exactly the addresses it
-// loaded), then B's product into A's place.  The store gathers from LDS
-// through the tile-local permutation, so HBM writes stay coalesced runs.
+// loaded), then B's product into A's place.  The product's results are
+// written to LDS at their places under the tile-local permutation, and the
+// store reads LDS in memory order, so HBM writes stay coalesced runs.
 //
 // Which t a wave skips depends on t's low bits, so a plain grid stride (the
 // low bits fixed per wave) would leave some waves with nearly every pair and
@@ -367,7 +369,6 @@ __global__ __launch_bounds__(64) void k_matk_mfma_perm(MfmaArgs a) {
     constexpr int CHUNKS = TILE / 128;
     constexpr int LDS_F2 = TILE + TILE / 16;
     __shared__ __attribute__((aligned(16))) float2 lds[LDS_F2];
-    __shared__ uint16_t pmap[TILE];         // destination tile element -> source element
     __shared__ float2 pg[EPI ? 32 : 1];
     __shared__ float2 tin[EPI ? 32 * kTinRow : 1];
     const int lane = threadIdx.x;
@@ -375,7 +376,6 @@ __global__ __launch_bounds__(64) void k_matk_mfma_perm(MfmaArgs a) {
         for (int e = lane; e < (32 << a.epi.ng); e += 64)
             tin[(e >> 5) * kTinRow + (e & 31)] = a.epi.tab[((e >> 5) * a.epi.nc + a.epi.inside) * 32 + (e & 31)];
     }
-    for (int e = lane; e < TILE; e += 64) pmap[e] = (uint16_t)permute_index((uint64_t)e, a.perm.tmap, a.ntile_bits);
     const int col = lane & 31;
     const int half = lane >> 5;
     constexpr int NA = (K == 32) ? 48 : 8;
@@ -384,7 +384,11 @@ __global__ __launch_bounds__(64) void k_matk_mfma_perm(MfmaArgs a) {
     for (int i = 0; i < NA; ++i) A[i] = a.apack[i * 64 + lane];
     const uint32_t glane = a.glane[lane];
     const uint32_t rd_lane = a.tr[half] | a.tc[col];
-    const uint32_t wr_lane = a.tr[4 * half] | a.tc[col];
+    // results are written to LDS at their permuted places (the permutation
+    // acts bitwise, so it maps the row part and the lane part separately),
+    // and the store then reads LDS in memory order
+    const uint32_t wr_lane =
+        (uint32_t)permute_index((uint64_t)(a.tr[4 * half] | a.tc[col]), a.perm.tmap, a.ntile_bits);
     __syncthreads();
 
     const uint64_t G = gridDim.x, b = blockIdx.x, steps = a.tiles / G;
@@ -433,14 +437,13 @@ __global__ __launch_bounds__(64) void k_matk_mfma_perm(MfmaArgs a) {
 #pragma unroll
                 for (int j = 0; j < CHUNKS; ++j) pf[j] = ld_nt(&a.sv4[n4 + a.gj[j] + glane]);
             }
-            tile_product<K>(lds, A, a, rd_lane, wr_lane);
+            tile_product<K>(lds, A, a, rd_lane, wr_lane, a.ptr);
             __syncthreads();
             const uint64_t dst = (ntiles - 1 - s) ? b4b : b4a;
 #pragma unroll
             for (int j = 0; j < CHUNKS; ++j) {
-                const uint32_t e = 2u * (uint32_t)(j * 64 + lane);
-                const float2 v0 = lds[padded(pmap[e])], v1 = lds[padded(pmap[e + 1])];
-                st_nt(make_float4(v0.x, v0.y, v1.x, v1.y), &a.sv4[dst + a.gj[j] + glane]);
+                const uint32_t tau = 2u * (uint32_t)(j * 64 + lane);
+                st_nt(*reinterpret_cast<const float4*>(&lds[padded(tau)]), &a.sv4[dst + a.gj[j] + glane]);
             }
             __syncthreads();
         }
@@ -502,7 +505,7 @@ __global__ __launch_bounds__(kWideThreads) void k_matk_mfma_wide(MfmaWideArgs a)
         *reinterpret_cast<float4*>(&lds[padded(tau)]) = v[u];
     }
     __syncthreads();
-    tile_product<32>(lds, A, a, rd_lane, wr_lane);
+    tile_product<32>(lds, A, a, rd_lane, wr_lane, a.tr);
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < kWideUnits; ++u) {
@@ -658,6 +661,12 @@ bool apply_matk_mfma(aqs_state s, const aqs_op& op, const float* dev_apack, cons
         MfmaPerm& pm = a.perm;
         pm.nr = (int)R.size();
         for (size_t j = 0; j < T.size(); ++j) pm.tmap[j] = tidx((*beta)[T[j]]);
+        for (int r = 0; r < 32; ++r) {
+            uint32_t v = 0;
+            for (size_t j = 0; j < T.size(); ++j)
+                if ((a.tr[r] >> j) & 1u) v |= 1u << pm.tmap[j];
+            a.ptr[r] = v;
+        }
         for (size_t j = 0; j < R.size(); ++j)
             pm.rmap[j] = (int)(std::find(R.begin(), R.end(), (*beta)[R[j]]) - R.begin());
     }

@@ -217,12 +217,48 @@ __device__ __forceinline__ void apply_regs(float2 (&a)[kRegs], const BatchOp& e,
         return;
     }
     if (e.kind == AQS_OP_PHASE) {
-#pragma unroll
-        for (int k = 0; k < kRegs; ++k) {
-            const bool ok = p && ((uint32_t)k & e.preg) == e.preg;
-            const float zr = ok ? e.m[0] : 1.f, zi = ok ? e.m[1] : 0.f;
-            const float2 x = a[k];
-            a[k] = make_float2(zr * x.x - zi * x.y, zr * x.y + zi * x.x);
+        // one case per register mask: each multiplies its fixed set of
+        // registers (16 / 2^|mask| of them); a real factor (CZ, Z) scales
+        const float zr = p ? e.m[0] : 1.f, zi = p ? e.m[1] : 0.f;
+        if (e.m[1] == 0.f) {
+            switch (e.preg) {
+#define AQS_PREG_REAL(M)                                               \
+    case M:                                                            \
+        _Pragma("unroll") for (int k = 0; k < kRegs; ++k) if ((k & (M)) == (M)) { \
+            a[k].x *= zr;                                              \
+            a[k].y *= zr;                                              \
+        }                                                              \
+        break;
+                AQS_PREG_REAL(0) AQS_PREG_REAL(1) AQS_PREG_REAL(2) AQS_PREG_REAL(3)
+                AQS_PREG_REAL(4) AQS_PREG_REAL(5) AQS_PREG_REAL(6) AQS_PREG_REAL(7)
+                AQS_PREG_REAL(8) AQS_PREG_REAL(9) AQS_PREG_REAL(10) AQS_PREG_REAL(11)
+                AQS_PREG_REAL(12) AQS_PREG_REAL(13) AQS_PREG_REAL(14)
+#undef AQS_PREG_REAL
+                default:  // mask 15
+                    a[15].x *= zr;
+                    a[15].y *= zr;
+                    break;
+            }
+            return;
+        }
+        switch (e.preg) {
+#define AQS_PREG_CPLX(M)                                               \
+    case M:                                                            \
+        _Pragma("unroll") for (int k = 0; k < kRegs; ++k) if ((k & (M)) == (M)) { \
+            const float2 x = a[k];                                     \
+            a[k] = make_float2(zr * x.x - zi * x.y, zr * x.y + zi * x.x); \
+        }                                                              \
+        break;
+            AQS_PREG_CPLX(0) AQS_PREG_CPLX(1) AQS_PREG_CPLX(2) AQS_PREG_CPLX(3)
+            AQS_PREG_CPLX(4) AQS_PREG_CPLX(5) AQS_PREG_CPLX(6) AQS_PREG_CPLX(7)
+            AQS_PREG_CPLX(8) AQS_PREG_CPLX(9) AQS_PREG_CPLX(10) AQS_PREG_CPLX(11)
+            AQS_PREG_CPLX(12) AQS_PREG_CPLX(13) AQS_PREG_CPLX(14)
+#undef AQS_PREG_CPLX
+            default: {  // mask 15
+                const float2 x = a[15];
+                a[15] = make_float2(zr * x.x - zi * x.y, zr * x.y + zi * x.x);
+                break;
+            }
         }
         return;
     }

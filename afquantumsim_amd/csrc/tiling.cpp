@@ -36,6 +36,7 @@
 #include <vector>
 
 #include "internal.h"
+#include "rotation.h"
 #include "schedule.h"
 
 using namespace aqs_hip;
@@ -245,38 +246,22 @@ class Tiler {
             pend1_[b] = p1 * u11;
             return true;
         }
-        const double c = std::abs(u00), s = std::abs(u10);
-        if (s == 0) return false;
-        cd g0, g1 = u10 / s, z(1);
-        if (c >= s) {
-            g0 = u00 / c;
-            z = u11 / (g1 * c);
-        } else if (c > 0) {
-            g0 = u00 / c;
-            z = -u01 / (g0 * s);
-        } else {
-            g0 = -u01 / s;
-        }
-        const double err = std::max(std::max(std::abs(g0 * c - u00), std::abs(-g0 * s * z - u01)),
-                                    std::max(std::abs(g1 * s - u10), std::abs(g1 * c * z - u11)));
-        if (!(err <= 1e-12 * std::max(c, s))) return false;
-        const cd ze = z * p1 / p0;
+        RotationForm f;
+        if (!factor_rotation(u00, u01, u10, u11, f)) return false;
+        const cd ze = f.z * p1 / p0;
         GOp r = o;
         r.kind = ze == cd(1) ? TILE_MATR : TILE_MATP;
-        // MATR: the real 2x2; MATP: the shear factors t = s / (1 + c) and s
-        // of the unit rotation (c, s) / |(c, s)| (t <= 1 since c, s >= 0),
-        // then z; the norm |(c, s)| joins the pending diagonal
-        // (a unitary's norm, 1 up to rounding, is taken as exactly 1 so that
-        // trivial pending phases stay exactly trivial)
-        const double nr0 = std::sqrt(c * c + s * s), nr = std::abs(nr0 - 1.0) < 1e-15 ? 1.0 : nr0;
-        const double cu = c / nr, su = s / nr;
-        const double mr[8] = {c, -s, s, c, 0, 0, 0, 0};
-        const double mp[8] = {su / (1.0 + cu), su, 0, 0, ze.real(), ze.imag(), 0, 0};
+        // MATR: the real 2x2; MATP: the shears of the unit rotation, then z;
+        // the rotation's norm joins the pending diagonal
+        double t, su, nr;
+        rotation_shears(f.c, f.s, t, su, nr);
+        const double mr[8] = {f.c, -f.s, f.s, f.c, 0, 0, 0, 0};
+        const double mp[8] = {t, su, 0, 0, ze.real(), ze.imag(), 0, 0};
         std::memcpy(r.m, r.kind == TILE_MATR ? mr : mp, sizeof(r.m));
         const double scale = r.kind == TILE_MATR ? 1.0 : nr;
         g_.push_back(std::move(r));
-        pend0_[b] = p0 * g0 * scale;
-        pend1_[b] = p0 * g1 * scale;
+        pend0_[b] = p0 * f.g0 * scale;
+        pend1_[b] = p0 * f.g1 * scale;
         return true;
     }
 

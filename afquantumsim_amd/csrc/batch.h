@@ -16,8 +16,14 @@ namespace aqs_hip {
 // pmask are tile-local positions outside the segment's register bits;
 // gmask holds the controls (for a PHASE: the required bits) outside the tile
 // in state-index positions, resolved once per tile.
+// An uncontrolled one-qubit gate factored as a rotation after a phase: the
+// target's 1-half times z = (m[4], m[5]), then the rotation as three shears
+// p -= t q, q += s p, p -= t q with t = m[0], s = m[1].  The gate's diagonal
+// on the left is carried by the host into the qubit's next gate.
+constexpr int32_t kBatchOpMatP = 64;
+
 struct BatchOp {
-    int32_t kind;    // AQS_OP_MAT1 / DIAG1 / PHASE / X / SWAP
+    int32_t kind;    // AQS_OP_MAT1 / DIAG1 / PHASE / X / SWAP, kBatchOpMatP
     int32_t t0, t1;  // targets (t1 for SWAP, plain segments only)
     int32_t any;     // AQS_CTRL_ANY: at least one control bit must be set
     uint32_t cmask;  // control bits inside the tile, outside the register bits

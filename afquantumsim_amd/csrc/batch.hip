@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "batch.h"
+#include "rotation.h"
 
 using namespace aqs_hip;
 
@@ -97,6 +98,22 @@ __device__ void apply_block(float2* lds, uint32_t N, const BatchOp& op) {
                 a.y = op.m[0] * x.y + op.m[1] * x.x + op.m[2] * y.y + op.m[3] * y.x;
                 b.x = op.m[4] * x.x - op.m[5] * x.y + op.m[6] * y.x - op.m[7] * y.y;
                 b.y = op.m[4] * x.y + op.m[5] * x.x + op.m[6] * y.y + op.m[7] * y.x;
+                lds[lds_slot(i0)] = a;
+                lds[lds_slot(i1)] = b;
+            }
+            break;
+        }
+        case kBatchOpMatP: {
+            for (uint32_t p = tid; p < N / 2; p += BLOCK) {
+                const uint32_t i0 = ins0(p, op.t0), i1 = i0 | (1u << op.t0);
+                const float2 x = lds[lds_slot(i0)], y = lds[lds_slot(i1)];
+                float2 a = x, b = make_float2(op.m[4] * y.x - op.m[5] * y.y, op.m[4] * y.y + op.m[5] * y.x);
+                a.x -= op.m[0] * b.x;
+                a.y -= op.m[0] * b.y;
+                b.x += op.m[1] * a.x;
+                b.y += op.m[1] * a.y;
+                a.x -= op.m[0] * b.x;
+                a.y -= op.m[0] * b.y;
                 lds[lds_slot(i0)] = a;
                 lds[lds_slot(i1)] = b;
             }
@@ -187,6 +204,26 @@ __device__ __forceinline__ void reg_mat_c(float2 (&a)[kRegs], const float (&m)[8
     }
 }
 
+// A rotation after a phase (kBatchOpMatP) on slot S: no controls.
+template <int S>
+__device__ __forceinline__ void reg_matp(float2 (&a)[kRegs], const float (&m)[8]) {
+#pragma unroll
+    for (int k = 0; k < kRegs; ++k) {
+        if ((k >> S) & 1) continue;
+        float2 x = a[k];
+        const float2 y = a[k | (1 << S)];
+        float2 w = make_float2(m[4] * y.x - m[5] * y.y, m[4] * y.y + m[5] * y.x);
+        x.x -= m[0] * w.x;
+        x.y -= m[0] * w.y;
+        w.x += m[1] * x.x;
+        w.y += m[1] * x.y;
+        x.x -= m[0] * w.x;
+        x.y -= m[0] * w.y;
+        a[k] = x;
+        a[k | (1 << S)] = w;
+    }
+}
+
 // Exchange across slot S on registers whose slots `creg` are all set, on
 // lanes with p.
 template <int S>
@@ -259,6 +296,16 @@ __device__ __forceinline__ void apply_regs(float2 (&a)[kRegs], const BatchOp& e,
                 a[15] = make_float2(zr * x.x - zi * x.y, zr * x.y + zi * x.x);
                 break;
             }
+        }
+        return;
+    }
+    if (e.kind == kBatchOpMatP) {
+        const float m[8] = {e.m[0], e.m[1], 0.f, 0.f, e.m[4], e.m[5], 0.f, 0.f};
+        switch (e.t0) {
+            case 0: reg_matp<0>(a, m); break;
+            case 1: reg_matp<1>(a, m); break;
+            case 2: reg_matp<2>(a, m); break;
+            default: reg_matp<3>(a, m); break;
         }
         return;
     }
@@ -443,6 +490,7 @@ Uses uses_of(const BatchOp& o) {
     switch (o.kind) {
         case AQS_OP_MAT1:
         case AQS_OP_X:
+        case kBatchOpMatP:
             return {t0, t0, o.cmask};
         case AQS_OP_SWAP:
             return {t0 | (1u << o.t1), t0 | (1u << o.t1), o.cmask};
@@ -540,17 +588,61 @@ std::vector<BatchOp> batch_absorb(const aqs_op* ops, size_t count, int n) {
     std::vector<BatchOp> out;
     std::vector<M2> pending(n);
     std::vector<char> has(n, 0);
-    auto flush = [&](int q) {
-        if (!has[q]) return;
+    // the qubit's last rotation op (kBatchOpMatP) and its 2x2, to merge a
+    // pending diagonal back into it at the end instead of a separate op
+    std::vector<long> last_rot(n, -1);
+    std::vector<M2> rot_mat(n);
+    auto emit_mat = [&](int q, const M2& m) {
         BatchOp o{};
         o.kind = AQS_OP_MAT1;
         o.t0 = n - 1 - q;
         for (int i = 0; i < 4; ++i) {
-            o.m[2 * i] = (float)pending[q].a[i].real();
-            o.m[2 * i + 1] = (float)pending[q].a[i].imag();
+            o.m[2 * i] = (float)m.a[i].real();
+            o.m[2 * i + 1] = (float)m.a[i].imag();
         }
-        out.push_back(o);
+        return o;
+    };
+    // the whole pending 2x2, before a gate that uses the qubit otherwise
+    auto flush = [&](int q) {
+        last_rot[q] = -1;
+        if (!has[q]) return;
+        out.push_back(emit_mat(q, pending[q]));
         has[q] = 0;
+    };
+    // before a gate that uses the qubit only diagonally (a control, a phase):
+    // the pending 2x2 U = diag(g0, g1) R diag(1, z) runs as one rotation op
+    // and diag(g0, g1) stays pending (a diagonal commutes with that gate);
+    // 10 multiply-adds per amplitude pair instead of 16
+    auto flush_rotation = [&](int q) {
+        if (!has[q]) return;
+        const M2& u = pending[q];
+        if (u.a[1] == cd(0) && u.a[2] == cd(0)) return;  // diagonal: stays pending
+        RotationForm f;
+        if (!factor_rotation(u.a[0], u.a[1], u.a[2], u.a[3], f)) {
+            flush(q);
+            return;
+        }
+        double t, su, nr;
+        rotation_shears(f.c, f.s, t, su, nr);
+        BatchOp o{};
+        o.kind = kBatchOpMatP;
+        o.t0 = n - 1 - q;
+        o.m[0] = (float)t;
+        o.m[1] = (float)su;
+        o.m[4] = (float)f.z.real();
+        o.m[5] = (float)f.z.imag();
+        last_rot[q] = (long)out.size();
+        out.push_back(o);
+        // the op's 2x2: R diag(1, z) with R the unit rotation
+        const double cu = f.c / nr;
+        rot_mat[q].a[0] = cu;
+        rot_mat[q].a[1] = -su * f.z;
+        rot_mat[q].a[2] = su;
+        rot_mat[q].a[3] = cu * f.z;
+        pending[q].a[0] = f.g0 * nr;
+        pending[q].a[1] = 0;
+        pending[q].a[2] = 0;
+        pending[q].a[3] = f.g1 * nr;
     };
     for (size_t i = 0; i < count; ++i) {
         const aqs_op& op = ops[i];
@@ -569,10 +661,16 @@ std::vector<BatchOp> batch_absorb(const aqs_op* ops, size_t count, int n) {
             }
             continue;
         }
-        for (int j = 0; j < op.ntargets; ++j) flush(op.targets[j]);
-        for (int j = 0; j < op.ncontrols; ++j) flush(op.controls[j]);
+        const bool diagonal = op.kind == AQS_OP_PHASE || op.kind == AQS_OP_DIAG1;
+        for (int j = 0; j < op.ntargets; ++j) {
+            if (diagonal)
+                flush_rotation(op.targets[j]);
+            else
+                flush(op.targets[j]);
+        }
+        for (int j = 0; j < op.ncontrols; ++j) flush_rotation(op.controls[j]);
         for (int q = 0; q < n; ++q)
-            if ((op.more_controls >> q) & 1ull) flush(q);
+            if ((op.more_controls >> q) & 1ull) flush_rotation(q);
         BatchOp o{};
         o.kind = op.kind;
         o.t0 = n - 1 - op.targets[0];
@@ -588,7 +686,17 @@ std::vector<BatchOp> batch_absorb(const aqs_op* ops, size_t count, int n) {
         for (int j = 0; j < 8; ++j) o.m[j] = (float)op.m[j];
         out.push_back(o);
     }
-    for (int q = 0; q < n; ++q) flush(q);
+    for (int q = 0; q < n; ++q) {
+        if (has[q] && last_rot[q] >= 0 && pending[q].a[1] == cd(0) && pending[q].a[2] == cd(0)) {
+            // every op on the qubit since its last rotation is diagonal on
+            // it (the pending product was diagonal at each of them and is
+            // at the end), so they commute: the rotation takes the pending
+            // diagonal and runs as the whole 2x2
+            out[(size_t)last_rot[q]] = emit_mat(q, mul(pending[q], rot_mat[q]));
+            has[q] = 0;
+        }
+        flush(q);
+    }
     return out;
 }
 

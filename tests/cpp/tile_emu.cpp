@@ -26,6 +26,16 @@ using cplx = std::complex<double>;
 
 namespace {
 
+// A batched rotation op (kBatchOpMatP, no controls) on the pair (x, y):
+// y *= z, then the shears x -= t y, y += su x, x -= t y (m[0] = t,
+// m[1] = su, m[4..5] = z).
+void batch_rotation(cplx& x, cplx& y, const float* m) {
+    y *= cplx(m[4], m[5]);
+    x -= (double)m[0] * y;
+    y += (double)m[1] * x;
+    x -= (double)m[0] * y;
+}
+
 void plain_apply(std::vector<cplx>& v, int n, const aqs_op& op) {
     const uint64_t N = 1ull << n;
     auto bit = [&](int q) { return 1ull << (n - 1 - q); };
@@ -251,6 +261,10 @@ void batch_op_apply(std::vector<cplx>& tile, const BatchOp& op) {
             case AQS_OP_X:
                 if (!(i & t) && ok(i)) std::swap(tile[i], tile[i | t]);
                 break;
+            case kBatchOpMatP:
+                if (op.cmask) throw 1;
+                if (!(i & t)) batch_rotation(tile[i], tile[i | t], op.m);
+                break;
             case AQS_OP_DIAG1:
                 if (ok(i)) tile[i] *= (i & t) ? m11 : m00;
                 break;
@@ -297,6 +311,10 @@ void batch_reg_apply(cplx (&a)[16], const BatchOp& e, uint32_t base) {
                 break;
             case AQS_OP_X:
                 if (!(k & t)) std::swap(a[k], a[k | t]);
+                break;
+            case kBatchOpMatP:
+                if (e.creg || e.cmask) throw 1;
+                if (!(k & t)) batch_rotation(a[k], a[k | t], e.m);
                 break;
             case AQS_OP_DIAG1:
                 a[k] *= (k & t) ? m11 : m00;

@@ -238,6 +238,39 @@ def test_fused_tables_ride_mfma_blocks(name, n):
     np.testing.assert_allclose(got, want, atol=5e-6)
 
 
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_tables_ride_wide_blocks(seed):
+    """From 18 qubits a 5-qubit block runs on the wide tiles (its targets and
+    the 7 lowest other bits), also with a diagonal table after or before it:
+    two table chunks may then touch the tile.  QFT ladders without the final
+    swaps (the last block's table comes before it) and random phase gates
+    between them (anchors and required bits in and outside the tile),
+    against the oracle."""
+    n = 19
+    rng = np.random.default_rng(seed)
+    gates = circuits.fourier_transform(n, final_swaps=False)
+    for _ in range(60):
+        kind = str(rng.choice(["RotZ", "Phase", "CZ", "CPhase", "CRotZ", "H", "RotY"],
+                              p=[0.15, 0.15, 0.15, 0.25, 0.2, 0.05, 0.05]))
+        q = [int(x) for x in rng.choice(n, size=3, replace=False)]
+        gates.append(circuits.gate(kind, q[0], q[1], q[2], angle=float(rng.uniform(-3, 3))))
+    gates += circuits.fourier_transform(n, final_swaps=False)
+    sv = random_state(n, 40 + seed)
+    want = oracle.run(sv, n, gates, threads=8)
+    sch = hip.Schedule(ops.lower_all(gates), n, 5)
+    tables = sum(k == hip.OP_DIAGT for k, _ in sch.passes())
+    st = hip.State(n, hip.C64)
+    st.upload(sv.astype(st.dtype))
+    st.profile(True)
+    st.apply_schedule(sch)
+    launched = st.profile_read().get("DIAGT", (0.0, 0))[1]
+    got = st.download()
+    st.close()
+    assert tables >= 1 and launched < tables
+    assert oracle.infidelity(got, want) < 1e-9
+    np.testing.assert_allclose(got, want, atol=5e-6)
+
+
 @pytest.mark.parametrize("n", [3, 7, 12, 17])
 def test_fused_swaps_are_relabelled_bit_exact(n):
     """Swap-only circuits compile to at most two in-place bit-permutation

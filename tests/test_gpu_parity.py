@@ -240,11 +240,11 @@ def test_fused_tables_ride_mfma_blocks(name, n):
 
 @pytest.mark.parametrize("seed", [0, 1, 2])
 def test_tables_ride_wide_blocks(seed):
-    """From 18 qubits a 5-qubit block runs on the wide tiles (its targets and
-    the 7 lowest other bits), also with a diagonal table after or before it:
-    two table chunks may then touch the tile.  QFT ladders without the final
-    swaps (the last block's table comes before it) and random phase gates
-    between them (anchors and required bits in and outside the tile),
+    """A 5-qubit block that carries a diagonal table keeps the one-wave
+    tiles at 19 qubits, where plain blocks take the wide ones.  QFT ladders
+    without the final swaps (the last block's table comes before it) and
+    random phase gates between them (anchors and required bits in and
+    outside the tile; tables that cannot ride get a pass of their own),
     against the oracle."""
     n = 19
     rng = np.random.default_rng(seed)

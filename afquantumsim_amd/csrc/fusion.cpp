@@ -500,9 +500,16 @@ class Fuser {
         sch_->push_plain(op, std::move(mat));
     }
 
-    void close_touching(const std::vector<int>& S) {
+    // Closes the open blocks touching S; returns the qubits of the block
+    // emitted last (empty when none was).
+    std::vector<int> close_touching(const std::vector<int>& S) {
         auto hit = touching(S);
-        for (size_t j = hit.size(); j-- > 0;) close_block(hit[j]);
+        std::vector<int> last;
+        for (size_t j = hit.size(); j-- > 0;) {
+            last = open_[hit[j]].qubits;
+            close_block(hit[j]);
+        }
+        return last;
     }
 
     // Which of the open blocks `hit` (touching S) to keep open so that op
@@ -654,7 +661,7 @@ class Fuser {
         for (auto& t : terms_)
             for (int q : t.q)
                 if (std::find(Q.begin(), Q.end(), q) == Q.end()) Q.push_back(q);
-        close_touching(Q);
+        const std::vector<int> last = close_touching(Q);
         cd global = 1.0;
         std::vector<Term> left;
         for (auto& t : terms_) {
@@ -665,12 +672,20 @@ class Fuser {
         }
         terms_.clear();
         while (!left.empty() || global != cd(1.0)) {
-            // anchor group: the (up to) 5 qubits in the most terms
+            // anchor group: the (up to) 5 qubits in the most terms; among
+            // equals, those of the block emitted right before the table, so
+            // that the table can ride that block's store (the epilogue form
+            // of the MFMA block holds fewer registers than the prologue form
+            // of the next one)
             std::map<int, int> deg;
             for (auto& t : left)
                 for (int q : t.q) deg[q]++;
             std::vector<std::pair<int, int>> byd(deg.begin(), deg.end());
-            std::stable_sort(byd.begin(), byd.end(), [](auto& a, auto& b) { return a.second > b.second; });
+            auto in_last = [&](int q) { return std::find(last.begin(), last.end(), q) != last.end(); };
+            std::stable_sort(byd.begin(), byd.end(), [&](auto& a, auto& b) {
+                if (a.second != b.second) return a.second > b.second;
+                return in_last(a.first) && !in_last(b.first);
+            });
             std::vector<int> G;
             for (auto& p : byd)
                 if ((int)G.size() < 5) G.push_back(p.first);

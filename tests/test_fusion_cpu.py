@@ -50,6 +50,21 @@ def test_blocks_qft(emu, n):
     assert perms <= 2
 
 
+@pytest.mark.parametrize("n", [20, 30])
+def test_qft_tables_anchor_on_preceding_block(n):
+    """Every table of the QFT schedule takes as anchor the qubits of the
+    block right before it, so it can ride that block's store.  In the last
+    table every qubit is in equally many terms (those of the last two
+    blocks); that tie goes to the preceding block too."""
+    hip.load()
+    ps = hip.Schedule(ops.lower_all(circuits.fourier_transform(n, final_swaps=True)), n, 5).passes()
+    tables = [i for i, (k, _) in enumerate(ps) if k == hip.OP_DIAGT]
+    assert len(tables) == (n + 4) // 5 - 1
+    for i in tables:
+        assert ps[i - 1][0] == hip.OP_MATK
+        assert sorted(ps[i][1]) == sorted(ps[i - 1][1]), (i, ps[i][1], ps[i - 1][1])
+
+
 @pytest.mark.parametrize("n", [6, 11, 16])
 def test_blocks_inverse_qft(emu, n):
     """Phases-then-Hadamard ladders are hoisted into Hadamard-then-phases

@@ -26,6 +26,7 @@ using namespace aqs_hip;
 namespace {
 
 constexpr int kBlock = 256;
+static_assert(kBlock == 256, "the block reductions combine four waves");
 constexpr int kRedGrid = 1024;       // fixed reduction grid (deterministic order)
 constexpr int kSampleBlock = 1024;   // amplitudes per sampling block
 constexpr double kFixScale = 1152921504606846976.0;  // 2^60
@@ -88,21 +89,75 @@ __device__ T block_reduce_sum(T v, T* sh) {
     return r;
 }
 
-// MODE 0: sum |a|^2; 1: sum |a|^2 over (i & mask) != 0; 2: sum |a|^2 (n - 2 popcount(i))
+// Sums over the 64 lanes of a wave (a fixed shuffle tree, so the order is
+// fixed); lane 0 holds the result.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += (uint64_t)__shfl_xor((unsigned long long)v, o);
+    return v;
+}
+
+__device__ __forceinline__ int64_t wave_max(int64_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (int64_t)__shfl_xor((long long)v, o));
+    return v;
+}
+
+// The term of amplitude i with probability p in reduction MODE.
+template <int MODE>
+__device__ __forceinline__ double reduce_term(uint64_t i, double p, uint64_t mask, int n) {
+    if (MODE == 0) return p;
+    if (MODE == 1) return (i & mask) ? p : 0.0;
+    return p * (double)(n - 2 * (int)__popcll(i));
+}
+
+// MODE 0: sum |a|^2; 1: sum |a|^2 over (i & mask) != 0; 2: sum |a|^2 (n - 2 popcount(i)).
+// 16-byte loads (two complex64 or one complex128 amplitude), four in
+// flight per thread, four accumulators; the grid is fixed by N, so the
+// summation order is too.  The block's sum: waves by shuffles, then one
+// barrier.  (One 8-byte load per step and an LDS tree with a barrier per
+// level read a 30-qubit complex64 state at 3.9 TB/s.)
 template <typename T2, int MODE>
 __global__ __launch_bounds__(kBlock) void k_reduce(const T2* __restrict__ a, uint64_t N, uint64_t mask, int n,
                                                    double* __restrict__ partial) {
-    __shared__ double sh[kBlock];
-    double acc = 0.0;
-    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
-    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += stride) {
-        const double p = prob_of(a[i]);
-        if (MODE == 0) acc += p;
-        if (MODE == 1) acc += (i & mask) ? p : 0.0;
-        if (MODE == 2) acc += p * (double)(n - 2 * (int)__popcll(i));
+    __shared__ double sh[kBlock / 64];
+    constexpr int A = sizeof(T2) == 8 ? 2 : 1;  // amplitudes per 16-byte load
+    constexpr int U = 4;
+    const float4* av = reinterpret_cast<const float4*>(a);
+    const uint64_t NV = N / A;
+    double acc[U] = {0.0, 0.0, 0.0, 0.0};
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock * U;
+    for (uint64_t b0 = (uint64_t)blockIdx.x * kBlock * U + threadIdx.x; b0 < NV; b0 += stride) {
+        float4 v[U];
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+            const uint64_t k = b0 + (uint64_t)j * kBlock;
+            if (k < NV) v[j] = ld_nt(&av[k]);
+        }
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+            const uint64_t k = b0 + (uint64_t)j * kBlock;
+            if (k >= NV) continue;
+            if (A == 2) {
+                acc[j] += reduce_term<MODE>(2 * k, prob_of(make_float2(v[j].x, v[j].y)), mask, n);
+                acc[j] += reduce_term<MODE>(2 * k + 1, prob_of(make_float2(v[j].z, v[j].w)), mask, n);
+            } else {
+                double2 d;
+                __builtin_memcpy(&d, &v[j], sizeof(d));
+                acc[j] += reduce_term<MODE>(k, prob_of(d), mask, n);
+            }
+        }
     }
-    double r = block_reduce_sum(acc, sh);
-    if (threadIdx.x == 0) partial[blockIdx.x] = r;
+    const double r = wave_sum((acc[0] + acc[1]) + (acc[2] + acc[3]));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = r;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
 }
 
 // <a|b> = sum conj(a_i) b_i
@@ -137,34 +192,56 @@ __global__ __launch_bounds__(kBlock) void k_reduce_final(const double* __restric
     }
 }
 
-template <typename T2, typename R>
-__global__ __launch_bounds__(kBlock) void k_scale(T2* __restrict__ a, uint64_t N, R f) {
-    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
-    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += stride) {
-        T2 v = a[i];
-        v.x *= f;
-        v.y *= f;
-        a[i] = v;
+// The amplitudes of 16-byte unit k of a state (two complex64 amplitudes,
+// 2k and 2k + 1, or one complex128 amplitude k) times f where keep(index)
+// holds, exactly 0 where it does not.
+template <typename T2, typename R, typename Keep>
+__device__ __forceinline__ float4 scale_unit(float4 v, uint64_t k, R f, Keep keep) {
+    if constexpr (sizeof(T2) == 8) {
+        const bool k0 = keep(2 * k), k1 = keep(2 * k + 1);
+        return make_float4(k0 ? v.x * f : 0.f, k0 ? v.y * f : 0.f, k1 ? v.z * f : 0.f, k1 ? v.w * f : 0.f);
+    } else {
+        double2 d;
+        __builtin_memcpy(&d, &v, sizeof(d));
+        const bool k0 = keep(k);
+        d.x = k0 ? d.x * f : 0.0;
+        d.y = k0 ? d.y * f : 0.0;
+        __builtin_memcpy(&v, &d, sizeof(d));
+        return v;
     }
+}
+
+// Every amplitude through scale_unit: one 16-byte unit per lane, one-wave
+// workgroups (the shape of the streaming gate passes).  (One 8-byte
+// amplitude per step in a grid-stride loop moved a 30-qubit complex64 state
+// at 4.5 TB/s, four 16-byte units per step at 4.8.)
+constexpr int kUnitBlock = 64;
+
+template <typename T2, typename R, typename Keep>
+__device__ __forceinline__ void scale_pass(T2* a, uint64_t N, R f, Keep keep) {
+    constexpr int A = sizeof(T2) == 8 ? 2 : 1;
+    float4* av = reinterpret_cast<float4*>(a);
+    const uint64_t k = (uint64_t)blockIdx.x * kUnitBlock + threadIdx.x;
+    if (k < N / A) st_nt(scale_unit<T2>(ld_nt(&av[k]), k, f, keep), &av[k]);
+}
+
+// Workgroups of scale_pass over a state of N amplitudes.
+unsigned unit_grid(uint64_t N, int precision) {
+    const uint64_t units = precision == AQS_C64 ? N / 2 : N;
+    return (unsigned)std::max<uint64_t>((units + kUnitBlock - 1) / kUnitBlock, 1);
+}
+
+template <typename T2, typename R>
+__global__ __launch_bounds__(kUnitBlock) void k_scale(T2* __restrict__ a, uint64_t N, R f) {
+    scale_pass(a, N, f, [](uint64_t) { return true; });
 }
 
 // Keep amplitudes whose bit matches `keep_one`, scaled by f; zero the rest.
 template <typename T2, typename R>
-__global__ __launch_bounds__(kBlock) void k_collapse_qubit(T2* __restrict__ a, uint64_t N, uint64_t mask, int keep_one,
+__global__ __launch_bounds__(kUnitBlock) void k_collapse_qubit(T2* __restrict__ a, uint64_t N, uint64_t mask, int keep_one,
                                                            R f) {
-    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
-    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += stride) {
-        const bool one = (i & mask) != 0;
-        T2 v = a[i];
-        if (one == (keep_one != 0)) {
-            v.x *= f;
-            v.y *= f;
-        } else {
-            v.x = 0;
-            v.y = 0;
-        }
-        a[i] = v;
-    }
+    const bool want = keep_one != 0;
+    scale_pass(a, N, f, [=](uint64_t i) { return ((i & mask) != 0) == want; });
 }
 
 template <typename T2>
@@ -182,52 +259,52 @@ __global__ __launch_bounds__(kBlock) void k_probs(const T2* __restrict__ a, uint
 // with non-zero quantised probability in block b (-1 when none).  Each
 // thread reads 16 bytes per load: complex64 amplitude pairs, complex128
 // single amplitudes.
+// One wavefront per sampling block (kSampleBlock amplitudes, 8 or 16
+// 16-byte loads per lane), four per workgroup: shuffles reduce the block's
+// integer sum and last non-zero index, with no LDS and no barrier (exact in
+// any order).  (A 256-thread workgroup per block with LDS trees read a
+// 30-qubit complex64 state at 4.9 TB/s.)
 template <typename T2>
-__global__ __launch_bounds__(kBlock) void k_block_sums(const T2* __restrict__ a, uint64_t N, uint64_t* __restrict__ sums,
-                                                       int64_t* __restrict__ lastnz) {
-    __shared__ uint64_t sh[kBlock];
-    __shared__ int64_t shl[kBlock];
-    const uint64_t b = blockIdx.x;
+__global__ __launch_bounds__(kBlock) void k_block_sums(const T2* __restrict__ a, uint64_t N, uint64_t nb,
+                                                       uint64_t* __restrict__ sums, int64_t* __restrict__ lastnz) {
+    constexpr int A = sizeof(T2) == 8 ? 2 : 1;       // amplitudes per 16-byte unit
+    constexpr int UPB = kSampleBlock / A;            // units per block
+    constexpr int UPL = UPB / 64;                    // units per lane
+    const uint64_t b = (uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    if (b >= nb) return;
+    const int lane = threadIdx.x & 63;
+    const float4* av = reinterpret_cast<const float4*>(a);
+    const uint64_t NV = N / A;
+    float4 v[UPL];
+#pragma unroll
+    for (int j = 0; j < UPL; ++j) {
+        const uint64_t k = b * UPB + (uint64_t)j * 64 + lane;
+        v[j] = k < NV ? ld_nt(&av[k]) : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
     uint64_t acc = 0;
     int64_t last = -1;
-    if constexpr (sizeof(T2) == 8) {
-        constexpr int kPairs = kSampleBlock / 2 / kBlock;  // float4 loads per thread
-        const float4* a4 = reinterpret_cast<const float4*>(a);
-        float4 v[kPairs];
 #pragma unroll
-        for (int j = 0; j < kPairs; ++j) {
-            const uint64_t i = b * kSampleBlock + 2 * ((uint64_t)j * kBlock + threadIdx.x);
-            v[j] = i < N ? a4[i >> 1] : make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-#pragma unroll
-        for (int j = 0; j < kPairs; ++j) {
-            const uint64_t i = b * kSampleBlock + 2 * ((uint64_t)j * kBlock + threadIdx.x);
+    for (int j = 0; j < UPL; ++j) {
+        const uint64_t k = b * UPB + (uint64_t)j * 64 + lane;
+        if (A == 2) {
             const uint64_t q0 = quantise(prob_of(make_float2(v[j].x, v[j].y)));
             const uint64_t q1 = quantise(prob_of(make_float2(v[j].z, v[j].w)));
             acc += q0 + q1;
-            if (q0) last = max(last, (int64_t)i);
-            if (q1) last = max(last, (int64_t)i + 1);
-        }
-    } else {
-        for (int j = 0; j < kSampleBlock / kBlock; ++j) {
-            const uint64_t i = b * kSampleBlock + (uint64_t)j * kBlock + threadIdx.x;
-            if (i < N) {
-                const uint64_t q = quantise(prob_of(a[i]));
-                acc += q;
-                if (q) last = (int64_t)i;
-            }
+            if (q0) last = max(last, (int64_t)(2 * k));
+            if (q1) last = max(last, (int64_t)(2 * k + 1));
+        } else {
+            double2 d;
+            __builtin_memcpy(&d, &v[j], sizeof(d));
+            const uint64_t q = quantise(prob_of(d));
+            acc += q;
+            if (q) last = max(last, (int64_t)k);
         }
     }
-    uint64_t s = block_reduce_sum(acc, sh);
-    shl[threadIdx.x] = last;
-    __syncthreads();
-    for (int st = kBlock / 2; st > 0; st >>= 1) {
-        if (threadIdx.x < st) shl[threadIdx.x] = max(shl[threadIdx.x], shl[threadIdx.x + st]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        sums[b] = s;
-        lastnz[b] = shl[0];
+    acc = wave_sum(acc);
+    last = wave_max(last);
+    if (lane == 0) {
+        sums[b] = acc;
+        lastnz[b] = last;
     }
 }
 
@@ -410,12 +487,13 @@ SampleTables sample_tables(void* base, uint64_t nb) {
 void launch_sampling(const void* amps, int precision, uint64_t N, const SampleTables& t, const double* du,
                      uint32_t shots, uint64_t* dout, hipStream_t stream) {
     const uint64_t nb = (N + kSampleBlock - 1) / kSampleBlock;
+    const unsigned gb = (unsigned)((nb + kBlock / 64 - 1) / (kBlock / 64));
     if (precision == AQS_C64)
-        hipLaunchKernelGGL(k_block_sums<float2>, dim3((unsigned)nb), dim3(kBlock), 0, stream, (const float2*)amps, N,
-                           t.sums, t.lastnz);
+        hipLaunchKernelGGL(k_block_sums<float2>, dim3(gb), dim3(kBlock), 0, stream, (const float2*)amps, N, nb, t.sums,
+                           t.lastnz);
     else
-        hipLaunchKernelGGL(k_block_sums<double2>, dim3((unsigned)nb), dim3(kBlock), 0, stream, (const double2*)amps,
-                           N, t.sums, t.lastnz);
+        hipLaunchKernelGGL(k_block_sums<double2>, dim3(gb), dim3(kBlock), 0, stream, (const double2*)amps, N, nb,
+                           t.sums, t.lastnz);
     const uint64_t nc = scan_chunks(nb);
     hipLaunchKernelGGL(k_scan_chunks, dim3((unsigned)nc), dim3(kScanChunk), 0, stream, (const uint64_t*)t.sums,
                        (const int64_t*)t.lastnz, nb, t.incl, t.ctot, t.clast);
@@ -500,14 +578,14 @@ void queue_sample_norm(const void* amps, int n, void* scratch, const double* du,
 }
 
 void scale_state(aqs_state s, double factor) {
-    const unsigned g = red_grid(s->N) * 4;
+    const unsigned g = unit_grid(s->N, s->precision);
     ProfScope ps(s, 0);
     if (s->precision == AQS_C64)
-        hipLaunchKernelGGL((k_scale<float2, float>), dim3(g), dim3(kBlock), 0, s->stream, (float2*)s->amps, s->N,
+        hipLaunchKernelGGL((k_scale<float2, float>), dim3(g), dim3(kUnitBlock), 0, s->stream, (float2*)s->amps, s->N,
                            (float)factor);
     else
-        hipLaunchKernelGGL((k_scale<double2, double>), dim3(g), dim3(kBlock), 0, s->stream, (double2*)s->amps, s->N,
-                           factor);
+        hipLaunchKernelGGL((k_scale<double2, double>), dim3(g), dim3(kUnitBlock), 0, s->stream, (double2*)s->amps,
+                           s->N, factor);
     AQS_HIP(hipGetLastError());
 }
 
@@ -636,13 +714,13 @@ aqs_status aqs_measure_qubit(aqs_state s, int qubit, double u, int* outcome, dou
         const double p = one ? p1 : 1.0 - p1;
         if (!(p > 0.0)) fail(AQS_ERR_RUNTIME, "measured outcome has zero probability");
         const double f = 1.0 / std::sqrt(p);
-        const unsigned g = red_grid(s->N) * 4;
+        const unsigned g = unit_grid(s->N, s->precision);
         ProfScope ps(s, 0);
         if (s->precision == AQS_C64)
-            hipLaunchKernelGGL((k_collapse_qubit<float2, float>), dim3(g), dim3(kBlock), 0, s->stream, (float2*)s->amps,
-                               s->N, mask, one, (float)f);
+            hipLaunchKernelGGL((k_collapse_qubit<float2, float>), dim3(g), dim3(kUnitBlock), 0, s->stream,
+                               (float2*)s->amps, s->N, mask, one, (float)f);
         else
-            hipLaunchKernelGGL((k_collapse_qubit<double2, double>), dim3(g), dim3(kBlock), 0, s->stream,
+            hipLaunchKernelGGL((k_collapse_qubit<double2, double>), dim3(g), dim3(kUnitBlock), 0, s->stream,
                                (double2*)s->amps, s->N, mask, one, f);
         AQS_HIP(hipGetLastError());
         if (outcome) *outcome = one;

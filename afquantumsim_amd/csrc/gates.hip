@@ -165,18 +165,48 @@ __global__ __launch_bounds__(64) void k_mat1_lx(Mat1V4Args a) {
     st_nt(n, &a.sv[f]);
 }
 
+// Variant 4's shape (one unit per lane, one-wave workgroups) with
+// consecutive workgroups taking their units from F equal parts of the unit
+// range in turn, so that the pass advances on F fronts (2F streams with the
+// partners) instead of one.
+template <int F>
+__global__ __launch_bounds__(64) void k_mat1_fronts(Mat1V4Args a) {
+    const uint32_t G = gridDim.x, b = blockIdx.x;
+    const uint32_t c = (b % F) * (G / F) + b / F;  // a bijection of 0..G-1 (F divides G)
+    const uint32_t u = c * 64u + threadIdx.x;
+    if (u >= (uint32_t)a.units) return;
+    const uint32_t idx = insert_zeros_t<uint32_t>(u, a.fixed) | (uint32_t)a.cmask;
+    const uint32_t stride = (uint32_t)a.stride;
+    const float4 x = ld_nt(&a.sv[idx]), y = ld_nt(&a.sv[idx + stride]);
+    const float m0r = a.m[0], m0i = a.m[1], m1r = a.m[2], m1i = a.m[3];
+    const float m2r = a.m[4], m2i = a.m[5], m3r = a.m[6], m3i = a.m[7];
+    float4 nx, ny;
+    nx.x = m0r * x.x - m0i * x.y + m1r * y.x - m1i * y.y;
+    nx.y = m0r * x.y + m0i * x.x + m1r * y.y + m1i * y.x;
+    ny.x = m2r * x.x - m2i * x.y + m3r * y.x - m3i * y.y;
+    ny.y = m2r * x.y + m2i * x.x + m3r * y.y + m3i * y.x;
+    nx.z = m0r * x.z - m0i * x.w + m1r * y.z - m1i * y.w;
+    nx.w = m0r * x.w + m0i * x.z + m1r * y.w + m1i * y.z;
+    ny.z = m2r * x.z - m2i * x.w + m3r * y.z - m3i * y.w;
+    ny.w = m2r * x.w + m2i * x.z + m3r * y.w + m3i * y.z;
+    st_nt(nx, &a.sv[idx]);
+    st_nt(ny, &a.sv[idx + stride]);
+}
+
 // Launch-shape variants of the float4 MAT1 pass: (units per thread, index
 // type, block size) = 0 (1, 32-bit, 256), 1 (2, 64-bit, 256), 2 (4, 64-bit,
 // 256), 3 (2, 32-bit, 256), 4 (1, 32-bit, 64), 5 (2, 32-bit, 64); 6 = the
 // lane-exchange pass (uncontrolled, partner 1..32 float4s away, 2^6..2^31
-// float4s; otherwise variant 4 or 1).
-constexpr int kMat1Variants = 7;
+// float4s; otherwise variant 4 or 1); 7, 8 = variant 4 on 2 and 4 fronts
+// (when the fronts divide the grid; otherwise variant 4).
+constexpr int kMat1Variants = 9;
 
 void launch_mat1_v4(const Mat1V4Args& a, int variant, uint64_t n_float4, hipStream_t st) {
     const bool fits32 = n_float4 <= (1ull << 32);
     if (variant == 6 && !(a.cmask == 0 && a.fixed.n == 1 && a.stride <= 32 && n_float4 >= 64 &&
                           n_float4 <= (1ull << 31) && 2 * a.units == n_float4))
         variant = 4;
+    if ((variant == 7 || variant == 8) && blocks_for(a.units, 1, 64) % (variant == 7 ? 2u : 4u) != 0) variant = 4;
     if (!fits32 && variant != 1 && variant != 2) variant = 1;
     switch (variant) {
         case 0:
@@ -196,6 +226,12 @@ void launch_mat1_v4(const Mat1V4Args& a, int variant, uint64_t n_float4, hipStre
             break;
         case 6:
             hipLaunchKernelGGL((k_mat1_lx<uint32_t>), dim3((unsigned)(n_float4 / 64)), dim3(64), 0, st, a);
+            break;
+        case 7:
+            hipLaunchKernelGGL((k_mat1_fronts<2>), dim3(blocks_for(a.units, 1, 64)), dim3(64), 0, st, a);
+            break;
+        case 8:
+            hipLaunchKernelGGL((k_mat1_fronts<4>), dim3(blocks_for(a.units, 1, 64)), dim3(64), 0, st, a);
             break;
         default:
             hipLaunchKernelGGL((k_mat1_v4<2, uint32_t, 64>), dim3(blocks_for(a.units, 2, 64)), dim3(64), 0, st, a);

@@ -279,7 +279,7 @@ class State:
         return list(out)
 
     def tune_set(self, qubit: int, variant: int):
-        """Pin the one-qubit launch shape (0..6) for this size and target qubit."""
+        """Pin the one-qubit launch shape (0..8) for this size and target qubit."""
         check(_lib.aqs_tune_set(self.h, qubit, variant))
 
     def apply_schedule(self, sch: "Schedule"):

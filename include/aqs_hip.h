@@ -157,13 +157,14 @@ aqs_status aqs_apply(aqs_state s, const aqs_op* ops, size_t count);
  * the choice per target qubit. */
 aqs_status aqs_autotune(aqs_state s, int reps, int* variant_by_qubit);
 
-/* Pins the one-qubit launch shape (0..6) for this state's qubit count and
+/* Pins the one-qubit launch shape (0..8) for this state's qubit count and
  * one target qubit on the current device, as aqs_autotune would store it.
  * Shapes: 0 (1 unit per thread, 32-bit index, 256 threads), 1 (2, 64-bit,
  * 256), 2 (4, 64-bit, 256), 3 (2, 32-bit, 256), 4 (1, 32-bit, 64),
  * 5 (2, 32-bit, 64), 6 (lane exchange: uncontrolled gates on target bits
- * 1..6 of 7..32-qubit states, else shape 4); 32-bit shapes fall back to 1
- * above 33 qubits. */
+ * 1..6 of 7..32-qubit states, else shape 4), 7 and 8 (shape 4 with
+ * consecutive workgroups spread over 2 and 4 parts of the state); 32-bit
+ * shapes fall back to 1 above 33 qubits. */
 aqs_status aqs_tune_set(aqs_state s, int qubit, int variant);
 
 /* ---- compile(): gate fusion (replaces QCircuit::compile, src/quantum.cpp:199-210) */

@@ -559,7 +559,7 @@ def test_autotune_keeps_state_and_results():
     st = hip.State(n, hip.C64)
     st.upload(sv)
     variants = st.autotune(reps=2)
-    assert len(variants) == n and all(0 <= v < 7 for v in variants)
+    assert len(variants) == n and all(0 <= v < 9 for v in variants)
     np.testing.assert_array_equal(st.download(), sv)
     gates = [oracle.gate("U3", q, angle=0.3 + q, phi=0.1 * q, lam=-0.2) for q in range(n)]
     want = oracle.run(sv.astype(np.complex128), n, gates)
@@ -576,14 +576,14 @@ def test_every_launch_shape(n):
     gates += [oracle.gate("CRotY", q, (q + 1) % n, angle=0.7 - 0.1 * q) for q in range(n)]
     want = oracle.run(sv.astype(np.complex128), n, gates)
     st = hip.State(n, hip.C64)
-    for v in range(7):
+    for v in range(9):
         for q in range(n):
             st.tune_set(q, v)
         st.upload(sv)
         st.apply(ops.lower_all(gates))
         np.testing.assert_allclose(st.download(), want, atol=TOL64, err_msg=f"variant {v}")
     with pytest.raises(hip.AqsError):
-        st.tune_set(0, 7)
+        st.tune_set(0, 9)
     with pytest.raises(hip.AqsError):
         st.tune_set(n, 0)
     for q in range(n):

@@ -461,6 +461,12 @@ def test_sampling_bit_exact(n):
     got = st.sample(u)
     want = oracle.sample_c64(st.download(), n, u)
     assert np.array_equal(got.astype(np.uint32), want)
+    # complex128: one amplitude per 16-byte load in the block sums
+    st128 = hip.State(n, hip.C128)
+    st128.upload(random_state(n, 50 + n))
+    got = st128.sample(u)
+    assert np.array_equal(got.astype(np.uint32), oracle.sample_c128(st128.download(), n, u))
+    st128.close()
 
 
 def test_sampling_sparse_and_clamp():
@@ -474,11 +480,12 @@ def test_sampling_sparse_and_clamp():
     assert list(st.sample(u)) == [5, 5, 3000, 3000, 3000, 3000]
 
 
-def test_measurement_reductions():
+@pytest.mark.parametrize("precision", [hip.C64, hip.C128], ids=["c64", "c128"])
+def test_measurement_reductions(precision):
     n = 11
-    sv = random_state(n, 77)
-    st = hip.State(n, hip.C128)
-    st.upload(sv)
+    st = hip.State(n, precision)
+    sv = random_state(n, 77).astype(st.dtype).astype(np.complex128)  # the values the state holds
+    st.upload(sv.astype(st.dtype))
     assert abs(st.norm2() - oracle.norm2(sv, n)) < 1e-13
     for q in range(n):
         assert abs(st.prob1(q) - oracle.prob1(sv, n, q)) < 1e-13
@@ -488,13 +495,18 @@ def test_measurement_reductions():
     np.testing.assert_allclose(p, np.abs(sv) ** 2, rtol=1e-6, atol=1e-12)
 
 
-def test_measure_qubit_collapse():
+@pytest.mark.parametrize("precision", [hip.C64, hip.C128], ids=["c64", "c128"])
+def test_measure_qubit_collapse(precision):
+    """Qubit n-1 (bit 0) splits each 16-byte pair of complex64 amplitudes
+    between kept and zeroed; the zeroed ones are exactly +0."""
     n = 6
-    sv = random_state(n, 12)
+    tol = 1e-13 if precision == hip.C128 else 2e-7
     for q in (0, 3, 5):
         for u in (0.01, 0.99):
-            st = hip.State(n, hip.C128)
+            st = hip.State(n, precision)
+            sv = random_state(n, 12).astype(st.dtype)
             st.upload(sv)
+            sv = sv.astype(np.complex128)
             outcome, p1 = st.measure_qubit(q, u)
             assert abs(p1 - oracle.prob1(sv, n, q)) < 1e-13
             assert outcome == int(u < p1)
@@ -502,7 +514,9 @@ def test_measure_qubit_collapse():
             idx = np.arange(1 << n)
             keep = ((idx & m) != 0) == bool(outcome)
             ref = np.where(keep, sv, 0) / math.sqrt(p1 if outcome else 1 - p1)
-            np.testing.assert_allclose(st.download(), ref, atol=1e-13)
+            got = st.download()
+            np.testing.assert_allclose(got, ref, atol=tol)
+            assert not got[~keep].view(np.uint32 if precision == hip.C64 else np.uint64).any()
 
 
 def test_state_preparation():

@@ -613,7 +613,7 @@ def run(args):
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic.get("bytes_per_launch") if traffic else None,
-                "kernel": "one-qubit U3 pass (k_mat1_v4 / k_mat1_lx / k_mat1_b0, launch shape tuned per target)",
+                "kernel": "one-qubit U3 pass (k_mat1_v4 / k_mat1_fronts / k_mat1_lx / k_mat1_b0, launch shape tuned per target)",
                 "avg_launch_ms": round(mat1_avg_ms, 4),
                 "algorithmic_bytes_per_launch": mat1_bytes}
 

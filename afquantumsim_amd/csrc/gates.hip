@@ -430,7 +430,10 @@ __global__ __launch_bounds__(BS) void k_phase_v4(PhaseArgs<float4, float> a) {
 // per amplitude.  With bit 3 required, the 64-byte half line without it is
 // neither read nor written (CZ on bits 2,3 at 28 qubits: 645 -> 486 us; on
 // bits 3,4: 344 -> 291 us).  Skipping 32- or 16-byte pieces instead was
-// slower (960 us for bits 0,2): HBM moves 64-byte granules.
+// slower (960 us for bits 0,2): HBM moves 64-byte granules.  For the same
+// reason every loaded float4 is stored back, changed or not: storing only
+// the changed ones (partial granules) took 685 against 486 us on bits 2,3
+// and 972 against 638 us on bits 0,1.
 struct PhaseLineArgs {
     float4* sv;
     uint64_t units;
